@@ -1,0 +1,138 @@
+/*
+ * ptn_hip.h -- C ABI of libptnhip.so, the MI355X (gfx950) kernels behind the
+ * PyTreeNet time-evolution hot path.
+ *
+ * Conventions
+ *   - Every buffer is caller-owned device memory; pointers are plain device
+ *     addresses, sizes are element counts unless named *_bytes.
+ *   - Complex data is interleaved (re, im): dtype PTN_C128 = 2 x double,
+ *     PTN_C64 = 2 x float.
+ *   - Every call is ordered on the given HIP stream (hipStream_t passed as
+ *     void*, NULL = default stream) and returns 0 on success, a negative
+ *     PTN_E* code on failure.  ptn_last_error() gives a message.
+ *   - The library keeps no mutable global state except the rocSOLVER handle
+ *     (one per device, created lazily under a mutex).
+ *
+ * Each entry point names the reference (PyTreeNet, pure Python/NumPy) function
+ * whose arithmetic it replaces.  The reference has no FFI: its boundary is the
+ * Python call surface, which pytreenet_amd/ mirrors and which reaches these
+ * symbols through ctypes (pytreenet_amd/_lib.py).
+ */
+#ifndef PTN_HIP_H
+#define PTN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PTN_C128 = 0, PTN_C64 = 1 };
+enum { PTN_COMPUTE_F64_4M = 0, PTN_COMPUTE_F32_3M = 1 };
+enum {
+  PTN_OK = 0,
+  PTN_EINVAL = -1,     /* bad argument / shape mismatch                       */
+  PTN_EHIP = -2,       /* HIP runtime error                                    */
+  PTN_ESOLVER = -3,    /* rocSOLVER/rocBLAS error                              */
+  PTN_ENOCONV = -4,    /* decomposition did not converge                       */
+  PTN_ERANGE = -5      /* size beyond a documented limit                       */
+};
+
+/* One pairwise tensor contraction executed as a complex GEMM whose operands
+ * are gathered through per-mode offset tables:
+ *
+ *   C[c_off_m[m] + c_off_n[n]] (beta_one ? += : =)
+ *       alpha * sum_k opA(A[a_off_m[m] + a_off_k[k]]) * opB(B[b_off_k[k] + b_off_n[n]])
+ *
+ * opX is the identity or complex conjugation (x_conj).  Offsets are in
+ * elements of the operand's dtype and live in device memory.  x_kmajor selects
+ * the coalesced load orientation (1: consecutive k are contiguous; 0:
+ * consecutive m (A) / n (B) are contiguous).  The leg permutations of the
+ * contraction are thereby fused into the LDS-staged tile loads; no transposed
+ * copy is written to HBM.
+ *
+ * Replaces every np.tensordot on the hot path:
+ *   contractions/state_operator_contraction.py:227-273 (contract_ket_ham_with_envs),
+ *   :407-694 (contract_any / contract_leaf / contract_subtrees_using_dictionary),
+ *   :984-1040 (contract_bond_tensor), contractions/contraction_util.py:82-290,
+ *   contractions/node_contraction.py:12-141 (contract_nodes), and
+ *   contractions/state_state_contraction.py:29-66 (contract_two_ttns).
+ */
+typedef struct ptn_gemm_desc {
+  int64_t M, N, K;
+  const void *A;
+  const int64_t *a_off_m, *a_off_k;
+  int32_t a_dtype, a_conj, a_kmajor, _pad0;
+  const void *B;
+  const int64_t *b_off_k, *b_off_n;
+  int32_t b_dtype, b_conj, b_kmajor, _pad1;
+  void *C;
+  const int64_t *c_off_m, *c_off_n;
+  int32_t c_dtype, beta_one;
+  double alpha_re, alpha_im;
+  int32_t compute, _pad2;
+} ptn_gemm_desc;
+
+int ptn_gemm_gather(void *stream, const ptn_gemm_desc *desc);
+
+/* Strided copy with optional conjugation and dtype conversion:
+ *   dst[sum_i idx_i * dst_strides[i]] = op(src[sum_i idx_i * src_strides[i]])
+ * over the box dims[0..rank-1] (rank <= 8).  Strides are in elements.
+ * Replaces np.transpose/reshape copies of util/tensor_util.py:44-81
+ * (tensor_matricization) and the KEEP zero-padding of
+ * util/tensor_splitting.py:98-106.                                          */
+int ptn_permute(void *stream, int32_t rank, const int64_t *dims,
+                const void *src, const int64_t *src_strides, int32_t src_dtype,
+                void *dst, const int64_t *dst_strides, int32_t dst_dtype,
+                int32_t conj);
+
+/* Fill a complex128 / complex64 buffer with a constant. */
+int ptn_fill(void *stream, void *dst, int64_t n, int32_t dtype, double re, double im);
+
+/* Krylov (Arnoldi) building blocks; all vectors complex128, contiguous.
+ * Replaces the re-orthogonalisation and norms of krylov_exp_action,
+ * time_evolution/time_evolution.py:721-821.
+ *
+ * ptn_krylov_orthogonalize: with Q = [q_0..q_{nvec-1}] (row stride ldq),
+ *   w0sq        = ||w||^2 before projection,
+ *   h1 = Q^H w;  w -= Q h1;  h2 = Q^H w;  w -= Q h2   (two classical passes),
+ *   beta_sq     = ||w||^2 after projection.
+ * Writes out[0..nvec) = h1 + h2 (complex128), out[nvec] = (w0sq, beta_sq).
+ * `work` must hold at least ptn_krylov_work_bytes(n, nvec) bytes.           */
+int64_t ptn_krylov_work_bytes(int64_t n, int32_t nvec);
+int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, int64_t ldq,
+                             int32_t nvec, void *w, void *out, void *work);
+
+/* out = sum_i coeffs[i] * Q[i]   (coeffs: device complex128[nvec]);
+ * also used for q_{j+1} = w / beta with nvec = 1.                           */
+int ptn_krylov_combine(void *stream, int64_t n, const void *Q, int64_t ldq,
+                       int32_t nvec, const void *coeffs, void *out);
+
+/* ||x||^2 of a complex128 vector into out[0] (double, device).              */
+int ptn_norm_sq(void *stream, int64_t n, const void *x, void *out, void *work);
+
+/* <x|y> = sum conj(x) y into out (complex128, device).                      */
+int ptn_dotc(void *stream, int64_t n, const void *x, const void *y, void *out,
+             void *work);
+
+/* Reduced QR of a column-major m x n complex128 matrix A (lda = m):
+ * on return the first k = min(m,n) columns of A hold Q (column-major, lda = m)
+ * and R (k x n, column-major, ldr = k) is written.  tau: k complex128.
+ * Replaces np.linalg.qr in tensor_qr_decomposition,
+ * util/tensor_splitting.py:60-107.                                          */
+int ptn_qr(void *stream, int64_t m, int64_t n, void *A, void *R, void *tau);
+
+/* Thin SVD of a column-major m x n complex128 matrix A (destroyed):
+ * S (double, k = min(m,n), descending), U (m x k col-major), Vh (k x n
+ * col-major).  Replaces np.linalg.svd / scipy gesvd in tensor_svd,
+ * util/tensor_splitting.py:122-177.                                         */
+int ptn_svd(void *stream, int64_t m, int64_t n, void *A, void *S, void *U,
+            void *Vh, void *E);
+
+const char *ptn_last_error(void);
+int ptn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTN_HIP_H */

@@ -1,0 +1,338 @@
+// Gather-GEMM: one pairwise complex tensor contraction on CDNA4 MFMA.
+//
+// C[c_off_m[m] + c_off_n[n]] = alpha * sum_k opA(A[..]) * opB(B[..]) (+ C)
+//
+// The tensor legs of A, B and C are flattened into (M, K), (K, N), (M, N)
+// index spaces whose element offsets come from per-mode offset tables, so any
+// leg permutation of a tensordot is absorbed by the global->LDS tile loads and
+// the epilogue stores.  No transposed copy of an operand is written to HBM.
+//
+// Tiling: 256 threads = 4 wave64s in a 2x2 grid, block tile 64x64 (complex),
+// K step 16, two LDS stages.  Each wave owns a 32x32 sub-tile = 2x2 MFMA
+// blocks of 16x16.
+//   F64_4M: v_mfma_f64_16x16x4_f64, complex product as 4 real MFMAs into
+//           (re, im) accumulators.
+//   F32_3M: v_mfma_f32_16x16x4_f32 with the 3-multiply (Gauss) split:
+//           T1 = ar*br, T2 = ai*bi, T3 = (ar+ai)(br+bi); re = T1-T2,
+//           im = T3-T1-T2; operands are converted to float while staging
+//           (complex128 inputs are read as double and rounded once).
+// MFMA operand layout (16x16x4): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register r of lane l is D[4*(l/16)+r][l%16].
+// Block -> tile mapping is XCD-aware: the hardware deals workgroups to the 8
+// XCDs round-robin, the remap gives each XCD a contiguous tile range so tiles
+// sharing an A row panel share one L2.
+#include "ptn_common.h"
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+
+struct KArgs {
+  int64_t M, N, K;
+  const void *A;
+  const int64_t *aom, *aok;
+  const void *B;
+  const int64_t *bok, *bon;
+  void *C;
+  const int64_t *com, *con;
+  double are, aim;
+  int64_t tiles_n;
+  int aconj, akmaj, bconj, bkmaj, beta_one, remap;
+};
+
+template <int MODE>
+struct Acc;
+template <>
+struct Acc<PTN_COMPUTE_F64_4M> {
+  typedef double T;
+  typedef v4d V;
+};
+template <>
+struct Acc<PTN_COMPUTE_F32_3M> {
+  typedef float T;
+  typedef v4f V;
+};
+
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int MODE, typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
+  typedef typename Acc<MODE>::T T;
+  typedef typename Acc<MODE>::V V;
+  __shared__ T sAr[2][BK][BM];
+  __shared__ T sAi[2][BK][BM];
+  __shared__ T sBr[2][BK][BN];
+  __shared__ T sBi[2][BK][BN];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  int64_t bid = blockIdx.x;
+  if (p.remap) {
+    const int64_t per = gridDim.x >> 3;
+    bid = (bid & 7) * per + (bid >> 3);
+  }
+  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const TA *A = static_cast<const TA *>(p.A);
+  const TB *B = static_cast<const TB *>(p.B);
+
+  // ---- per-thread load coordinates ----------------------------------------
+  // A: m-major -> (ml = tid%64, kl = tid/64 + 4i); k-major -> (kl = tid%16, ml = tid/16 + 16i)
+  int64_t a_moff[4];
+  bool a_mval[4];
+  int a_kl[4], a_ml[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (p.akmaj) {
+      a_kl[i] = tid & 15;
+      a_ml[i] = (tid >> 4) + 16 * i;
+    } else {
+      a_kl[i] = (tid >> 6) + 4 * i;
+      a_ml[i] = tid & 63;
+    }
+    const int64_t gm = m0 + a_ml[i];
+    a_mval[i] = gm < p.M;
+    a_moff[i] = a_mval[i] ? p.aom[gm] : 0;
+  }
+  int64_t b_noff[4];
+  bool b_nval[4];
+  int b_kl[4], b_nl[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (p.bkmaj) {
+      b_kl[i] = tid & 15;
+      b_nl[i] = (tid >> 4) + 16 * i;
+    } else {
+      b_kl[i] = (tid >> 6) + 4 * i;
+      b_nl[i] = tid & 63;
+    }
+    const int64_t gn = n0 + b_nl[i];
+    b_nval[i] = gn < p.N;
+    b_noff[i] = b_nval[i] ? p.bon[gn] : 0;
+  }
+
+  T ra_r[4], ra_i[4], rb_r[4], rb_i[4];
+  const T asg = p.aconj ? T(-1) : T(1);
+  const T bsg = p.bconj ? T(-1) : T(1);
+
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t gk = k0 + a_kl[i];
+      T re = 0, im = 0;
+      if (a_mval[i] && gk < p.K) ptn::cload(A + a_moff[i] + p.aok[gk], re, im);
+      ra_r[i] = re;
+      ra_i[i] = asg * im;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t gk = k0 + b_kl[i];
+      T re = 0, im = 0;
+      if (b_nval[i] && gk < p.K) ptn::cload(B + p.bok[gk] + b_noff[i], re, im);
+      rb_r[i] = re;
+      rb_i[i] = bsg * im;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      sAr[buf][a_kl[i]][a_ml[i]] = ra_r[i];
+      sAi[buf][a_kl[i]][a_ml[i]] = ra_i[i];
+      sBr[buf][b_kl[i]][b_nl[i]] = rb_r[i];
+      sBi[buf][b_kl[i]][b_nl[i]] = rb_i[i];
+    }
+  };
+
+  V c0[2][2], c1[2][2], c2[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      c0[i][j] = V{0, 0, 0, 0};
+      c1[i][j] = V{0, 0, 0, 0};
+      c2[i][j] = V{0, 0, 0, 0};
+    }
+
+  const int64_t KT = (p.K + BK - 1) / BK;
+  if (KT > 0) {
+    gload(0);
+    sstore(0);
+  }
+  __syncthreads();
+  const int fr = lane & 15, fk = lane >> 4;
+  for (int64_t kt = 0; kt < KT; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < KT) gload((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int kr = kk * 4 + fk;
+      T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = sAr[buf][kr][wm * 32 + i * 16 + fr];
+        ai[i] = sAi[buf][kr][wm * 32 + i * 16 + fr];
+        br[i] = sBr[buf][kr][wn * 32 + i * 16 + fr];
+        bi[i] = sBi[buf][kr][wn * 32 + i * 16 + fr];
+      }
+      if constexpr (MODE == PTN_COMPUTE_F64_4M) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            c0[i][j] = mfma(ar[i], br[j], c0[i][j]);
+            c1[i][j] = mfma(ar[i], bi[j], c1[i][j]);
+            c0[i][j] = mfma(-ai[i], bi[j], c0[i][j]);
+            c1[i][j] = mfma(ai[i], br[j], c1[i][j]);
+          }
+      } else {
+        T as[2], bs[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          as[i] = ar[i] + ai[i];
+          bs[i] = br[i] + bi[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            c0[i][j] = mfma(ar[i], br[j], c0[i][j]);
+            c1[i][j] = mfma(ai[i], bi[j], c1[i][j]);
+            c2[i][j] = mfma(as[i], bs[j], c2[i][j]);
+          }
+      }
+    }
+    if (kt + 1 < KT) sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // ---- epilogue -------------------------------------------------------------
+  TC *C = static_cast<TC *>(p.C);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t gn = n0 + wn * 32 + j * 16 + fr;
+    if (gn >= p.N) continue;
+    const int64_t noff = p.con[gn];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm * 32 + i * 16 + fk * 4 + r;
+        if (gm >= p.M) continue;
+        double re, im;
+        if constexpr (MODE == PTN_COMPUTE_F64_4M) {
+          re = c0[i][j][r];
+          im = c1[i][j][r];
+        } else {
+          const float t1 = c0[i][j][r], t2 = c1[i][j][r], t3 = c2[i][j][r];
+          re = (double)(t1 - t2);
+          im = (double)(t3 - t1 - t2);
+        }
+        double ore = p.are * re - p.aim * im;
+        double oim = p.are * im + p.aim * re;
+        TC *dst = C + p.com[gm] + noff;
+        if (p.beta_one) {
+          double xr, xi;
+          ptn::cload(dst, xr, xi);
+          ore += xr;
+          oim += xi;
+        }
+        ptn::cstore(dst, ore, oim);
+      }
+    }
+  }
+}
+
+template <int MODE, typename TA, typename TB, typename TC>
+int launch(hipStream_t s, const KArgs &a, int64_t tiles) {
+  hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC>), dim3((unsigned)tiles),
+                     dim3(NT), 0, s, a);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+template <int MODE, typename TA, typename TB>
+int dispatch_c(hipStream_t s, const KArgs &a, int64_t tiles, int cdt) {
+  if (cdt == PTN_C128) return launch<MODE, TA, TB, double2>(s, a, tiles);
+  return launch<MODE, TA, TB, float2>(s, a, tiles);
+}
+
+template <int MODE, typename TA>
+int dispatch_b(hipStream_t s, const KArgs &a, int64_t tiles, int bdt, int cdt) {
+  if (bdt == PTN_C128) return dispatch_c<MODE, TA, double2>(s, a, tiles, cdt);
+  return dispatch_c<MODE, TA, float2>(s, a, tiles, cdt);
+}
+
+}  // namespace
+
+extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
+  if (!d) {
+    ptn::set_error("ptn_gemm_gather: null descriptor");
+    return PTN_EINVAL;
+  }
+  if (d->M < 0 || d->N < 0 || d->K < 0) {
+    ptn::set_error("ptn_gemm_gather: negative extent");
+    return PTN_EINVAL;
+  }
+  if (d->M == 0 || d->N == 0) return PTN_OK;
+  const bool dt_ok = (unsigned)d->a_dtype <= 1 && (unsigned)d->b_dtype <= 1 &&
+                     (unsigned)d->c_dtype <= 1;
+  if (!dt_ok || (d->compute != PTN_COMPUTE_F64_4M && d->compute != PTN_COMPUTE_F32_3M)) {
+    ptn::set_error("ptn_gemm_gather: bad dtype/compute code");
+    return PTN_EINVAL;
+  }
+  if (d->compute == PTN_COMPUTE_F64_4M &&
+      (d->a_dtype != PTN_C128 || d->b_dtype != PTN_C128 || d->c_dtype != PTN_C128)) {
+    ptn::set_error("ptn_gemm_gather: F64_4M requires complex128 operands");
+    return PTN_EINVAL;
+  }
+  if (!d->A || !d->B || !d->C || !d->a_off_m || !d->b_off_n || !d->c_off_m || !d->c_off_n ||
+      (d->K > 0 && (!d->a_off_k || !d->b_off_k))) {
+    ptn::set_error("ptn_gemm_gather: null pointer");
+    return PTN_EINVAL;
+  }
+  KArgs a;
+  a.M = d->M;
+  a.N = d->N;
+  a.K = d->K;
+  a.A = d->A;
+  a.aom = d->a_off_m;
+  a.aok = d->a_off_k;
+  a.B = d->B;
+  a.bok = d->b_off_k;
+  a.bon = d->b_off_n;
+  a.C = d->C;
+  a.com = d->c_off_m;
+  a.con = d->c_off_n;
+  a.are = d->alpha_re;
+  a.aim = d->alpha_im;
+  a.aconj = d->a_conj;
+  a.akmaj = d->a_kmajor;
+  a.bconj = d->b_conj;
+  a.bkmaj = d->b_kmajor;
+  a.beta_one = d->beta_one;
+  const int64_t tiles_m = (d->M + BM - 1) / BM;
+  a.tiles_n = (d->N + BN - 1) / BN;
+  const int64_t tiles = tiles_m * a.tiles_n;
+  if (tiles > 0x7fffffffLL) {
+    ptn::set_error("ptn_gemm_gather: too many tiles");
+    return PTN_ERANGE;
+  }
+  a.remap = (tiles % 8 == 0) ? 1 : 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (d->compute == PTN_COMPUTE_F64_4M)
+    return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles);
+  if (d->a_dtype == PTN_C128)
+    return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype);
+  return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype);
+}

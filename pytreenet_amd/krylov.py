@@ -1,0 +1,232 @@
+"""Matrix-free Krylov exponential on the device, and the time-evolution enums.
+
+``krylov_exp_action`` keeps the reference algorithm and stop rules
+(``time_evolution/time_evolution.py:696-821``): Arnoldi with two
+re-orthogonalisation passes accumulated into one Hessenberg column, happy
+breakdown at beta <= max(eps*n, 1e-12*||A q_j||), the Saad estimate
+||psi|| * beta * |[exp(f dt T_m) e_1]_{m-1}| < krylov_tol, and the small
+propagator from eigh((T+T^H)/2) (hermitian) or expm(T).
+
+MI355X form: the basis lives in one device buffer; each step's two projection
+passes run as block classical Gram-Schmidt (CGS2 -- two passes of Q^H w /
+w -= Q h, each streaming the basis once) inside ``ptn_krylov_orthogonalize``,
+which also returns ||w||^2 before and after.  One device->host copy of the
+(j+2) scalars per step feeds the host-side m x m propagator (m <= max dim,
+FP64), as in the reference.  The two-pass CGS and the reference's two-pass MGS
+differ only in rounding.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from enum import Enum
+from typing import Any, Callable, Dict, Optional, Union
+
+import numpy as np
+import torch
+from scipy.linalg import expm
+
+from . import _lib
+from . import engine as E
+
+
+class EvoDirection(Enum):
+    """FORWARD evolves with exp(-i dt H), BACKWARD with exp(+i dt H)."""
+    FORWARD = -1
+    BACKWARD = 1
+
+    @staticmethod
+    def from_bool(forward: bool) -> "EvoDirection":
+        return EvoDirection.FORWARD if forward else EvoDirection.BACKWARD
+
+    def exp_sign(self) -> int:
+        return self.value
+
+    def exp_factor(self) -> complex:
+        return 1.0j * self.exp_sign()
+
+
+class TimeEvoMode(Enum):
+    """Solver selection (``time_evolution.py:431-655``).  KRYLOV is the device path."""
+    FASTEST = "fastest"
+    EXPM = "expm"
+    CHEBYSHEV = "chebyshev"
+    SPARSE = "sparse"
+    RK45 = "RK45"
+    RK23 = "RK23"
+    DOP853 = "DOP853"
+    BDF = "BDF"
+    KRYLOV = "krylov"
+
+    def __str__(self):
+        return self.value
+
+    @staticmethod
+    def default_solver_options(mode: "TimeEvoMode") -> Dict[str, Any]:
+        if mode is TimeEvoMode.KRYLOV:
+            return {"krylov_tol": 1e-10}
+        return {}
+
+    @staticmethod
+    def valid_solver_options(mode: "TimeEvoMode") -> set:
+        if mode is TimeEvoMode.KRYLOV:
+            return {"krylov_tol", "max_krylov_dim", "hermitian", "exp_factor"}
+        return {"exp_factor"}
+
+    def action_evolvable(self) -> bool:
+        return self is TimeEvoMode.KRYLOV
+
+    def time_evolve_action(self, psi: torch.Tensor, time_evo_action: Callable,
+                           time_difference: float,
+                           forward: Union[EvoDirection, bool] = EvoDirection.FORWARD,
+                           **options) -> torch.Tensor:
+        if isinstance(forward, bool):
+            forward = EvoDirection.from_bool(forward)
+        override = options.pop("exp_factor", None)
+        exp_factor = (forward.exp_factor() if override is None
+                      else (override if forward is EvoDirection.FORWARD else -override))
+        hermitian = options.pop("hermitian", True)
+        if self is not TimeEvoMode.KRYLOV:
+            raise NotImplementedError(
+                f"mode {self.value}: the device path evolves matrix-free with "
+                "TimeEvoMode.KRYLOV only")
+        return krylov_exp_action(psi, lambda y: time_evo_action(0.0, y), time_difference,
+                                 exp_factor, krylov_tol=options.pop("krylov_tol", 1e-10),
+                                 max_krylov_dim=options.pop("max_krylov_dim", None),
+                                 hermitian=hermitian)
+
+
+# ---------------------------------------------------------------------------------------
+# device level-1 helpers
+# ---------------------------------------------------------------------------------------
+_WORK: Dict[torch.device, torch.Tensor] = {}
+
+
+def _work(device) -> torch.Tensor:
+    w = _WORK.get(device)
+    if w is None:
+        nbytes = int(_lib.load().ptn_krylov_work_bytes(0, 1))
+        w = torch.empty(nbytes // 16 + 1, dtype=torch.complex128, device=device)
+        _WORK[device] = w
+    return w
+
+
+def _vp(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _flat(t: torch.Tensor) -> torch.Tensor:
+    if t.dtype != torch.complex128:
+        raise TypeError("Krylov vectors are complex128")
+    return t.reshape(-1) if t.is_contiguous() else t.contiguous().reshape(-1)
+
+
+def dotc(x: torch.Tensor, y: torch.Tensor) -> complex:
+    """sum conj(x) * y (complex128 device vectors)."""
+    E.require_device(x, y)
+    xf, yf = _flat(x), _flat(y)
+    out = torch.empty(1, dtype=torch.complex128, device=x.device)
+    _lib.check(_lib.load().ptn_dotc(ctypes.c_void_p(E.stream_handle()), xf.numel(), _vp(xf),
+                                    _vp(yf), _vp(out), _vp(_work(x.device))), "ptn_dotc")
+    return complex(out.cpu().item())
+
+
+def norm_sq(x: torch.Tensor) -> float:
+    E.require_device(x)
+    xf = _flat(x)
+    out = torch.empty(1, dtype=torch.complex128, device=x.device)
+    _lib.check(_lib.load().ptn_norm_sq(ctypes.c_void_p(E.stream_handle()), xf.numel(),
+                                       _vp(xf), _vp(out), _vp(_work(x.device))),
+               "ptn_norm_sq")
+    return float(out.cpu().item().real)
+
+
+def combine(Q: torch.Tensor, coeffs, out: torch.Tensor, nvec: Optional[int] = None):
+    """out = sum_i coeffs[i] Q[i] over the first nvec rows of Q."""
+    nvec = len(coeffs) if nvec is None else nvec
+    c = torch.as_tensor(np.asarray(coeffs, dtype=np.complex128)).to(Q.device)
+    _lib.check(_lib.load().ptn_krylov_combine(ctypes.c_void_p(E.stream_handle()),
+                                              Q.shape[1], _vp(Q), Q.stride(0), nvec, _vp(c),
+                                              _vp(out)), "ptn_krylov_combine")
+    return out
+
+
+def small_propagator(t_mat: np.ndarray, dt: float, exp_factor: complex,
+                     hermitian: bool = True) -> np.ndarray:
+    """First column of exp(exp_factor * dt * T) for the m x m projection T (host FP64)."""
+    if hermitian:
+        herm = 0.5 * (t_mat + t_mat.conj().T)
+        evals, vecs = np.linalg.eigh(herm)
+        return vecs @ (np.exp(exp_factor * dt * evals) * vecs[0].conj())
+    return expm(exp_factor * dt * t_mat)[:, 0]
+
+
+class KrylovStats:
+    def __init__(self):
+        self.calls = 0
+        self.applies = 0
+        self.dims = []
+
+    def record(self, m: int):
+        self.calls += 1
+        self.applies += m
+        self.dims.append(m)
+
+
+STATS = KrylovStats()
+
+
+def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.Tensor],
+                      time_difference: float, exp_factor: complex,
+                      krylov_tol: float = 1e-10, max_krylov_dim: Optional[int] = None,
+                      hermitian: bool = True) -> torch.Tensor:
+    """exp(exp_factor * dt * A) psi with A given by ``action`` (device tensors)."""
+    E.require_device(psi)
+    shape = psi.shape
+    dev = psi.device
+    v0 = _flat(psi.to(torch.complex128))
+    n = v0.numel()
+    beta0 = math.sqrt(norm_sq(v0))
+    if beta0 == 0.0:
+        return psi.clone()
+    max_steps = n if max_krylov_dim is None else min(n, int(max_krylov_dim))
+    cap = min(max_steps + 1, 16)
+    Q = torch.empty((cap, n), dtype=torch.complex128, device=dev)
+    combine(v0.view(1, n), [1.0 / beta0], Q[0])
+    hess = np.zeros((max_steps + 1, max_steps + 1), dtype=complex)
+    breakdown = np.finfo(float).eps * max(n, 1)
+    packed = torch.empty(max_steps + 2, dtype=torch.complex128, device=dev)
+    lib = _lib.load()
+    work = _work(dev)
+    m = 0
+    for j in range(max_steps):
+        w = action(Q[j].view(shape))
+        w = _flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
+        if w.data_ptr() == Q[j].data_ptr():
+            w = w.clone()
+        _lib.check(lib.ptn_krylov_orthogonalize(ctypes.c_void_p(E.stream_handle()), n, _vp(Q),
+                                                Q.stride(0), j + 1, _vp(w), _vp(packed),
+                                                _vp(work)), "ptn_krylov_orthogonalize")
+        host = packed[: j + 2].cpu().numpy()
+        hess[: j + 1, j] = host[: j + 1]
+        w0_norm = math.sqrt(max(host[j + 1].real, 0.0))
+        beta = math.sqrt(max(host[j + 1].imag, 0.0))
+        hess[j + 1, j] = beta
+        m = j + 1
+        if beta <= max(breakdown, 1e-12 * w0_norm):
+            break
+        if krylov_tol > 0.0:
+            col = small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
+            if beta0 * beta * abs(col[m - 1]) < krylov_tol:
+                break
+        if j + 1 >= Q.shape[0]:
+            grown = torch.empty((min(2 * Q.shape[0], max_steps + 1), n),
+                                dtype=torch.complex128, device=dev)
+            grown[: Q.shape[0]].copy_(Q)
+            Q = grown
+        combine(w.view(1, n), [1.0 / beta], Q[j + 1])
+    coeffs = beta0 * small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
+    out = torch.empty(n, dtype=torch.complex128, device=dev)
+    combine(Q, coeffs, out, nvec=m)
+    STATS.record(m)
+    return out.view(shape)

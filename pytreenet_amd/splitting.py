@@ -1,0 +1,272 @@
+"""QR / SVD node splits with the reference truncation rules.
+
+Drop-in counterparts of ``util/tensor_splitting.py``: ``SplitMode`` (``:23-58``),
+``tensor_qr_decomposition`` (``:60-107``), ``tensor_svd`` (``:122-177``),
+``SVDParameters`` (``:253-328``), ``renormalise_singular_values`` (``:330-352``),
+``sum_truncation`` (``:354-395``), ``value_truncation`` (``:397-418``),
+``truncate_singular_values`` (``:420-486``), ``truncated_tensor_svd``
+(``:488-527``) and ``contr_truncated_svd_splitting`` (``:541-572``).
+
+Device side: the matricisation is one strided copy into a column-major panel
+(``ptn_permute``), the factorisation runs in FP64 on the device
+(``ptn_qr`` / ``ptn_svd``), and Q, U, S*Vh are written straight into their
+final leg layout by strided copies or one gather-GEMM.  Only the singular
+values travel to the host, where the truncation rules (tiny vectors, exact
+reference semantics) pick the kept dimension.
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+from dataclasses import dataclass
+from enum import Enum
+from numbers import Integral
+from typing import List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import engine as E
+
+
+class SplitMode(Enum):
+    FULL = "full"
+    REDUCED = "reduced"
+    KEEP = "keep"
+
+
+class ContractionMode(Enum):
+    UCONTR = "ucontr"
+    VCONTR = "vcontr"
+    EQUAL = "equal"
+
+
+@dataclass
+class SVDParameters:
+    max_bond_dim: int = 100
+    rel_tol: float = 1e-15
+    total_tol: float = 1e-15
+    renorm: bool = False
+    sum_trunc: bool = False
+    sum_renorm: bool = False
+    random: bool = False
+
+    def __post_init__(self):
+        self.check_truncation_parameters()
+
+    def check_truncation_parameters(self):
+        mbd = self.max_bond_dim
+        if not isinstance(mbd, Integral) and mbd != float("inf"):
+            raise TypeError(f"'max_bond_dim' has to be an integer or inf, not {type(mbd)}!")
+        if mbd <= 0:
+            raise ValueError("'max_bond_dim' has to be positive.")
+        if self.rel_tol < 0 and self.rel_tol != float("-inf"):
+            raise ValueError("'rel_tol' has to be positive or -inf.")
+        if self.total_tol < 0 and self.total_tol != float("-inf"):
+            raise ValueError("'total_tol' has to be positive or -inf.")
+        if self.random:
+            raise ValueError("randomised SVD is not part of the device path")
+
+
+# ---------------------------------------------------------------------------------------
+# truncation rules (host, on the singular-value vector)
+# ---------------------------------------------------------------------------------------
+def renormalise_singular_values(s: np.ndarray, new_s: np.ndarray) -> np.ndarray:
+    return new_s * np.linalg.norm(s) / np.linalg.norm(new_s)
+
+
+def _sum_truncation_index(s: np.ndarray, total_tol: float, norming: bool = True) -> int:
+    normsq = float(np.linalg.norm(s)) ** 2
+    if normsq == 0:
+        return 0
+    thresh = total_tol ** 2
+    acc = 0.0
+    for i, sv in enumerate(reversed(s)):
+        acc += float(sv) ** 2
+        if (acc / normsq if norming else acc) > thresh:
+            return len(s) - i
+    return 0
+
+
+def sum_truncation(s: np.ndarray, total_tol: float, norming: bool = True) -> np.ndarray:
+    return s[: _sum_truncation_index(s, total_tol, norming=norming)]
+
+
+def value_truncation(s: np.ndarray, total_tol: float, rel_tol: float) -> np.ndarray:
+    return s[s > max(rel_tol * s[0], total_tol)]
+
+
+def truncate_singular_values(s: np.ndarray, svd_params) -> Tuple[np.ndarray, np.ndarray]:
+    if len(s) == 0:
+        raise ValueError("No singular values to truncate!")
+    if getattr(svd_params, "sum_trunc", False):
+        kept = sum_truncation(s, svd_params.total_tol, getattr(svd_params, "sum_renorm", False))
+    else:
+        kept = value_truncation(s, svd_params.total_tol, svd_params.rel_tol)
+    mbd = svd_params.max_bond_dim
+    if len(kept) > mbd:
+        new_s, s_trunc = kept[: int(mbd)], s[int(mbd):]
+    elif len(kept) == 0:
+        warnings.warn("All singular values were truncated. Returning only the largest "
+                      "singular value.")
+        new_s, s_trunc = np.array([s[0]]), s[1:]
+    else:
+        new_s, s_trunc = kept, s[len(kept):]
+    if getattr(svd_params, "renorm", False):
+        new_s = renormalise_singular_values(s, new_s)
+    return new_s, s_trunc
+
+
+# ---------------------------------------------------------------------------------------
+# device factorisations
+# ---------------------------------------------------------------------------------------
+def _cstrides(dims: Sequence[int]) -> List[int]:
+    out, acc = [], 1
+    for d in reversed(list(dims)):
+        out.append(acc)
+        acc *= d
+    return list(reversed(out))
+
+
+def _colmajor_panel(t: torch.Tensor, row_legs: Sequence[int], col_legs: Sequence[int]):
+    """Column-major (m x n) copy: row-major (n, m) = t permuted to (col_legs, row_legs)."""
+    return E.permute_copy(t, list(col_legs) + list(row_legs), dtype=torch.complex128)
+
+
+def qr_colmajor(panel: torch.Tensor, m: int, n: int) -> Tuple[torch.Tensor, torch.Tensor, int]:
+    """In-place reduced QR of a column-major m x n panel. Returns (panel, R_cm, k)."""
+    k = min(m, n)
+    R = torch.empty(k * n, dtype=torch.complex128, device=panel.device)
+    tau = torch.empty(k, dtype=torch.complex128, device=panel.device)
+    _lib.check(_lib.load().ptn_qr(ctypes.c_void_p(E.stream_handle()), m, n,
+                                  ctypes.c_void_p(panel.data_ptr()),
+                                  ctypes.c_void_p(R.data_ptr()),
+                                  ctypes.c_void_p(tau.data_ptr())), "ptn_qr")
+    return panel, R, k
+
+
+def tensor_qr_decomposition(tensor: torch.Tensor, q_legs: Sequence[int], r_legs: Sequence[int],
+                            mode: SplitMode = SplitMode.REDUCED
+                            ) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Q legs (q_legs..., new), R legs (new, r_legs...); KEEP zero-pads to the R dims."""
+    E.require_device(tensor)
+    if mode is SplitMode.FULL:
+        raise NotImplementedError("SplitMode.FULL is not part of the device path")
+    q_legs, r_legs = list(q_legs), list(r_legs)
+    qdims = [tensor.shape[i] for i in q_legs]
+    rdims = [tensor.shape[i] for i in r_legs]
+    m, n = int(np.prod(qdims)), int(np.prod(rdims))
+    panel = _colmajor_panel(tensor, q_legs, r_legs)
+    panel, R, k = qr_colmajor(panel, m, n)
+    kk = n if (mode is SplitMode.KEEP and k < n) else k
+    Q = torch.empty(qdims + [kk], dtype=torch.complex128, device=tensor.device)
+    if kk > k:
+        E.fill_(Q, 0.0)
+    # panel rows 0..k-1 of the row-major (n, m) buffer are Q's columns
+    E.strided_copy([k] + qdims, panel, [m] + _cstrides(qdims), Q,
+                   [Q.stride(-1)] + list(Q.stride()[:-1]))
+    Rt = torch.empty([kk] + rdims, dtype=torch.complex128, device=tensor.device)
+    if kk > k:
+        E.fill_(Rt, 0.0)
+    # R_cm element (i, j) at j*k + i
+    E.strided_copy([k] + rdims, R, [1] + [k * s for s in _cstrides(rdims)], Rt,
+                   list(Rt.stride()))
+    return Q, Rt
+
+
+def qr_split_to_neighbour(ttn, node_id: str, neighbour_id: str):
+    """QR(KEEP) of a node off its leg to ``neighbour_id``.
+
+    Returns (Q in the node's canonical layout with the new bond at the old
+    leg's position, R with legs (new, old)).
+    """
+    t = ttn.tensors[node_id]
+    leg = ttn.nodes[node_id].neighbour_index(neighbour_id)
+    others = [i for i in range(t.dim()) if i != leg]
+    q, r = tensor_qr_decomposition(t, others, [leg], mode=SplitMode.KEEP)
+    perm = list(range(q.dim() - 1))
+    perm.insert(leg, q.dim() - 1)
+    return E.permute_copy(q, perm), r
+
+
+def absorb_into_neighbour(ttn, node_id: str, from_id: str, r: torch.Tensor):
+    """node[.., new, ..] = sum_old r[new, old] node[.., old, ..] on the leg to ``from_id``."""
+    t = ttn.tensors[node_id]
+    leg = ttn.nodes[node_id].neighbour_index(from_id)
+    lt = list(range(t.dim()))
+    lt[leg] = "old"
+    lo = list(range(t.dim()))
+    lo[leg] = "new"
+    ttn.tensors[node_id] = E.contract(r, ["new", "old"], t, lt, lo)
+
+
+def svd_colmajor(panel: torch.Tensor, m: int, n: int):
+    """Thin SVD of a column-major m x n panel (destroyed): (s host, U_cm, Vh_cm, k)."""
+    k = min(m, n)
+    dev = panel.device
+    S = torch.empty(k, dtype=torch.float64, device=dev)
+    U = torch.empty(m * k, dtype=torch.complex128, device=dev)
+    Vh = torch.empty(k * n, dtype=torch.complex128, device=dev)
+    Ew = torch.empty(k + 2, dtype=torch.float64, device=dev)
+    _lib.check(_lib.load().ptn_svd(ctypes.c_void_p(E.stream_handle()), m, n,
+                                   ctypes.c_void_p(panel.data_ptr()),
+                                   ctypes.c_void_p(S.data_ptr()),
+                                   ctypes.c_void_p(U.data_ptr()),
+                                   ctypes.c_void_p(Vh.data_ptr()),
+                                   ctypes.c_void_p(Ew.data_ptr())), "ptn_svd")
+    tail = Ew[k:].cpu()
+    info = int(np.frombuffer(tail.numpy().tobytes(), dtype=np.int32)[0])
+    if info != 0:
+        raise np.linalg.LinAlgError(f"device SVD did not converge (info={info})")
+    return S.cpu().numpy(), U, Vh, k
+
+
+def tensor_svd(tensor: torch.Tensor, u_legs: Sequence[int], v_legs: Sequence[int]):
+    """(U (u_legs..., k), s (host), Vh (k, v_legs...)) -- thin SVD."""
+    E.require_device(tensor)
+    u_legs, v_legs = list(u_legs), list(v_legs)
+    udims = [tensor.shape[i] for i in u_legs]
+    vdims = [tensor.shape[i] for i in v_legs]
+    m, n = int(np.prod(udims)), int(np.prod(vdims))
+    panel = _colmajor_panel(tensor, u_legs, v_legs)
+    s, U, Vh, k = svd_colmajor(panel, m, n)
+    Ut = torch.empty(udims + [k], dtype=torch.complex128, device=tensor.device)
+    E.strided_copy([k] + udims, U, [m] + _cstrides(udims), Ut,
+                   [Ut.stride(-1)] + list(Ut.stride()[:-1]))
+    Vt = torch.empty([k] + vdims, dtype=torch.complex128, device=tensor.device)
+    E.strided_copy([k] + vdims, Vh, [1] + [k * s_ for s_ in _cstrides(vdims)], Vt,
+                   list(Vt.stride()))
+    return Ut, s, Vt
+
+
+def truncated_tensor_svd(tensor: torch.Tensor, u_legs: Sequence[int], v_legs: Sequence[int],
+                         svd_params: SVDParameters):
+    u, s, vh = tensor_svd(tensor, u_legs, v_legs)
+    s_new, _ = truncate_singular_values(s, svd_params)
+    r = len(s_new)
+    return u[..., :r].contiguous(), np.asarray(s_new), vh[:r].contiguous()
+
+
+def contr_truncated_svd_splitting(tensor: torch.Tensor, u_legs: Sequence[int],
+                                  v_legs: Sequence[int],
+                                  contr_mode: ContractionMode = ContractionMode.VCONTR,
+                                  svd_params: SVDParameters = SVDParameters()):
+    u, s, vh = truncated_tensor_svd(tensor, u_legs, v_legs, svd_params)
+    dev = tensor.device
+    if contr_mode == ContractionMode.VCONTR:
+        return u, scale_leg(vh, 0, s)
+    if contr_mode == ContractionMode.UCONTR:
+        return scale_leg(u, u.dim() - 1, s), vh
+    root = np.sqrt(s)
+    return scale_leg(u, u.dim() - 1, root), scale_leg(vh, 0, root)
+
+
+def scale_leg(t: torch.Tensor, leg: int, s: np.ndarray) -> torch.Tensor:
+    """t with leg ``leg`` scaled by the vector s (diag(s) contracted on that leg)."""
+    d = torch.as_tensor(np.diag(np.asarray(s, dtype=np.complex128))).to(t.device)
+    lt = list(range(t.dim()))
+    lt[leg] = "old"
+    lo = list(range(t.dim()))
+    lo[leg] = "new"
+    return E.contract(d, ["new", "old"], t, lt, lo)

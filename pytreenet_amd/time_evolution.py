@@ -1,0 +1,555 @@
+"""TDVP time evolution on the device: the reference's drop-in call surface.
+
+Classes and methods keep the reference names, arguments and sweep schedules:
+
+* ``TimeEvolution`` run loop and step count -- ``time_evolution/time_evolution.py:31-368``
+* ``TTNTimeEvolutionConfig`` / operator evaluation -- ``ttn_time_evolution.py:23-279``
+* ``TDVPConfig`` / ``TDVPAlgorithm`` -- ``tdvp_algorithms/tdvp_algorithm.py:24-221``
+* ``OneSiteTDVP`` -- ``tdvp_algorithms/onesitetdvp.py:17-134``
+* ``FirstOrderOneSiteTDVP`` -- ``tdvp_algorithms/firstorderonesite.py:7-104``
+* ``SecondOrderOneSiteTDVP`` -- ``tdvp_algorithms/secondorderonesite.py:11-193``
+* ``TwoSiteTDVPConfig`` / ``TwoSiteTDVP`` -- ``tdvp_algorithms/twositetdvp.py:18-146``
+* ``SecondOrderTwoSiteTDVP`` -- ``tdvp_algorithms/secondordertwosite.py:11-193``
+* update / orthogonalisation paths -- ``time_evo_util/update_path.py:63-420``
+
+All tensors stay on the device.  The transient link tensor of a one-site
+update is the R factor held as a (node side, next side) matrix rather than a
+node inserted into the tree; the two-site tensor has legs
+(outer legs of the first node, outer legs of the second, phys1, phys2).
+``TDVPConfig.precision`` selects FP64 (complex128, 4M FP64 MFMA) or the mixed
+form (complex64 environments, 3M FP32 MFMA contractions; Krylov
+re-orthogonalisation, the small propagator, QR and SVD stay FP64).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional, Union
+
+import numpy as np
+import torch
+
+from . import engine as E
+from .contractions import (SandwichCache, bond_apply, contract_any,
+                           contract_ket_ham_with_envs, expectation_value, scalar_product)
+from .krylov import EvoDirection, TimeEvoMode, krylov_exp_action
+from .splitting import (SVDParameters, SplitMode, _colmajor_panel, _cstrides, qr_colmajor,
+                        svd_colmajor, truncate_singular_values)
+from .ttn import TreeTensorNetworkState
+from .ttno import TensorProduct, TreeTensorNetworkOperator
+
+
+# ---------------------------------------------------------------------------------------
+# configs and results
+# ---------------------------------------------------------------------------------------
+@dataclass
+class TimeEvoConfig:
+    time_dep: bool = False
+
+
+@dataclass
+class TTNTimeEvolutionConfig(TimeEvoConfig):
+    record_bond_dim: bool = False
+    record_max_bdim: bool = False
+    record_average_bdim: bool = False
+    record_total_size: bool = False
+    record_norm: bool = False
+    record_loschmidt_amplitude: bool = False
+
+
+class PathFinderMode:
+    LeafToRoot = "LeafToRoot"
+
+
+@dataclass
+class TDVPConfig(TTNTimeEvolutionConfig):
+    time_evo_mode: TimeEvoMode = TimeEvoMode.FASTEST
+    main_path_mode: str = PathFinderMode.LeafToRoot
+    precision: str = E.FP64
+
+
+@dataclass
+class TwoSiteTDVPConfig(TDVPConfig, SVDParameters):
+    pass
+
+
+class Results:
+    """Time series of operator results (``time_evolution/results.py``), kept on the host."""
+
+    def __init__(self):
+        self._times: List[float] = []
+        self._data: Dict[str, List[Any]] = {}
+
+    def initialize(self, keys, length: int):
+        self._times = [0.0] * length
+        self._data = {k: [None] * length for k in keys}
+
+    def set_element(self, key: str, index: int, value):
+        self._data.setdefault(key, [None] * len(self._times))[index] = value
+
+    def set_time(self, index: int, t: float):
+        self._times[index] = t
+
+    def times(self) -> np.ndarray:
+        return np.asarray(self._times, dtype=float)
+
+    def operator_result(self, key: str, realise: bool = False) -> np.ndarray:
+        arr = np.asarray(self._data[key])
+        return arr.real if realise else arr
+
+    def operator_results(self, realise: bool = False) -> Dict[str, np.ndarray]:
+        return {k: self.operator_result(k, realise) for k in self._data}
+
+    def save_npz(self, path: str):
+        np.savez(path, times=self.times(),
+                 **{f"op_{k}": np.asarray(v) for k, v in self._data.items()})
+
+
+# ---------------------------------------------------------------------------------------
+# base classes
+# ---------------------------------------------------------------------------------------
+class TimeEvolution:
+    config_class = TimeEvoConfig
+
+    def __init__(self, initial_state, time_step_size: float, final_time: float, operators,
+                 solver_options: Optional[Dict[str, Any]] = None, config=None):
+        self._initial_state = initial_state.copy()
+        self.state = initial_state.copy()
+        if time_step_size <= 0:
+            raise ValueError("size of one time step has to be positive")
+        if final_time <= 0:
+            raise ValueError("final time has to be positive")
+        self._time_step_size = time_step_size
+        self._final_time = final_time
+        self._num_time_steps = self._compute_num_time_steps()
+        self.operators = self.init_operators(operators)
+        self.config = self.config_class() if config is None else config
+        self.solver_options = {} if solver_options is None else dict(solver_options)
+        self.results = Results()
+
+    def _compute_num_time_steps(self) -> int:
+        decimal, integer = math.modf(self._final_time / self._time_step_size)
+        return int(integer) if decimal < 0.1 else int(integer + 1)
+
+    @staticmethod
+    def init_operators(operators) -> Dict[str, Any]:
+        if isinstance(operators, dict) and not isinstance(operators, TensorProduct):
+            return dict(operators)
+        if isinstance(operators, (list, tuple)):
+            return {str(i): op for i, op in enumerate(operators)}
+        return {"0": operators}
+
+    @property
+    def time_step_size(self) -> float:
+        return self._time_step_size
+
+    @property
+    def num_time_steps(self) -> int:
+        return self._num_time_steps
+
+    @property
+    def final_time(self) -> float:
+        return self._final_time
+
+    def run_one_time_step(self, **kwargs):
+        raise NotImplementedError
+
+    def evaluate_operator(self, operator) -> complex:
+        raise NotImplementedError
+
+    def evaluate_operators(self, index: int):
+        for key, op in self.operators.items():
+            self.results.set_element(key, index, self.evaluate_operator(op))
+
+    def run(self, evaluation_time: Union[int, str] = 1, pgbar: bool = False):
+        n = self._num_time_steps
+        if evaluation_time == "inf":
+            idx = {n: 0}
+            length = 1
+        else:
+            idx = {i: i // evaluation_time for i in range(0, n + 1, evaluation_time)}
+            length = len(idx)
+        self.results.initialize(list(self.operators.keys()), length)
+        for i in range(n + 1):
+            if i != 0:
+                self.run_one_time_step()
+            if i in idx:
+                self.evaluate_operators(idx[i])
+                self.results.set_time(idx[i], i * self._time_step_size)
+
+
+class TTNOBasedTimeEvolution(TimeEvolution):
+    config_class = TTNTimeEvolutionConfig
+
+    def __init__(self, initial_state, hamiltonian, time_step_size, final_time, operators,
+                 config=None, solver_options=None):
+        super().__init__(initial_state, time_step_size, final_time, operators,
+                         solver_options=solver_options, config=config)
+        self.hamiltonian = hamiltonian
+
+    def evaluate_operator(self, operator) -> complex:
+        return self.state.operator_expectation_value(operator)
+
+    def evaluate_operators(self, index: int):
+        super().evaluate_operators(index)
+        cfg = self.config
+        if getattr(cfg, "record_max_bdim", False):
+            self.results.set_element("max_bond_dim", index, self.state.max_bond_dim())
+        if getattr(cfg, "record_norm", False):
+            self.results.set_element("norm", index, self.state.norm())
+        if getattr(cfg, "record_loschmidt_amplitude", False):
+            self.results.set_element("loschmidt_amplitude", index,
+                                     scalar_product(self.state, self._initial_state))
+
+
+# ---------------------------------------------------------------------------------------
+# path finding (update_path.py:63-254, 400-420)
+# ---------------------------------------------------------------------------------------
+def _branch_rec(ttn, nid: str) -> List[str]:
+    node = ttn.nodes[nid]
+    if node.is_leaf():
+        return [nid]
+    out: List[str] = []
+    for c in node.children:
+        out.extend(_branch_rec(ttn, c))
+    out.append(nid)
+    return out
+
+
+def find_update_path_leaf_to_root(ttn) -> List[str]:
+    root = ttn.root_id
+    dist = ttn.distance_to_node(root)
+    start = max(dist, key=dist.get)
+    main_path = ttn.find_path_to_root(start)
+    path: List[str] = []
+    for origin in main_path:
+        if origin != root:
+            for c in ttn.nodes[origin].children:
+                if c not in main_path:
+                    path.extend(_branch_rec(ttn, c))
+            path.append(origin)
+            continue
+        if ttn.nodes[root].has_x_children(1):
+            path.append(root)
+            continue
+        leaves = [l for l in ttn.get_leaves() if l not in path]
+        leaf_dist = {n: d for n, d in dist.items() if n in leaves}
+        final = max(leaf_dist, key=leaf_dist.get)
+        down = list(reversed(ttn.find_path_to_root(final)))
+        for o in down:
+            excluded = (main_path[-2], down[1]) if o == root else tuple(down)
+            for c in ttn.nodes[o].children:
+                if c not in excluded:
+                    path.extend(_branch_rec(ttn, c))
+            path.append(o)
+    return path
+
+
+def find_orthogonalisation_path(update_path: List[str], ttn,
+                                include_start: bool = False) -> List[List[str]]:
+    out = []
+    for i in range(len(update_path) - 1):
+        sub = ttn.path_from_to(update_path[i], update_path[i + 1])
+        out.append(sub if include_start else sub[1:])
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# TDVP
+# ---------------------------------------------------------------------------------------
+class TDVPAlgorithm(TTNOBasedTimeEvolution):
+    config_class = TDVPConfig
+
+    def __init__(self, initial_state: TreeTensorNetworkState,
+                 hamiltonian: TreeTensorNetworkOperator, time_step_size: float,
+                 final_time: float, operators, config: Optional[TDVPConfig] = None,
+                 solver_options: Optional[Dict[str, Any]] = None):
+        assert len(initial_state.nodes) == len(hamiltonian.nodes)
+        super().__init__(initial_state, hamiltonian, time_step_size, final_time, operators,
+                         config=config, solver_options=solver_options)
+        mode = self.config.time_evo_mode
+        if mode is TimeEvoMode.FASTEST:
+            mode = TimeEvoMode.KRYLOV
+        if mode is not TimeEvoMode.KRYLOV:
+            raise NotImplementedError("the device path evolves with TimeEvoMode.KRYLOV")
+        self._mode = mode
+        self.precision = getattr(self.config, "precision", E.FP64)
+        if self.precision not in (E.FP64, E.FP32):
+            raise ValueError(f"unknown precision {self.precision!r}")
+        self.update_path = self._finds_update_path()
+        self.orthogonalization_path = find_orthogonalisation_path(self.update_path, self.state)
+        self._orthogonalize_init()
+        self.partial_tree_cache = self._init_partial_tree_cache()
+
+    def _finds_update_path(self) -> List[str]:
+        return find_update_path_leaf_to_root(self.state)
+
+    def _orthogonalize_init(self, force_new: bool = False):
+        if self.state.orthogonality_center_id is None or force_new:
+            self.state.canonical_form(self.update_path[0], mode=SplitMode.KEEP)
+        else:
+            self.state.move_orthogonalization_center(self.update_path[0], mode=SplitMode.KEEP)
+
+    def _init_partial_tree_cache(self) -> SandwichCache:
+        return SandwichCache.init_cache_but_one(self.state, self.hamiltonian,
+                                                self.update_path[0], precision=self.precision)
+
+    def update_tree_cache(self, node_id: str, next_node_id: str):
+        self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
+
+    def _krylov(self, x: torch.Tensor, action, dt: float, direction: EvoDirection):
+        opts = dict(TimeEvoMode.default_solver_options(self._mode))
+        opts.update(self.solver_options)
+        return self._mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
+                                             **opts)
+
+    def _update_site(self, node_id: str, time_step_factor: float = 1):
+        node, tensor = self.state[node_id]
+        op_node = self.hamiltonian.nodes[node_id]
+        op = self.hamiltonian.tensor_in(node_id, E.storage_dtype(self.precision))
+        cache = self.partial_tree_cache
+
+        def action(y):
+            return contract_ket_ham_with_envs(node, y, op_node, op, cache,
+                                              precision=self.precision)
+
+        self.state.tensors[node_id] = self._krylov(tensor, action,
+                                                   self.time_step_size * time_step_factor,
+                                                   EvoDirection.FORWARD)
+
+    def _move_orth_and_update_cache_for_path(self, path: List[str]):
+        if len(path) == 0:
+            return
+        assert self.state.orthogonality_center_id == path[0]
+        for i, node_id in enumerate(path[1:]):
+            self.state.move_orthogonalization_center(node_id, mode=SplitMode.KEEP)
+            self.partial_tree_cache.update_tree_cache(path[i], node_id)
+
+
+class OneSiteTDVP(TDVPAlgorithm):
+    """Site + link updates; the link is the R factor of a QR(KEEP) split."""
+
+    def _update_link(self, node_id: str, next_node_id: str, time_step_factor: float = 1):
+        assert self.state.orthogonality_center_id == node_id
+        from .splitting import qr_split_to_neighbour, absorb_into_neighbour
+        q, r = qr_split_to_neighbour(self.state, node_id, next_node_id)
+        self.state.tensors[node_id] = q
+        self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
+        env_a = self.partial_tree_cache.get_entry(node_id, next_node_id)
+        env_b = self.partial_tree_cache.get_entry(next_node_id, node_id)
+        r = self._krylov(r, lambda y: bond_apply(y, env_a, env_b, precision=self.precision),
+                         self.time_step_size * time_step_factor, EvoDirection.BACKWARD)
+        absorb_into_neighbour(self.state, next_node_id, node_id, r)
+        self.state.orthogonality_center_id = next_node_id
+
+    @staticmethod
+    def create_link_id(node_id: str, next_node_id: str) -> str:
+        return "link_" + node_id + "_with_" + next_node_id
+
+
+class FirstOrderOneSiteTDVP(OneSiteTDVP):
+    def _update_site_and_link(self, node_id: str, update_index: int):
+        self._update_site(node_id)
+        self._update_link(node_id, self.orthogonalization_path[update_index][0])
+
+    def _reset_for_next_time_step(self):
+        self.state.move_orthogonalization_center(self.update_path[0], mode=SplitMode.KEEP)
+        self.partial_tree_cache = self._init_partial_tree_cache()
+
+    def run_one_time_step(self, **kwargs):
+        up, op = self.update_path, self.orthogonalization_path
+        for i, node_id in enumerate(up):
+            if i == len(up) - 1:
+                if len(op) > 0:
+                    self._move_orth_and_update_cache_for_path(op[-1])
+                self._update_site(node_id)
+                self._reset_for_next_time_step()
+            elif i == 0:
+                self._update_site_and_link(node_id, 0)
+            else:
+                self._move_orth_and_update_cache_for_path(op[i - 1])
+                self._update_site_and_link(node_id, i)
+
+
+class SecondOrderOneSiteTDVP(OneSiteTDVP):
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.backwards_update_path = list(reversed(self.update_path))
+        self.backwards_orth_path = self._init_second_order_orth_path()
+
+    def _init_second_order_orth_path(self) -> List[List[str]]:
+        bp = self.backwards_update_path
+        out = []
+        for i, node_id in enumerate(bp[1:-1]):
+            out.append(self.state.path_from_to(node_id, bp[i + 2])[:-1])
+        out.append([bp[-1]])
+        return out
+
+    def forward_sweep(self):
+        for i, node_id in enumerate(self.update_path[:-1]):
+            if i > 0:
+                self._move_orth_and_update_cache_for_path(self.orthogonalization_path[i - 1])
+            nxt = self.orthogonalization_path[i][0]
+            self._update_site(node_id, time_step_factor=0.5)
+            self._update_link(node_id, nxt, time_step_factor=0.5)
+
+    def _final_forward_update(self):
+        node_id = self.update_path[-1]
+        assert self.state.orthogonality_center_id == node_id
+        self._update_site(node_id)
+
+    def backward_sweep(self):
+        bp = self.backwards_update_path
+        self._update_link(self.state.orthogonality_center_id, bp[1], time_step_factor=0.5)
+        for i, node_id in enumerate(bp[1:-1]):
+            idx = i + 1
+            assert self.state.orthogonality_center_id == node_id
+            self._update_site(node_id, time_step_factor=0.5)
+            self._move_orth_and_update_cache_for_path(self.backwards_orth_path[idx - 1])
+            self._update_link(self.state.orthogonality_center_id, bp[idx + 1],
+                              time_step_factor=0.5)
+        self._update_site(bp[-1], time_step_factor=0.5)
+
+    def run_one_time_step(self, **kwargs):
+        self.forward_sweep()
+        self._final_forward_update()
+        self.backward_sweep()
+
+
+class TwoSiteTDVP(TDVPAlgorithm):
+    config_class = TwoSiteTDVPConfig
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        if self.state.num_nodes() < 2:
+            raise ValueError("Two-site TDVP cannot be used for systems with less than 2 sites!")
+        self.kept_dims: List[int] = []
+
+    def _two_site_action(self, n1: str, n2: str):
+        """H_eff on the merged tensor with legs (outer1..., outer2..., p1, p2)."""
+        s, h, cache = self.state, self.hamiltonian, self.partial_tree_cache
+        dt = E.storage_dtype(self.precision)
+        node1, node2 = s.nodes[n1], s.nodes[n2]
+        op1n, op2n = h.nodes[n1], h.nodes[n2]
+        nb1 = [n for n in node1.neighbouring_nodes() if n != n2]
+        nb2 = [n for n in node2.neighbouring_nodes() if n != n1]
+        k1, k2 = len(nb1), len(nb2)
+        # operator labels: per operator leg, the matching ket-side label
+        def op_lab(op_node, own_nbs, offset, other, phys_out, phys_in):
+            lab = [None] * (op_node.nneighbours() + 2)
+            for i, nb in enumerate(own_nbs):
+                lab[op_node.neighbour_index(nb)] = ("w", offset + i)
+            lab[op_node.neighbour_index(other)] = "wl"
+            lab[-2], lab[-1] = phys_out, phys_in
+            return lab
+        tensors = [None, h.tensor_in(n1, dt), h.tensor_in(n2, dt)]
+        labels = [[("a", i) for i in range(k1 + k2)] + ["p1", "p2"],
+                  op_lab(op1n, nb1, 0, n2, "q1", "p1"),
+                  op_lab(op2n, nb2, k1, n1, "q2", "p2")]
+        for i, nb in enumerate(nb1):
+            tensors.append(cache.get_entry(nb, n1))
+            labels.append([("a", i), ("w", i), ("b", i)])
+        for i, nb in enumerate(nb2):
+            tensors.append(cache.get_entry(nb, n2))
+            labels.append([("a", k1 + i), ("w", k1 + i), ("b", k1 + i)])
+        out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
+
+        def action(y):
+            tensors[0] = y
+            return E.contract_network(tensors, labels, out, precision=self.precision)
+
+        return action, k1, k2
+
+    def _update_two_site_nodes(self, target_node_id: str, next_node_id: str,
+                               time_step_factor: float = 1):
+        s = self.state
+        n1, n2 = target_node_id, next_node_id
+        assert s.orthogonality_center_id == n1
+        i12 = s.nodes[n1].neighbour_index(n2)
+        i21 = s.nodes[n2].neighbour_index(n1)
+        A1, A2 = s.tensors[n1], s.tensors[n2]
+        k1, k2 = A1.dim() - 2, A2.dim() - 2
+        l1 = [("o1", i) for i in range(k1 + 1)] + ["p1"]
+        l1[i12] = "bond"
+        l1 = [x for x in l1]
+        l2 = [("o2", i) for i in range(k2 + 1)] + ["p2"]
+        l2[i21] = "bond"
+        outer1 = [x for x in l1[:-1] if x != "bond"]
+        outer2 = [x for x in l2[:-1] if x != "bond"]
+        x = E.contract(A1, l1, A2, l2, outer1 + outer2 + ["p1", "p2"])
+        action, _, _ = self._two_site_action(n1, n2)
+        x = self._krylov(x, action, self.time_step_size * time_step_factor,
+                         EvoDirection.FORWARD)
+        # truncated SVD, S absorbed into V (VCONTR)
+        dims = list(x.shape)
+        udims = dims[:k1] + [dims[k1 + k2]]
+        vdims = dims[k1:k1 + k2] + [dims[k1 + k2 + 1]]
+        u_legs = list(range(k1)) + [k1 + k2]
+        v_legs = list(range(k1, k1 + k2)) + [k1 + k2 + 1]
+        m, n = int(np.prod(udims)), int(np.prod(vdims))
+        panel = _colmajor_panel(x, u_legs, v_legs)
+        svals, U, Vh, kfull = svd_colmajor(panel, m, n)
+        s_new, _ = truncate_singular_values(svals, self.config)
+        r = len(s_new)
+        self.kept_dims.append(r)
+        dev = x.device
+        # U (m x kfull col-major) -> node 1 layout with the new bond at i12
+        shape1 = [A1.shape[i] for i in range(A1.dim())]
+        shape1[i12] = r
+        new1 = torch.empty(shape1, dtype=torch.complex128, device=dev)
+        st1 = list(new1.stride())
+        dst_u = [st1[i12]] + [st1[j] for j in range(A1.dim()) if j != i12]
+        E.strided_copy([r] + udims, U, [m] + _cstrides(udims), new1, dst_u)
+        # S Vh (kfull x n col-major) -> node 2 layout with the new bond at i21
+        vh_view = torch.as_strided(Vh, [r] + vdims, [1] + [kfull * c for c in _cstrides(vdims)])
+        sdiag = torch.as_tensor(np.diag(np.asarray(s_new, dtype=np.complex128))).to(dev)
+        lv = ["old"] + [("v", i) for i in range(len(vdims))]
+        lo = [("v", i) for i in range(len(vdims) - 1)]
+        lo.insert(i21, "new")
+        lo.append(("v", len(vdims) - 1))
+        new2 = E.contract(sdiag, ["new", "old"], vh_view, lv, lo)
+        s.tensors[n1] = new1
+        s.tensors[n2] = new2
+        s.orthogonality_center_id = n2
+        self.update_tree_cache(n1, n2)
+
+    def _single_site_backwards_update(self, node_id: str, time_step_factor: float = 1):
+        self._update_site(node_id, -1 * time_step_factor)
+
+    @staticmethod
+    def create_two_site_id(node_id: str, next_node_id: str) -> str:
+        return "TwoSite_" + node_id + "_contr_" + next_node_id
+
+
+class SecondOrderTwoSiteTDVP(TwoSiteTDVP):
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.backwards_update_path = list(reversed(self.update_path))
+        self.backwards_orth_path = [list(reversed(p))
+                                    for p in reversed(self.orthogonalization_path)]
+
+    def forward_sweep(self):
+        up, op = self.update_path, self.orthogonalization_path
+        for i, node_id in enumerate(up[:-2]):
+            if i > 0:
+                self._move_orth_and_update_cache_for_path(op[i - 1])
+            nxt = op[i][0]
+            self._update_two_site_nodes(node_id, nxt, time_step_factor=0.5)
+            self._single_site_backwards_update(nxt, time_step_factor=0.5)
+        self._update_two_site_nodes(up[-2], up[-1], time_step_factor=0.5)
+
+    def backwards_sweep(self):
+        bp, bo = self.backwards_update_path, self.backwards_orth_path
+        for i in range(len(bp) - 1):
+            if i == 0:
+                self._update_two_site_nodes(bp[0], bp[1], time_step_factor=0.5)
+            else:
+                path = bo[i]
+                self._move_orth_and_update_cache_for_path(path)
+                target = path[-1]
+                self._single_site_backwards_update(target, time_step_factor=0.5)
+                self._update_two_site_nodes(target, bp[i + 1], time_step_factor=0.5)
+
+    def run_one_time_step(self, **kwargs):
+        self.forward_sweep()
+        self.backwards_sweep()

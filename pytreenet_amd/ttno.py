@@ -1,0 +1,206 @@
+"""Tree tensor network operators and the Hamiltonians of the benchmark models.
+
+``TreeTensorNetworkOperator`` mirrors ``ttno/ttno_class.py:35-138``: one tensor
+per node with legs (parent, children..., out, in); ``out`` meets the bra and
+``in`` the ket (``contractions/state_operator_contraction.py:1042-1069``).
+
+The reference builds TTNOs from symbolic Hamiltonians through a state diagram
+and symbolic Gaussian elimination (``ttno_class.py:59-138``), a host-side
+one-off construction outside the hot path.  Here ``from_edge_terms`` builds the
+operator directly for the Hamiltonians the time evolution uses: one-body terms
+on nodes and two-body terms on tree edges, with the finite-state construction
+
+    bond state 0   nothing placed in the subtree below the edge (identity),
+    bond state 1   one complete term placed in the subtree,
+    bond state 2+j operator of edge term j placed at the child, waiting for
+                   its partner at the parent (coefficient applied there),
+
+so an edge carrying t two-body terms has TTNO bond dimension 2 + t (3 for the
+transverse-field Ising model, 5 for XXZ/XYZ/Heisenberg), the dimension the
+reference's state diagram gives for these models.  Sign convention of the
+reference models (``operators/models/two_site_model.py:126-288``):
+H = -sum_e (Jx XX + Jy YY + Jz ZZ) - sum_i g_i Z_i.
+"""
+from __future__ import annotations
+
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .ttn import Node, TreeTensorNetwork, as_device_tensor
+
+PAULI_X = np.array([[0, 1], [1, 0]], dtype=complex)
+PAULI_Y = np.array([[0, -1j], [1j, 0]], dtype=complex)
+PAULI_Z = np.array([[1, 0], [0, -1]], dtype=complex)
+IDENTITY2 = np.eye(2, dtype=complex)
+
+
+def pauli_matrices() -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    return PAULI_X, PAULI_Y, PAULI_Z
+
+
+class TensorProduct(dict):
+    """{node_id: operator matrix} -- a product of single-site operators."""
+
+
+class TreeTensorNetworkOperator(TreeTensorNetwork):
+    """TTNO with legs (parent, children..., out, in) per node."""
+
+    def __init__(self):
+        super().__init__()
+        self._lowp: Dict[str, torch.Tensor] = {}
+
+    def tensor_in(self, node_id: str, dtype: torch.dtype) -> torch.Tensor:
+        """The node tensor in ``dtype`` (complex64 copies are cached)."""
+        t = self._tensors[node_id]
+        if dtype == t.dtype:
+            return t
+        hit = self._lowp.get(node_id)
+        if hit is None or hit.shape != t.shape:
+            hit = t.to(dtype)
+            self._lowp[node_id] = hit
+        return hit
+
+    @classmethod
+    def from_tensors(cls, structure: TreeTensorNetwork,
+                     tensors: Dict[str, np.ndarray]) -> "TreeTensorNetworkOperator":
+        """TTNO on ``structure``'s tree from canonical-layout tensors."""
+        op = cls()
+        dev = structure.device
+        for nid in _parents_first(structure):
+            snode = structure.nodes[nid]
+            n = Node(identifier=nid)
+            n.parent, n.children = snode.parent, list(snode.children)
+            t = as_device_tensor(tensors[nid], device=dev)
+            n.shape = tuple(t.shape)
+            op._nodes[nid] = n
+            op._tensors[nid] = t
+        op._root_id = structure.root_id
+        return op
+
+    @classmethod
+    def from_edge_terms(cls, structure: TreeTensorNetwork,
+                        site_terms: Optional[Dict[str, np.ndarray]] = None,
+                        edge_terms: Optional[Iterable[tuple]] = None,
+                        phys_dim: int = 2) -> "TreeTensorNetworkOperator":
+        """H = sum_i site_terms[i] + sum (coef * A_{n1} B_{n2}) over edge_terms.
+
+        ``edge_terms`` holds (n1, n2, coef, A, B) with n1, n2 adjacent in the tree.
+        """
+        return cls.from_tensors(structure, build_edge_ttno_tensors(structure, site_terms,
+                                                                   edge_terms, phys_dim))
+
+
+TTNO = TreeTensorNetworkOperator
+
+
+def _parents_first(structure: TreeTensorNetwork) -> List[str]:
+    order, stack = [], [structure.root_id]
+    while stack:
+        nid = stack.pop()
+        order.append(nid)
+        stack.extend(reversed(structure.nodes[nid].children))
+    return order
+
+
+def build_edge_ttno_tensors(structure: TreeTensorNetwork,
+                            site_terms: Optional[Dict[str, np.ndarray]],
+                            edge_terms: Optional[Iterable[tuple]],
+                            phys_dim: int = 2) -> Dict[str, np.ndarray]:
+    site_terms = dict(site_terms or {})
+    per_edge: Dict[Tuple[str, str], List[Tuple[complex, np.ndarray, np.ndarray]]] = {}
+    for n1, n2, coef, a, b in (edge_terms or []):
+        a = np.asarray(a, dtype=complex)
+        b = np.asarray(b, dtype=complex)
+        if structure.nodes[n1].parent == n2:
+            child, parent, op_c, op_p = n1, n2, a, b
+        elif structure.nodes[n2].parent == n1:
+            child, parent, op_c, op_p = n2, n1, b, a
+        else:
+            raise ValueError(f"{n1} and {n2} are not adjacent")
+        per_edge.setdefault((child, parent), []).append((complex(coef), op_c, op_p))
+    eye = np.eye(phys_dim, dtype=complex)
+    out: Dict[str, np.ndarray] = {}
+    for nid, node in structure.nodes.items():
+        kids = list(node.children)
+        has_parent = node.parent is not None
+        dims = ([2 + len(per_edge.get((nid, node.parent), []))] if has_parent else [])
+        dims += [2 + len(per_edge.get((c, nid), [])) for c in kids]
+        W = np.zeros(dims + [phys_dim, phys_dim], dtype=complex)
+        nk = len(kids)
+
+        def put(parent_state: Optional[int], child_states: Sequence[int], op: np.ndarray):
+            idx = ([parent_state] if has_parent else []) + list(child_states)
+            W[tuple(idx)] += op
+
+        zeros = [0] * nk
+        # parent state I: identity everywhere below
+        if has_parent:
+            put(0, zeros, eye)
+        # parent state F (or the root's only slice)
+        f_state = 1 if has_parent else None
+        for i in range(nk):
+            cs = list(zeros)
+            cs[i] = 1
+            put(f_state, cs, eye)
+            for j, (coef, _op_c, op_p) in enumerate(per_edge.get((kids[i], nid), [])):
+                cs = list(zeros)
+                cs[i] = 2 + j
+                put(f_state, cs, coef * op_p)
+        if nid in site_terms:
+            put(f_state, zeros, np.asarray(site_terms[nid], dtype=complex))
+        # parent state 2+j: this node holds the child-side operator of edge term j
+        if has_parent:
+            for j, (_coef, op_c, _op_p) in enumerate(per_edge.get((nid, node.parent), [])):
+                put(2 + j, zeros, op_c)
+        out[nid] = W
+    return out
+
+
+# ---- benchmark models (reference sign convention) -------------------------------------
+def _edges(structure: TreeTensorNetwork) -> List[Tuple[str, str]]:
+    return [(n.parent, nid) for nid, n in structure.nodes.items() if n.parent is not None]
+
+
+def transverse_field_ising(structure: TreeTensorNetwork, coupling=1.0, field=0.5,
+                           edge_couplings: Optional[Dict[Tuple[str, str], float]] = None
+                           ) -> TreeTensorNetworkOperator:
+    """H = -sum_e J_e X X - g sum_i Z_i."""
+    X, _, Z = pauli_matrices()
+    terms = []
+    for p, c in _edges(structure):
+        j = coupling if edge_couplings is None else edge_couplings[(p, c)]
+        terms.append((p, c, -j, X, X))
+    site = {nid: -field * Z for nid in structure.nodes}
+    return TreeTensorNetworkOperator.from_edge_terms(structure, site, terms)
+
+
+def xyz_model(structure: TreeTensorNetwork, couplings: Dict[Tuple[str, str], Sequence[float]],
+              fields: Optional[Dict[str, float]] = None) -> TreeTensorNetworkOperator:
+    """H = -sum_e (Jx_e XX + Jy_e YY + Jz_e ZZ) - sum_i h_i Z_i."""
+    X, Y, Z = pauli_matrices()
+    terms = []
+    for p, c in _edges(structure):
+        jx, jy, jz = couplings[(p, c)]
+        terms += [(p, c, -jx, X, X), (p, c, -jy, Y, Y), (p, c, -jz, Z, Z)]
+    site = {nid: -h * Z for nid, h in (fields or {}).items()}
+    return TreeTensorNetworkOperator.from_edge_terms(structure, site, terms)
+
+
+def xxz_model(structure: TreeTensorNetwork, jxy=1.0, jz=0.8, field=0.0):
+    return xyz_model(structure, {e: (jxy, jxy, jz) for e in _edges(structure)},
+                     {nid: field for nid in structure.nodes} if field else None)
+
+
+def random_xyz_model(structure: TreeTensorNetwork, seed: int) -> TreeTensorNetworkOperator:
+    """J_e^{x,y,z} ~ U[-1, 1] per edge and axis, edges in tree insertion order."""
+    rng = np.random.default_rng(seed)
+    return xyz_model(structure, {e: tuple(rng.uniform(-1, 1, 3)) for e in _edges(structure)})
+
+
+def random_field_heisenberg(structure: TreeTensorNetwork, disorder: float, seed: int):
+    """H = -sum_e (XX + YY + ZZ) - sum_i h_i Z_i, h_i ~ U[-W, W] in node insertion order."""
+    rng = np.random.default_rng(seed)
+    fields = {nid: float(rng.uniform(-disorder, disorder)) for nid in structure.nodes}
+    return xyz_model(structure, {e: (1.0, 1.0, 1.0) for e in _edges(structure)}, fields)

@@ -1,0 +1,164 @@
+"""GPU kernel tests through the C ABI: gather-GEMM, permute, Krylov level-1, QR, SVD."""
+import numpy as np
+import pytest
+import torch
+
+from pytreenet_amd import engine as E
+from pytreenet_amd import krylov as K
+from pytreenet_amd import splitting as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _crand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _dev(a, dtype=torch.complex128):
+    return torch.as_tensor(a, dtype=dtype).to("cuda")
+
+
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (16, 16, 4), (70, 50, 33), (129, 65, 300), (5, 200, 2)])
+def test_gemm_plain_fp64(gpu, mnk):
+    m, n, k = mnk
+    _say(f"gemm plain fp64 {mnk}")
+    rng = np.random.default_rng(m * 1000 + n)
+    a, b = _crand(rng, (m, k)), _crand(rng, (k, n))
+    c = E.contract(_dev(a), "mk", _dev(b), "kn", "mn").cpu().numpy()
+    ref = a @ b
+    assert np.abs(c - ref).max() <= 1e-13 * np.abs(ref).max() * max(1, np.sqrt(k))
+
+
+CASES = [
+    # (shapeA, labelsA, shapeB, labelsB, out)
+    ((6, 5, 7, 2), "abcp", (5, 3, 9), "bwx", "axcpw"),
+    ((6, 5, 7, 2), "abcp", (7, 4, 6), "cwa", "wbp"),
+    ((3, 4, 5), "ijk", (5, 4, 6), "kjl", "li"),
+    ((40, 3, 30), "xyz", (30, 3, 50), "zyq", "qx"),
+    ((2, 3), "ij", (4, 5), "kl", "likj"),            # outer product (K empty)
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[4] for c in CASES])
+@pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
+def test_gather_gemm_permutations_fp64(gpu, case, conj):
+    sa, la, sb, lb, lo = case
+    _say(f"gather fp64 {la},{lb}->{lo} conj={conj}")
+    rng = np.random.default_rng(len(lo) + 7 * conj[0] + 3 * conj[1])
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    got = E.contract(_dev(a), la, _dev(b), lb, lo, conj_a=conj[0], conj_b=conj[1]).cpu().numpy()
+    ref = np.einsum(f"{la},{lb}->{lo}", a.conj() if conj[0] else a, b.conj() if conj[1] else b)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_gather_gemm_views_accumulate_alpha(gpu):
+    _say("gather strided views + accumulate + alpha")
+    rng = np.random.default_rng(11)
+    a, b = _crand(rng, (8, 9, 10)), _crand(rng, (10, 8, 7))
+    ta = _dev(a).permute(2, 0, 1)          # non-contiguous view, labels "kij"
+    tb = _dev(b)[:, ::2, :]                # strided slice (10, 4, 7)
+    out = _dev(_crand(rng, (9, 4, 7)))
+    base = out.cpu().numpy().copy()
+    E.contract(ta, "kij", tb, "klm", "jlm", out=out, accumulate=True, alpha=0.5 - 2j)
+    ref = base + (0.5 - 2j) * np.einsum("ijk,klm->jlm", a, b[:, ::2, :])
+    np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-11)
+
+
+@pytest.mark.parametrize("dt", [torch.complex128, torch.complex64])
+def test_gather_gemm_fp32_3m(gpu, dt):
+    _say(f"gather fp32 3M inputs {dt}")
+    rng = np.random.default_rng(12)
+    a, b = _crand(rng, (50, 64, 3)), _crand(rng, (64, 3, 70))
+    got = E.contract(_dev(a, dt), "mkw", _dev(b, dt), "kwn", "nm", precision=E.FP32,
+                     out_dtype=torch.complex64).cpu().numpy()
+    ref = np.einsum("mkw,kwn->nm", a, b)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-6, err
+
+
+def test_permute_fill(gpu):
+    _say("permute/fill")
+    rng = np.random.default_rng(13)
+    a = _crand(rng, (3, 4, 5, 6))
+    got = E.permute_copy(_dev(a), [2, 0, 3, 1], conj=True).cpu().numpy()
+    np.testing.assert_array_equal(got, a.conj().transpose(2, 0, 3, 1))
+    low = E.permute_copy(_dev(a), [0, 1, 2, 3], dtype=torch.complex64).cpu().numpy()
+    np.testing.assert_allclose(low, a, rtol=1e-6)
+    z = torch.empty(1000, dtype=torch.complex128, device="cuda")
+    E.fill_(z, 1.5 - 0.5j)
+    assert torch.all(z == (1.5 - 0.5j))
+
+
+@pytest.mark.parametrize("n,nvec", [(1, 1), (37, 3), (100000, 9), (300001, 17)])
+def test_krylov_orthogonalize(gpu, n, nvec):
+    _say(f"krylov orth n={n} nvec={nvec}")
+    rng = np.random.default_rng(n + nvec)
+    nv = min(nvec, n)
+    q, _ = np.linalg.qr(_crand(rng, (n, nv)))
+    Q = np.ascontiguousarray(q.T)
+    w = _crand(rng, n)
+    tQ, tw = _dev(Q), _dev(w)
+    packed = torch.empty(nv + 1, dtype=torch.complex128, device="cuda")
+    import ctypes
+    from pytreenet_amd import _lib
+    _lib.check(_lib.load().ptn_krylov_orthogonalize(
+        ctypes.c_void_p(E.stream_handle()), n, ctypes.c_void_p(tQ.data_ptr()), n, nv,
+        ctypes.c_void_p(tw.data_ptr()), ctypes.c_void_p(packed.data_ptr()),
+        ctypes.c_void_p(K._work(tQ.device).data_ptr())), "orth")
+    h = packed.cpu().numpy()
+    h_ref = Q.conj() @ w
+    w1 = w - Q.T @ h_ref
+    h2 = Q.conj() @ w1
+    w2 = w1 - Q.T @ h2
+    np.testing.assert_allclose(h[:nv], h_ref + h2, atol=1e-10 * np.linalg.norm(w))
+    assert abs(h[nv].real - np.vdot(w, w).real) < 1e-10 * np.vdot(w, w).real
+    assert abs(h[nv].imag - np.vdot(w2, w2).real) < 1e-9 * np.vdot(w, w).real + 1e-300
+    np.testing.assert_allclose(tw.cpu().numpy(), w2, atol=1e-10 * np.linalg.norm(w))
+
+
+def test_dot_norm_combine(gpu):
+    _say("dot/norm/combine")
+    rng = np.random.default_rng(21)
+    x, y = _crand(rng, 5000), _crand(rng, 5000)
+    assert abs(K.dotc(_dev(x), _dev(y)) - np.vdot(x, y)) < 1e-10 * abs(np.vdot(x, y))
+    assert abs(K.norm_sq(_dev(x)) - np.vdot(x, x).real) < 1e-10 * np.vdot(x, x).real
+    Q = _crand(rng, (3, 5000))
+    c = np.array([1 - 1j, 0.5, 2j])
+    out = torch.empty(5000, dtype=torch.complex128, device="cuda")
+    K.combine(_dev(Q), c, out)
+    np.testing.assert_allclose(out.cpu().numpy(), c @ Q, atol=1e-12)
+
+
+@pytest.mark.parametrize("shape,q_legs,r_legs", [((4, 3, 5), [0, 1], [2]), ((2, 2, 8), [0, 1], [2]),
+                                                 ((6, 7, 2), [1, 2], [0])])
+def test_qr_keep(gpu, shape, q_legs, r_legs):
+    _say(f"qr keep {shape}")
+    rng = np.random.default_rng(31)
+    t = _crand(rng, shape)
+    q, r = S.tensor_qr_decomposition(_dev(t), q_legs, r_legs, mode=S.SplitMode.KEEP)
+    q, r = q.cpu().numpy(), r.cpu().numpy()
+    n = int(np.prod([shape[i] for i in r_legs]))
+    assert q.shape[-1] == n and r.shape[0] == n
+    rec = np.tensordot(q, r, axes=([q.ndim - 1], [0]))
+    perm = np.argsort(q_legs + r_legs)
+    np.testing.assert_allclose(rec.transpose(perm), t, atol=1e-12)
+    qm = q.reshape(-1, n)
+    g = qm.conj().T @ qm
+    k = min(qm.shape[0], n)
+    np.testing.assert_allclose(g, np.diag([1.0] * k + [0.0] * (n - k)), atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(3, 4, 5), (30, 2, 40), (64, 2, 2)])
+def test_svd(gpu, shape):
+    _say(f"svd {shape}")
+    rng = np.random.default_rng(41)
+    t = _crand(rng, shape)
+    u, s, vh = S.tensor_svd(_dev(t), [0, 2], [1])
+    mat = t.transpose(0, 2, 1).reshape(shape[0] * shape[2], shape[1])
+    np.testing.assert_allclose(s, np.linalg.svd(mat, compute_uv=False), rtol=1e-12)
+    rec = np.einsum("aek,k,kb->aeb", u.cpu().numpy(), s, vh.cpu().numpy()).transpose(0, 2, 1)
+    np.testing.assert_allclose(rec, t, atol=1e-11)

@@ -1,0 +1,178 @@
+"""Parity of the device path with the CPU oracle on the same seeded inputs.
+
+Bar (FP64): gauge-invariant results (energies, overlaps, <Z_i>) within 1e-12
+relative; kept bond dimensions identical.  Mixed precision: stated bound.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pytreenet_amd as ptn
+from oracle import contract as OC
+from oracle.tdvp import OracleState, OracleTDVP
+from pytreenet_amd import engine as E
+from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
+from pytreenet_amd.ttno import PAULI_Z, build_edge_ttno_tensors
+from tests.helpers import host_tensors, topology_from_ttn
+
+pytestmark = pytest.mark.gpu
+REL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _setup(branching, depth, chi, seed, model="xyz"):
+    ids, parent, children = complete_tree(branching, depth)
+    tensors = random_state_tensors(ids, parent, children, chi, seed)
+    st = build_state(ids, parent, children, tensors, device="cuda")
+    if model == "tfim":
+        H = ptn.transverse_field_ising(st, 1.0, 0.5)
+    elif model == "xxz":
+        H = ptn.xxz_model(st, 1.0, 0.8)
+    elif model == "heis":
+        H = ptn.random_field_heisenberg(st, 2.0, seed)
+    else:
+        H = ptn.random_xyz_model(st, seed)
+    topo = topology_from_ttn(st)
+    W = host_tensors(H)
+    return st, H, topo, tensors, W
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_env_heff_bond_match_oracle(gpu):
+    _say("env/heff/bond parity, ternary depth 2 chi 6")
+    st, H, topo, tensors, W = _setup(3, 2, 6, 3)
+    cache = ptn.PartialTreeCachDict()
+    oenv = {}
+    c = "site1"
+    dist = topo.distance_to_node(c)
+    for d in range(max(dist.values()), 0, -1):
+        for n in [m for m in dist if dist[m] == d]:
+            to = min(topo.neighbours(n), key=lambda x: dist[x])
+            cache.add_entry(n, to, ptn.contract_any(n, to, st, H, cache))
+            oenv[(n, to)] = OC.env_update(topo, n, to, tensors[n], W[n], oenv)
+            got = cache.get_entry(n, to).cpu().numpy()
+            np.testing.assert_allclose(got, oenv[(n, to)], rtol=0,
+                                       atol=REL * np.abs(oenv[(n, to)]).max())
+    node, x = st[c]
+    y = ptn.contract_ket_ham_with_envs(node, x, H.nodes[c], H.tensors[c], cache)
+    ref = OC.heff_apply(topo, c, tensors[c], W[c], oenv)
+    np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=0, atol=REL * np.abs(ref).max())
+    # bond action across (site1, site0): E[(site1, site0)] and E[(site0, site1)]
+    rng = np.random.default_rng(0)
+    dim = tensors["site1"].shape[0]
+    r = rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))
+    e_a = OC.env_update(topo, "site1", "site0", tensors["site1"], W["site1"], oenv)
+    got_a = ptn.contract_any("site1", "site0", st, H, cache).cpu().numpy()
+    np.testing.assert_allclose(got_a, e_a, rtol=0, atol=REL * np.abs(e_a).max())
+    e_b = oenv[("site0", "site1")]
+    yb = ptn.contractions.bond_apply(torch.as_tensor(r).cuda(), torch.as_tensor(e_a).cuda(),
+                                     torch.as_tensor(e_b).cuda()).cpu().numpy()
+    refb = OC.bond_apply(r, e_a, e_b)
+    np.testing.assert_allclose(yb, refb, rtol=0, atol=REL * np.abs(refb).max())
+
+
+def test_observables_match_oracle(gpu):
+    _say("observables parity, binary depth 3 chi 8")
+    st, H, topo, tensors, W = _setup(2, 3, 8, 5)
+    e = ptn.expectation_value(st, H)
+    e_ref = OC.expectation_value(topo, tensors, W)
+    assert _rel(e, e_ref) < REL
+    n2 = st.scalar_product(use_orthogonal_center=False)
+    assert _rel(n2, OC.overlap(topo, tensors, tensors)) < REL
+    for nid in ["site0", "site4", "site14"]:
+        z = st.single_site_operator_expectation_value(nid, PAULI_Z)
+        zr = OC.single_site_expectation(topo, tensors, nid, PAULI_Z)
+        assert abs(z - zr) < REL * abs(n2)
+
+
+def _run_pair(kind, branching, depth, chi, seed, model, dt, steps, krylov_tol=1e-10,
+              svd=None, precision="fp64"):
+    st, H, topo, tensors, W = _setup(branching, depth, chi, seed, model)
+    ops = {nid: ptn.TensorProduct({nid: PAULI_Z}) for nid in list(st.nodes)[:4]}
+    opts = {"krylov_tol": krylov_tol, "max_krylov_dim": 10}
+    if kind == "second2":
+        cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
+                                    **(svd or {}))
+        alg = ptn.SecondOrderTwoSiteTDVP(st, H, dt, dt * steps, ops, config=cfg,
+                                         solver_options=opts)
+    else:
+        cls = ptn.FirstOrderOneSiteTDVP if kind == "first1" else ptn.SecondOrderOneSiteTDVP
+        cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision)
+        alg = cls(st, H, dt, dt * steps, ops, config=cfg, solver_options=opts)
+    ost = OracleState(topo, tensors)
+    otd = OracleTDVP(ost, W, dt, kind=kind, krylov_tol=krylov_tol, max_krylov_dim=10,
+                     svd_params=svd)
+    return alg, otd, ops
+
+
+@pytest.mark.parametrize("kind,model,tol", [("second1", "xyz", 1e-10), ("second1", "tfim", 0.0),
+                                            ("first1", "xxz", 1e-10)])
+def test_tdvp_one_site_matches_oracle(gpu, kind, model, tol):
+    _say(f"tdvp {kind} {model} krylov_tol={tol}")
+    alg, otd, ops = _run_pair(kind, 2, 3, 8, 11, model, 0.05, 3, krylov_tol=tol)
+    for step in range(3):
+        alg.run_one_time_step()
+        otd.run_one_time_step()
+        e = ptn.expectation_value(alg.state, alg.hamiltonian)
+        e_ref = otd.energy()
+        assert _rel(e, e_ref) < REL, (step, e, e_ref)
+        for nid in ops:
+            z = alg.state.single_site_operator_expectation_value(nid, PAULI_Z)
+            assert abs(z - otd.expect_site(nid, PAULI_Z)) < REL * abs(e_ref), nid
+
+
+def test_tdvp_two_site_truncation_matches_oracle(gpu):
+    _say("tdvp second2 heisenberg truncation")
+    svd = dict(max_bond_dim=6, rel_tol=1e-10, total_tol=1e-10)
+    alg, otd, ops = _run_pair("second2", 2, 3, 4, 17, "heis", 0.05, 2, svd=svd)
+    otd_dims = []
+    for step in range(2):
+        alg.run_one_time_step()
+        n0 = len(otd.krylov_dims)
+        otd.run_one_time_step()
+        e = ptn.expectation_value(alg.state, alg.hamiltonian)
+        assert _rel(e, otd.energy()) < REL
+    dims = {k: v.shape for k, v in alg.state.tensors.items()}
+    odims = {k: v.shape for k, v in otd.s.A.items()}
+    assert dims == {k: tuple(v) for k, v in odims.items()}
+
+
+def test_run_loop_config1_tfim(gpu):
+    """Binary tree of 15 nodes, chi <= 16, TFIM J=1 g=0.5, dt=0.01, 20 steps, Krylov dim 10."""
+    _say("config C1 full run vs oracle")
+    st, H, topo, tensors, W = _setup(2, 3, 16, 2024, "tfim")
+    ops = {nid: ptn.TensorProduct({nid: PAULI_Z}) for nid in st.nodes}
+    ops["energy"] = H
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV)
+    alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.01, 0.2, ops, config=cfg,
+                                     solver_options={"max_krylov_dim": 10})
+    alg.run()
+    otd = OracleTDVP(OracleState(topo, tensors), W, 0.01, kind="second1", max_krylov_dim=10)
+    e_series = alg.results.operator_result("energy")
+    assert len(e_series) == 21
+    for i in range(21):
+        if i:
+            otd.run_one_time_step()
+        e_ref = otd.energy()
+        assert _rel(e_series[i], e_ref) < REL, i
+        for nid in ["site0", "site7", "site14"]:
+            assert abs(alg.results.operator_result(nid)[i] - otd.expect_site(nid, PAULI_Z)) \
+                < REL * abs(e_ref)
+
+
+def test_mixed_precision_error_bound(gpu):
+    _say("mixed precision fp32 3M vs fp64")
+    a64, _, _ = _run_pair("second1", 2, 3, 16, 5, "xyz", 0.05, 2)
+    a32, _, _ = _run_pair("second1", 2, 3, 16, 5, "xyz", 0.05, 2, precision="fp32")
+    for _ in range(2):
+        a64.run_one_time_step()
+        a32.run_one_time_step()
+    e64 = ptn.expectation_value(a64.state, a64.hamiltonian)
+    e32 = ptn.expectation_value(a32.state, a32.hamiltonian)
+    assert abs(e64 - e32) < 1e-4 * abs(e64)

@@ -1,0 +1,140 @@
+"""Host-side logic of the product (no GPU compute): paths, truncation rules, C ABI exports."""
+import ctypes
+import json
+import os
+import re
+import warnings
+
+import numpy as np
+import pytest
+
+import pytreenet_amd as ptn
+from pytreenet_amd.random_states import bond_dims, complete_tree
+from pytreenet_amd.time_evolution import (find_orthogonalisation_path,
+                                          find_update_path_leaf_to_root)
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _cpu_state(root, edges, chi=2):
+    st = ptn.TreeTensorNetworkState()
+    nchildren = {}
+    for c, p in edges:
+        nchildren[p] = nchildren.get(p, 0) + 1
+    def shape(nid, has_parent):
+        return ([chi] if has_parent else []) + [chi] * nchildren.get(nid, 0) + [2]
+    import torch
+    st.add_root(ptn.Node(identifier=root), torch.zeros(shape(root, False), dtype=torch.complex128))
+    for c, p in edges:
+        t = torch.zeros(shape(c, True), dtype=torch.complex128)
+        st.add_child_to_parent(ptn.Node(identifier=c), t, 0, p, st.nodes[p].nvirt_legs())
+    return st
+
+
+with open(os.path.join(GOLD, "paths.json")) as f:
+    CASES = json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c["name"])
+def test_product_paths_match_reference_kats(case):
+    st = _cpu_state(case["root"], case["edges"])
+    up = find_update_path_leaf_to_root(st)
+    assert up == case["update_path"]
+    assert find_orthogonalisation_path(up, st) == case["orth_path"]
+
+
+def test_add_child_to_parent_leg_order():
+    import torch
+    st = ptn.TreeTensorNetworkState()
+    root = torch.arange(2 * 3 * 4, dtype=torch.float64).reshape(2, 3, 4).to(torch.complex128)
+    st.add_root(ptn.Node(identifier="r"), root)
+    child = torch.zeros((5, 3), dtype=torch.complex128)       # leg 1 meets the root
+    st.add_child_to_parent(ptn.Node(identifier="c"), child, 1, "r", 1)
+    assert st.nodes["r"].children == ["c"]
+    assert st.tensors["r"].shape == (3, 2, 4)                  # (child, open...)
+    assert st.tensors["c"].shape == (3, 5)                     # (parent, open)
+    assert torch.equal(st.tensors["r"], root.permute(1, 0, 2))
+
+
+def test_product_truncation_rules_match_kats():
+    with open(os.path.join(GOLD, "truncation.json")) as f:
+        g = json.load(f)
+    s = np.array(g["s"])
+    for c in g["value_truncation"]:
+        assert len(ptn.value_truncation(s, c["total_tol"], c["rel_tol"])) == c["kept"]
+    for c in g["sum_truncation"]:
+        assert len(ptn.sum_truncation(s, c["total_tol"])) == c["kept"]
+    for c in g["truncate_singular_values"]:
+        params = ptn.SVDParameters(c["max_bond_dim"], c["rel_tol"], c["total_tol"])
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            kept, trunc = ptn.truncate_singular_values(s, params)
+        assert len(kept) == c["kept"]
+        np.testing.assert_allclose(trunc, c["truncated"])
+        assert bool(w) == c.get("warns", False)
+
+
+def test_svd_parameter_validation():
+    with pytest.raises(TypeError):
+        ptn.SVDParameters(max_bond_dim=2.5)
+    with pytest.raises(ValueError):
+        ptn.SVDParameters(rel_tol=-1)
+    cfg = ptn.TwoSiteTDVPConfig(max_bond_dim=128, rel_tol=1e-10, total_tol=1e-10)
+    assert cfg.max_bond_dim == 128 and cfg.time_evo_mode is ptn.TimeEvoMode.FASTEST
+
+
+def test_step_count_rule():
+    from pytreenet_amd.time_evolution import TimeEvolution
+    class T(TimeEvolution):
+        pass
+    import torch
+    st = _cpu_state("r", [("a", "r")])
+    assert T(st, 0.01, 0.2, []).num_time_steps == 20
+    assert T(st, 0.3, 1.0, []).num_time_steps == 4   # 3.33.. -> frac 0.33 >= 0.1 -> +1
+
+
+def test_bond_dim_caps_of_benchmark_trees():
+    ids, parent, children = complete_tree(3, 3)
+    b = bond_dims(ids, parent, children, 256)
+    assert len(ids) == 40
+    assert sorted(set(b.values())) == [2, 16, 256]
+    ids, parent, children = complete_tree(2, 5)
+    b = bond_dims(ids, parent, children, 64)
+    assert len(ids) == 63 and sorted(set(b.values())) == [2, 8, 64]
+
+
+def test_network_planner_optimal_root_cost():
+    """Flop-optimal order at the ternary root (chi=256, D=5, d=2): 2472.3 GFLOP."""
+    from pytreenet_amd.engine import plan_network
+    labels = [[("a", i) for i in range(3)] + ["p"],
+              [("w", i) for i in range(3)] + ["q", "p"]] + \
+             [[("a", i), ("w", i), ("b", i)] for i in range(3)]
+    dims = {("a", i): 256 for i in range(3)}
+    dims.update({("b", i): 256 for i in range(3)})
+    dims.update({("w", i): 5 for i in range(3)})
+    dims.update({"p": 2, "q": 2})
+    plan = plan_network(labels, dims, [("b", i) for i in range(3)] + ["q"])
+    assert abs(plan.flops / 1e9 - 2472.3) < 0.1
+    assert len(plan.steps) == 4
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "ptn_hip.h")).read()
+    declared = set(re.findall(r"\b(ptn_[a-z0-9_]+)\s*\(", header))
+    from pytreenet_amd import _lib
+    assert declared == set(_lib.EXPORTED)
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libptnhip.so not built")
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert _lib.load().ptn_version() >= 100
+
+
+def test_product_refuses_cpu_tensors():
+    import torch
+    from pytreenet_amd import engine as E
+    a = torch.zeros((2, 2), dtype=torch.complex128)
+    with pytest.raises(ptn.PtnBackendError):
+        E.contract(a, "ij", a, "jk", "ik")
