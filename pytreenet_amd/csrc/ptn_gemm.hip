@@ -17,7 +17,8 @@
 //           im = T3-T1-T2; operands are converted to float while staging
 //           (complex128 inputs are read as double and rounded once).
 // MFMA operand layout (16x16x4): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; accumulator register r of lane l is D[4*(l/16)+r][l%16].
+// B[l/16][l%16]; accumulator register r of lane l holds column l%16 and row
+// 4*r + l/16 for the f64 instruction, row 4*(l/16) + r for the f32 one.
 // Block -> tile mapping is XCD-aware: the hardware deals workgroups to the 8
 // XCDs round-robin, the remap gives each XCD a contiguous tile range so tiles
 // sharing an A row panel share one L2.
@@ -227,7 +228,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + wm * 32 + i * 16 + fk * 4 + r;
+        // accumulator row of (lane, r): f64 16x16x4 -> 4*r + lane/16,
+        // f32 16x16x4 -> 4*(lane/16) + r
+        const int rr = (MODE == PTN_COMPUTE_F64_4M) ? (r * 4 + fk) : (fk * 4 + r);
+        const int64_t gm = m0 + wm * 32 + i * 16 + rr;
         if (gm >= p.M) continue;
         double re, im;
         if constexpr (MODE == PTN_COMPUTE_F64_4M) {

@@ -328,15 +328,16 @@ class TreeTensorNetworkState(TreeTensorNetwork):
 
     def scalar_product(self, other: Optional["TreeTensorNetworkState"] = None,
                        use_orthogonal_center: bool = True) -> complex:
-        """<self|other>; with ``other`` None the squared norm (``ttns.py:65-93``)."""
-        from .contractions import contract_two_ttns
+        """sum self * conj(other) as the reference computes it (``ttns.py:65-93``);
+        with ``other`` None the squared norm."""
+        from .contractions import scalar_product
         if other is None:
             c = self.orthogonality_center_id
             if use_orthogonal_center and c is not None:
                 t = self._tensors[c]
                 return complex(_dotc(t, t))
             other = self
-        return contract_two_ttns(self, other)
+        return scalar_product(self, other)
 
     def norm(self) -> float:
         return float(abs(self.scalar_product()) ** 0.5)

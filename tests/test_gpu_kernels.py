@@ -61,10 +61,10 @@ def test_gather_gemm_views_accumulate_alpha(gpu):
     a, b = _crand(rng, (8, 9, 10)), _crand(rng, (10, 8, 7))
     ta = _dev(a).permute(2, 0, 1)          # non-contiguous view, labels "kij"
     tb = _dev(b)[:, ::2, :]                # strided slice (10, 4, 7)
-    out = _dev(_crand(rng, (9, 4, 7)))
+    out = _dev(_crand(rng, (9, 4, 8, 7))).permute(0, 1, 3, 2)   # strided output view
     base = out.cpu().numpy().copy()
-    E.contract(ta, "kij", tb, "klm", "jlm", out=out, accumulate=True, alpha=0.5 - 2j)
-    ref = base + (0.5 - 2j) * np.einsum("ijk,klm->jlm", a, b[:, ::2, :])
+    E.contract(ta, "kij", tb, "klm", "jlmi", out=out, accumulate=True, alpha=0.5 - 2j)
+    ref = base + (0.5 - 2j) * np.einsum("ijk,klm->jlmi", a, b[:, ::2, :])
     np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-11)
 
 
