@@ -71,6 +71,8 @@ typedef struct ptn_gemm_desc {
   int32_t c_dtype, beta_one;
   double alpha_re, alpha_im;
   int32_t compute, _pad2;
+  void *ws;            /* optional scratch for split-K partials (may be NULL) */
+  int64_t ws_bytes;
 } ptn_gemm_desc;
 
 int ptn_gemm_gather(void *stream, const ptn_gemm_desc *desc);
@@ -114,6 +116,21 @@ int ptn_norm_sq(void *stream, int64_t n, const void *x, void *out, void *work);
 /* <x|y> = sum conj(x) y into out (complex128, device).                      */
 int ptn_dotc(void *stream, int64_t n, const void *x, const void *y, void *out,
              void *work);
+
+/* Cholesky factor of a Hermitian positive definite n x n complex128 matrix
+ * (n <= 1024): L (n x n row-major, lower, zero upper part) with
+ * L L^H = G + s I, s = shift_factor * trace(G) (the shift of shifted
+ * CholeskyQR).  *status (device int) is set to 1 if a pivot is <= min_pivot
+ * (the panel is numerically rank deficient), else 0.  One workgroup.
+ * Together with ptn_trtri_lower and gather-GEMM Gram/update products this
+ * forms the CholeskyQR3 node split that replaces np.linalg.qr in
+ * tensor_qr_decomposition (util/tensor_splitting.py:60-107).                */
+int ptn_potrf_lower(void *stream, int32_t n, const void *G, int64_t ldg, void *L,
+                    double shift_factor, double min_pivot, void *status);
+
+/* X = L^{-1} for a lower triangular n x n complex128 matrix (row-major,
+ * n <= 1024).                                                              */
+int ptn_trtri_lower(void *stream, int32_t n, const void *L, void *X);
 
 /* Reduced QR of a column-major m x n complex128 matrix A (lda = m):
  * on return the first k = min(m,n) columns of A hold Q (column-major, lda = m)

@@ -30,6 +30,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+constexpr int64_t SPLITK_MIN_K = 512;
+constexpr int64_t SPLITK_TARGET_BLOCKS = 2048;  // 4 waves of 2 blocks x 256 CUs
 
 struct KArgs {
   int64_t M, N, K;
@@ -41,6 +43,8 @@ struct KArgs {
   const int64_t *com, *con;
   double are, aim;
   int64_t tiles_n;
+  int64_t kchunk;   // K range per split (multiple of BK)
+  double2 *ws;      // split-K partials [ksplit][M][N], null when gridDim.y == 1
   int aconj, akmaj, bconj, bkmaj, beta_one, remap;
 };
 
@@ -85,6 +89,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
   }
   const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
   const int64_t m0 = tm * BM, n0 = tn * BN;
+  // split-K: this block covers k in [kbeg, kend)
+  const int64_t kbeg = (int64_t)blockIdx.y * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
   const TA *A = static_cast<const TA *>(p.A);
   const TB *B = static_cast<const TB *>(p.B);
 
@@ -123,35 +130,49 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
     b_noff[i] = b_nval[i] ? p.bon[gn] : 0;
   }
 
+  // Software pipeline: the k-offsets of stage s+1 are fetched while stage s's
+  // data loads are in flight, and the loaded values are only consumed (sign of
+  // the conjugation applied) when they are written to LDS after the MFMA block.
   T ra_r[4], ra_i[4], rb_r[4], rb_i[4];
+  int64_t a_koff[4], b_koff[4];
   const T asg = p.aconj ? T(-1) : T(1);
   const T bsg = p.bconj ? T(-1) : T(1);
 
-  auto gload = [&](int64_t k0) {
+  auto kload = [&](int64_t k0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int64_t gk = k0 + a_kl[i];
-      T re = 0, im = 0;
-      if (a_mval[i] && gk < p.K) ptn::cload(A + a_moff[i] + p.aok[gk], re, im);
-      ra_r[i] = re;
-      ra_i[i] = asg * im;
+      a_koff[i] = (gk < kend) ? p.aok[gk] : -1;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int64_t gk = k0 + b_kl[i];
+      b_koff[i] = (gk < kend) ? p.bok[gk] : -1;
+    }
+  };
+  auto gload = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
       T re = 0, im = 0;
-      if (b_nval[i] && gk < p.K) ptn::cload(B + p.bok[gk] + b_noff[i], re, im);
+      if (a_mval[i] && a_koff[i] >= 0) ptn::cload(A + a_moff[i] + a_koff[i], re, im);
+      ra_r[i] = re;
+      ra_i[i] = im;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      T re = 0, im = 0;
+      if (b_nval[i] && b_koff[i] >= 0) ptn::cload(B + b_koff[i] + b_noff[i], re, im);
       rb_r[i] = re;
-      rb_i[i] = bsg * im;
+      rb_i[i] = im;
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       sAr[buf][a_kl[i]][a_ml[i]] = ra_r[i];
-      sAi[buf][a_kl[i]][a_ml[i]] = ra_i[i];
+      sAi[buf][a_kl[i]][a_ml[i]] = asg * ra_i[i];
       sBr[buf][b_kl[i]][b_nl[i]] = rb_r[i];
-      sBi[buf][b_kl[i]][b_nl[i]] = rb_i[i];
+      sBi[buf][b_kl[i]][b_nl[i]] = bsg * rb_i[i];
     }
   };
 
@@ -165,16 +186,21 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
       c2[i][j] = V{0, 0, 0, 0};
     }
 
-  const int64_t KT = (p.K + BK - 1) / BK;
+  const int64_t KT = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (KT > 0) {
-    gload(0);
+    kload(kbeg);
+    gload();
     sstore(0);
+    if (KT > 1) kload(kbeg + BK);
   }
   __syncthreads();
   const int fr = lane & 15, fk = lane >> 4;
   for (int64_t kt = 0; kt < KT; ++kt) {
     const int buf = kt & 1;
-    if (kt + 1 < KT) gload((kt + 1) * BK);
+    if (kt + 1 < KT) {
+      gload();                                   // data of stage kt+1
+      if (kt + 2 < KT) kload(kbeg + (kt + 2) * BK);  // offsets of stage kt+2
+    }
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       const int kr = kk * 4 + fk;
@@ -242,6 +268,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
           re = (double)(t1 - t2);
           im = (double)(t3 - t1 - t2);
         }
+        if (gridDim.y > 1) {
+          // raw partial sum; the reduction kernel applies alpha, beta and the scatter
+          p.ws[((int64_t)blockIdx.y * p.M + gm) * p.N + gn] = make_double2(re, im);
+          continue;
+        }
         double ore = p.are * re - p.aim * im;
         double oim = p.are * im + p.aim * re;
         TC *dst = C + p.com[gm] + noff;
@@ -257,24 +288,57 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
   }
 }
 
+// sum the split-K partials in split order, then alpha / beta / scatter store
+template <typename TC>
+__global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
+  const int64_t total = p.M * p.N;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  TC *C = static_cast<TC *>(p.C);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+    double re = 0, im = 0;
+    for (int s = 0; s < ksplit; ++s) {
+      const double2 v = p.ws[(int64_t)s * total + t];
+      re += v.x;
+      im += v.y;
+    }
+    const int64_t m = t / p.N, n = t - m * p.N;
+    double ore = p.are * re - p.aim * im;
+    double oim = p.are * im + p.aim * re;
+    TC *dst = C + p.com[m] + p.con[n];
+    if (p.beta_one) {
+      double xr, xi;
+      ptn::cload(dst, xr, xi);
+      ore += xr;
+      oim += xi;
+    }
+    ptn::cstore(dst, ore, oim);
+  }
+}
+
 template <int MODE, typename TA, typename TB, typename TC>
-int launch(hipStream_t s, const KArgs &a, int64_t tiles) {
-  hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC>), dim3((unsigned)tiles),
+int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
+  hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC>), dim3((unsigned)tiles, ksplit),
                      dim3(NT), 0, s, a);
   PTN_LAUNCH_CHECK();
+  if (ksplit > 1) {
+    int64_t g = (a.M * a.N + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL((splitk_reduce<TC>), dim3((unsigned)g), dim3(256), 0, s, a, ksplit);
+    PTN_LAUNCH_CHECK();
+  }
   return PTN_OK;
 }
 
 template <int MODE, typename TA, typename TB>
-int dispatch_c(hipStream_t s, const KArgs &a, int64_t tiles, int cdt) {
-  if (cdt == PTN_C128) return launch<MODE, TA, TB, double2>(s, a, tiles);
-  return launch<MODE, TA, TB, float2>(s, a, tiles);
+int dispatch_c(hipStream_t s, const KArgs &a, int64_t tiles, int cdt, int ks) {
+  if (cdt == PTN_C128) return launch<MODE, TA, TB, double2>(s, a, tiles, ks);
+  return launch<MODE, TA, TB, float2>(s, a, tiles, ks);
 }
 
 template <int MODE, typename TA>
-int dispatch_b(hipStream_t s, const KArgs &a, int64_t tiles, int bdt, int cdt) {
-  if (bdt == PTN_C128) return dispatch_c<MODE, TA, double2>(s, a, tiles, cdt);
-  return dispatch_c<MODE, TA, float2>(s, a, tiles, cdt);
+int dispatch_b(hipStream_t s, const KArgs &a, int64_t tiles, int bdt, int cdt, int ks) {
+  if (bdt == PTN_C128) return dispatch_c<MODE, TA, double2>(s, a, tiles, cdt, ks);
+  return dispatch_c<MODE, TA, float2>(s, a, tiles, cdt, ks);
 }
 
 }  // namespace
@@ -333,10 +397,28 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
     return PTN_ERANGE;
   }
   a.remap = (tiles % 8 == 0) ? 1 : 0;
+  // split-K when the tile grid cannot fill the chip and K is long: each split
+  // keeps >= SPLITK_MIN_K of K, partials go to the caller's workspace
+  int ksplit = 1;
+  if (tiles < 256 && d->K >= 2 * SPLITK_MIN_K && d->ws && d->ws_bytes > 0) {
+    int64_t want = (SPLITK_TARGET_BLOCKS + tiles - 1) / tiles;
+    const int64_t by_k = d->K / SPLITK_MIN_K;
+    if (want > by_k) want = by_k;
+    const int64_t by_ws = d->ws_bytes / ((int64_t)sizeof(double2) * d->M * d->N);
+    if (want > by_ws) want = by_ws;
+    if (want > 1024) want = 1024;
+    if (want > 1) ksplit = (int)want;
+  }
+  const int64_t kc = (d->K + ksplit - 1) / ksplit;
+  a.kchunk = ((kc + BK - 1) / BK) * BK;
+  ksplit = (int)((d->K + a.kchunk - 1) / a.kchunk);
+  if (ksplit < 1) ksplit = 1;
+  a.ws = ksplit > 1 ? static_cast<double2 *>(d->ws) : nullptr;
+  if (d->K == 0) a.kchunk = BK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (d->compute == PTN_COMPUTE_F64_4M)
-    return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles);
+    return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles, ksplit);
   if (d->a_dtype == PTN_C128)
-    return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype);
-  return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype);
+    return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
+  return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
 }

@@ -1,8 +1,13 @@
-// Dense factorizations for the node splits: reduced QR and thin SVD in FP64.
+// Dense factorizations for the node splits, FP64.
 //
-// Both run on the caller's stream through rocSOLVER (AMD's LAPACK on ROCm)
-// with one lazily created handle per device.  R and the packed singular
-// vectors are produced by small kernels here.
+// * ptn_potrf_lower / ptn_trtri_lower: the small n x n Cholesky factor and its
+//   inverse used by the CholeskyQR3 node split (Gram matrices and triangular
+//   updates of that split run as gather-GEMMs).  One workgroup factors the
+//   matrix in place in global memory (L2 resident for n <= 1024); the inverse
+//   runs one workgroup per column.
+// * ptn_qr / ptn_svd: Householder QR and thin SVD through rocSOLVER (AMD's
+//   LAPACK on ROCm) with one lazily created handle per device -- the QR is the
+//   fallback for rank-deficient panels, the SVD serves the truncated splits.
 #include "ptn_common.h"
 
 #include <mutex>
@@ -52,7 +57,149 @@ unsigned grid_for(int64_t total) {
   return (unsigned)g;
 }
 
+// ---- Cholesky (lower, in place), optional diagonal shift = shift_factor * trace
+constexpr int CT = 1024;
+
+__global__ __launch_bounds__(CT) void potrf_lower_kernel(int n, const double2 *G, int64_t ldg,
+                                                         double2 *L, double shift_factor,
+                                                         double min_pivot, int *status) {
+  __shared__ double2 colj[1024];
+  __shared__ double red[CT / 64];
+  __shared__ double sh_shift, sh_inv;
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x;
+  // trace for the shift
+  double tr = 0;
+  for (int i = tid; i < n; i += CT) tr += G[(int64_t)i * ldg + i].x;
+  for (int off = 32; off > 0; off >>= 1) tr += __shfl_down(tr, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = tr;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0;
+    for (int w = 0; w < CT / 64; ++w) t += red[w];
+    sh_shift = shift_factor * t;
+    sh_fail = 0;
+  }
+  __syncthreads();
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t t = tid; t < nn; t += CT) {
+    const int i = (int)(t / n), j = (int)(t - (int64_t)i * n);
+    double2 v = make_double2(0, 0);
+    if (j <= i) {
+      v = G[(int64_t)i * ldg + j];
+      if (i == j) v = make_double2(v.x + sh_shift, 0.0);
+    }
+    L[t] = v;
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      double d = L[(int64_t)j * n + j].x;
+      if (!(d > min_pivot)) {  // also catches NaN
+        sh_fail = 1;
+        d = 1.0;
+      }
+      const double r = sqrt(d);
+      L[(int64_t)j * n + j] = make_double2(r, 0.0);
+      sh_inv = 1.0 / r;
+    }
+    __syncthreads();
+    const double inv = sh_inv;
+    for (int i = j + 1 + tid; i < n; i += CT) {
+      double2 v = L[(int64_t)i * n + j];
+      v.x *= inv;
+      v.y *= inv;
+      L[(int64_t)i * n + j] = v;
+      colj[i] = v;
+    }
+    __syncthreads();
+    const int rem = n - j - 1;
+    const int64_t cnt = (int64_t)rem * rem;
+    for (int64_t t = tid; t < cnt; t += CT) {
+      const int i = j + 1 + (int)(t / rem), k = j + 1 + (int)(t % rem);
+      if (k > i) continue;
+      const double2 a = colj[i], b = colj[k];
+      double2 v = L[(int64_t)i * n + k];
+      // v -= a * conj(b)
+      v.x -= a.x * b.x + a.y * b.y;
+      v.y -= a.y * b.x - a.x * b.y;
+      L[(int64_t)i * n + k] = v;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *status = sh_fail;
+}
+
+// X = L^{-1} (lower); block j solves L x = e_j by forward substitution
+constexpr int TT = 256;
+__global__ __launch_bounds__(TT) void trtri_lower_kernel(int n, const double2 *L, double2 *X) {
+  __shared__ double2 x[1024];
+  __shared__ double2 red[TT / 64];
+  const int j = blockIdx.x, tid = threadIdx.x;
+  for (int i = tid; i < n; i += TT) x[i] = make_double2(0, 0);
+  __syncthreads();
+  if (tid == 0) {
+    const double2 d = L[(int64_t)j * n + j];
+    const double den = d.x * d.x + d.y * d.y;
+    x[j] = make_double2(d.x / den, -d.y / den);
+  }
+  __syncthreads();
+  for (int i = j + 1; i < n; ++i) {
+    double sr = 0, si = 0;
+    for (int k = j + tid; k < i; k += TT) {
+      const double2 l = L[(int64_t)i * n + k], xk = x[k];
+      sr += l.x * xk.x - l.y * xk.y;
+      si += l.x * xk.y + l.y * xk.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sr += __shfl_down(sr, off, 64);
+      si += __shfl_down(si, off, 64);
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = make_double2(sr, si);
+    __syncthreads();
+    if (tid == 0) {
+      double tr = 0, ti = 0;
+      for (int w = 0; w < TT / 64; ++w) {
+        tr += red[w].x;
+        ti += red[w].y;
+      }
+      const double2 d = L[(int64_t)i * n + i];
+      const double den = d.x * d.x + d.y * d.y;
+      // x_i = -(sum) / d
+      x[i] = make_double2(-(tr * d.x + ti * d.y) / den, -(ti * d.x - tr * d.y) / den);
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += TT) X[(int64_t)i * n + j] = x[i];
+}
+
 }  // namespace
+
+extern "C" int ptn_potrf_lower(void *stream, int32_t n, const void *G, int64_t ldg, void *L,
+                               double shift_factor, double min_pivot, void *status) {
+  if (n <= 0 || n > 1024 || ldg < n || !G || !L || !status) {
+    ptn::set_error("ptn_potrf_lower: bad arguments (n=%d, max 1024)", n);
+    return n > 1024 ? PTN_ERANGE : PTN_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(potrf_lower_kernel, dim3(1), dim3(CT), 0, s, n,
+                     static_cast<const double2 *>(G), ldg, static_cast<double2 *>(L),
+                     shift_factor, min_pivot, static_cast<int *>(status));
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+extern "C" int ptn_trtri_lower(void *stream, int32_t n, const void *L, void *X) {
+  if (n <= 0 || n > 1024 || !L || !X) {
+    ptn::set_error("ptn_trtri_lower: bad arguments (n=%d, max 1024)", n);
+    return n > 1024 ? PTN_ERANGE : PTN_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(trtri_lower_kernel, dim3(n), dim3(TT), 0, s, n,
+                     static_cast<const double2 *>(L), static_cast<double2 *>(X));
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
 
 extern "C" int ptn_qr(void *stream, int64_t m, int64_t n, void *A, void *R, void *tau) {
   if (m <= 0 || n <= 0 || !A || !R || !tau || m > 0x7fffffff || n > 0x7fffffff) {

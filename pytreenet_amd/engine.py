@@ -95,9 +95,63 @@ class _OffsetCache:
 _CACHE = _OffsetCache()
 
 
+class LaunchProfiler:
+    """Executed-flop counters and HIP-event timings of gather-GEMM launches.
+
+    Events are recorded on the stream the kernels are launched on (torch's
+    current stream) around every launch with at least ``min_flops``; the
+    per-shape averages give the duration of the dominant kernel.
+    """
+
+    def __init__(self, min_flops: float = 1e9):
+        self.min_flops = min_flops
+        self.active = False
+        self.reset()
+
+    def reset(self):
+        self.flops = 0.0
+        self.launches = 0
+        self.events: Dict[tuple, list] = {}
+
+    def start(self):
+        self.reset()
+        self.active = True
+
+    def stop(self):
+        self.active = False
+
+    def summary(self) -> List[dict]:
+        torch.cuda.synchronize()
+        rows = []
+        for key, evs in self.events.items():
+            ms = [a.elapsed_time(b) for a, b in evs]
+            rows.append({"M": key[0], "N": key[1], "K": key[2], "compute": key[3],
+                         "flops_per_launch": 8.0 * key[0] * key[1] * key[2],
+                         "launches": len(ms), "total_ms": float(sum(ms)),
+                         "avg_ms": float(sum(ms) / len(ms))})
+        rows.sort(key=lambda r: -r["total_ms"])
+        return rows
+
+
+PROFILER = LaunchProfiler()
+
+
 def clear_caches():
     _CACHE.d.clear()
     _PLANS.clear()
+
+
+_SPLITK_WS: Dict[torch.device, torch.Tensor] = {}
+SPLITK_WS_BYTES = 256 << 20
+
+
+def _splitk_workspace(device) -> torch.Tensor:
+    """Per-device scratch for split-K partial sums (kernels on one stream share it)."""
+    ws = _SPLITK_WS.get(device)
+    if ws is None:
+        ws = torch.empty(SPLITK_WS_BYTES // 16, dtype=torch.complex128, device=device)
+        _SPLITK_WS[device] = ws
+    return ws
 
 
 def _inner_is_unit(dims, strides) -> bool:
@@ -184,7 +238,23 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     alpha = complex(alpha)
     d.alpha_re, d.alpha_im = alpha.real, alpha.imag
     d.compute = _COMPUTE[precision]
+    ws = _splitk_workspace(A.device)
+    d.ws, d.ws_bytes = ws.data_ptr(), ws.numel() * ws.element_size()
     lib = _lib.load()
+    prof = PROFILER
+    if prof.active:
+        fl = 8.0 * M * N * K
+        prof.flops += fl
+        prof.launches += 1
+        if fl >= prof.min_flops:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            _lib.check(lib.ptn_gemm_gather(ctypes.c_void_p(stream_handle()), ctypes.byref(d)),
+                       "ptn_gemm_gather")
+            ev1.record()
+            prof.events.setdefault((M, N, K, precision), []).append((ev0, ev1))
+            return out
     _lib.check(lib.ptn_gemm_gather(ctypes.c_void_p(stream_handle()), ctypes.byref(d)),
                "ptn_gemm_gather")
     return out

@@ -175,14 +175,79 @@ def tensor_qr_decomposition(tensor: torch.Tensor, q_legs: Sequence[int], r_legs:
     return Q, Rt
 
 
+_U = 2.0 ** -53
+
+
+def _potrf(G: torch.Tensor, shift_factor: float, min_pivot: float, status: torch.Tensor):
+    n = G.shape[0]
+    L = torch.empty((n, n), dtype=torch.complex128, device=G.device)
+    _lib.check(_lib.load().ptn_potrf_lower(ctypes.c_void_p(E.stream_handle()), n,
+                                           ctypes.c_void_p(G.data_ptr()), G.stride(0),
+                                           ctypes.c_void_p(L.data_ptr()), shift_factor,
+                                           min_pivot, ctypes.c_void_p(status.data_ptr())),
+               "ptn_potrf_lower")
+    X = torch.empty((n, n), dtype=torch.complex128, device=G.device)
+    _lib.check(_lib.load().ptn_trtri_lower(ctypes.c_void_p(E.stream_handle()), n,
+                                           ctypes.c_void_p(L.data_ptr()),
+                                           ctypes.c_void_p(X.data_ptr())), "ptn_trtri_lower")
+    return L, X
+
+
+def cholesky_qr3_leg(t: torch.Tensor, leg: int):
+    """Shifted CholeskyQR3 of ``t`` matricised as (all other legs) x (leg ``leg``).
+
+    Returns (Q with the new bond at position ``leg``, R (new, old), ok).  Each pass
+    is a Gram gather-GEMM G = A^H A, a device Cholesky L L^H = G (+ shift in the
+    first pass, s = 11 (mn + n(n+1)) u tr(G)), and the update A <- A L^{-H} as a
+    gather-GEMM straight in the tensor layout.  ``ok`` is False when a pivot
+    collapsed (numerically rank-deficient panel); the caller then falls back to
+    Householder QR.
+    """
+    n = t.shape[leg]
+    m = t.numel() // n
+    lab = list(range(t.dim()))
+    la = list(lab)
+    la[leg] = "c"
+    lb = list(lab)
+    lb[leg] = "c2"
+    status = torch.zeros(3, dtype=torch.int32, device=t.device)
+    q = t
+    Ls = []
+    for p in range(3):
+        G = E.contract(q, la, q, lb, ["c", "c2"], conj_a=True)
+        shift = 11.0 * (m * n + n * (n + 1)) * _U if p == 0 else 0.0
+        L, X = _potrf(G, shift, 0.0 if p == 0 else 1e-14, status[p:p + 1])
+        Ls.append(L)
+        lq = list(lab)
+        lq[leg] = "k"
+        q = E.contract(q, lq, X, ["c", "k"], la, conj_b=True)
+    P = E.contract(Ls[0], ["a", "b"], Ls[1], ["b", "c"], ["a", "c"])
+    P = E.contract(P, ["a", "c"], Ls[2], ["c", "d"], ["a", "d"])
+    R = E.permute_copy(P, [1, 0], conj=True)
+    ok = int(status.sum().cpu().item()) == 0
+    return q, R, ok
+
+
+QR_STATS = {"cholqr3": 0, "householder": 0}
+
+
 def qr_split_to_neighbour(ttn, node_id: str, neighbour_id: str):
     """QR(KEEP) of a node off its leg to ``neighbour_id``.
 
     Returns (Q in the node's canonical layout with the new bond at the old
-    leg's position, R with legs (new, old)).
+    leg's position, R with legs (new, old)).  Panels with rows >= columns use
+    shifted CholeskyQR3; wide or rank-deficient panels use Householder QR.
     """
     t = ttn.tensors[node_id]
     leg = ttn.nodes[node_id].neighbour_index(neighbour_id)
+    n = t.shape[leg]
+    m = t.numel() // n
+    if m >= n and n <= 1024:
+        q, r, ok = cholesky_qr3_leg(t, leg)
+        if ok:
+            QR_STATS["cholqr3"] += 1
+            return q, r
+    QR_STATS["householder"] += 1
     others = [i for i in range(t.dim()) if i != leg]
     q, r = tensor_qr_decomposition(t, others, [leg], mode=SplitMode.KEEP)
     perm = list(range(q.dim() - 1))

@@ -276,6 +276,7 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         self.precision = getattr(self.config, "precision", E.FP64)
         if self.precision not in (E.FP64, E.FP32):
             raise ValueError(f"unknown precision {self.precision!r}")
+        self.apply_log: List[tuple] = []   # (kind, node or edge, Krylov applies)
         self.update_path = self._finds_update_path()
         self.orthogonalization_path = find_orthogonalisation_path(self.update_path, self.state)
         self._orthogonalize_init()
@@ -313,9 +314,12 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             return contract_ket_ham_with_envs(node, y, op_node, op, cache,
                                               precision=self.precision)
 
+        from .krylov import STATS
+        n0 = STATS.applies
         self.state.tensors[node_id] = self._krylov(tensor, action,
                                                    self.time_step_size * time_step_factor,
                                                    EvoDirection.FORWARD)
+        self.apply_log.append(("site", node_id, STATS.applies - n0))
 
     def _move_orth_and_update_cache_for_path(self, path: List[str]):
         if len(path) == 0:
@@ -337,8 +341,11 @@ class OneSiteTDVP(TDVPAlgorithm):
         self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
         env_a = self.partial_tree_cache.get_entry(node_id, next_node_id)
         env_b = self.partial_tree_cache.get_entry(next_node_id, node_id)
+        from .krylov import STATS
+        n0 = STATS.applies
         r = self._krylov(r, lambda y: bond_apply(y, env_a, env_b, precision=self.precision),
                          self.time_step_size * time_step_factor, EvoDirection.BACKWARD)
+        self.apply_log.append(("bond", (node_id, next_node_id), STATS.applies - n0))
         absorb_into_neighbour(self.state, next_node_id, node_id, r)
         self.state.orthogonality_center_id = next_node_id
 
@@ -478,8 +485,11 @@ class TwoSiteTDVP(TDVPAlgorithm):
         outer2 = [x for x in l2[:-1] if x != "bond"]
         x = E.contract(A1, l1, A2, l2, outer1 + outer2 + ["p1", "p2"])
         action, _, _ = self._two_site_action(n1, n2)
+        from .krylov import STATS
+        a0 = STATS.applies
         x = self._krylov(x, action, self.time_step_size * time_step_factor,
                          EvoDirection.FORWARD)
+        self.apply_log.append(("two_site", (n1, n2), STATS.applies - a0))
         # truncated SVD, S absorbed into V (VCONTR)
         dims = list(x.shape)
         udims = dims[:k1] + [dims[k1 + k2]]

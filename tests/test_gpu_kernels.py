@@ -162,3 +162,57 @@ def test_svd(gpu, shape):
     np.testing.assert_allclose(s, np.linalg.svd(mat, compute_uv=False), rtol=1e-12)
     rec = np.einsum("aek,k,kb->aeb", u.cpu().numpy(), s, vh.cpu().numpy()).transpose(0, 2, 1)
     np.testing.assert_allclose(rec, t, atol=1e-11)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gather_gemm_split_k(gpu, prec):
+    """Few output tiles, long K: the K range is split across workgroups."""
+    _say(f"split-K {prec}")
+    rng = np.random.default_rng(51)
+    a, b = _crand(rng, (40, 5, 3000)), _crand(rng, (3000, 5, 16))
+    dt = torch.complex128 if prec == "fp64" else torch.complex64
+    out = _dev(_crand(rng, (16, 40)), dt)
+    base = out.cpu().numpy().copy()
+    E.contract(_dev(a, dt), "mwk", _dev(b, dt), "kwn", "nm", precision=prec, out=out,
+               accumulate=True, alpha=2.0)
+    ref = base + 2.0 * np.einsum("mwk,kwn->nm", a, b)
+    err = np.abs(out.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < (1e-13 if prec == "fp64" else 2e-6), err
+
+
+@pytest.mark.parametrize("shape,leg", [((8, 8, 2), 0), ((16, 4, 4, 2), 2), ((64, 64, 2), 1),
+                                       ((300, 40), 1)])
+def test_cholesky_qr3_split(gpu, shape, leg):
+    _say(f"cholqr3 {shape} leg {leg}")
+    rng = np.random.default_rng(61)
+    t = _crand(rng, shape)
+    q, r, ok = S.cholesky_qr3_leg(_dev(t), leg)
+    assert ok
+    q, r = q.cpu().numpy(), r.cpu().numpy()
+    n = shape[leg]
+    rec = np.moveaxis(np.tensordot(np.moveaxis(q, leg, -1), r, axes=([q.ndim - 1], [0])), -1, leg)
+    np.testing.assert_allclose(rec, t, atol=1e-12 * np.abs(t).max())
+    qm = np.moveaxis(q, leg, -1).reshape(-1, n)
+    np.testing.assert_allclose(qm.conj().T @ qm, np.eye(n), atol=1e-13)
+
+
+def test_cholesky_qr3_flags_rank_deficient(gpu):
+    _say("cholqr3 rank deficient")
+    rng = np.random.default_rng(62)
+    a = _crand(rng, (20, 3)) @ _crand(rng, (3, 6))       # rank 3, 6 columns
+    q, r, ok = S.cholesky_qr3_leg(_dev(a), 1)
+    assert not ok
+
+
+def test_potrf_trtri(gpu):
+    _say("potrf/trtri")
+    rng = np.random.default_rng(63)
+    a = _crand(rng, (300, 200))
+    g = a.conj().T @ a
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    L, X = S._potrf(_dev(g), 0.0, 0.0, status)
+    L, X = L.cpu().numpy(), X.cpu().numpy()
+    assert status.item() == 0
+    np.testing.assert_allclose(L @ L.conj().T, g, atol=1e-10 * np.abs(g).max())
+    assert np.abs(np.triu(L, 1)).max() == 0
+    np.testing.assert_allclose(X @ L, np.eye(200), atol=1e-11)
