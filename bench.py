@@ -53,6 +53,7 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool):
     import pytreenet_amd as ptn
     from pytreenet_amd import engine as E
     cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision)
+    E.PROFILER.phase = f"{precision}-init"
     t0 = time.time()
     alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.05 * (warmup + steps), [], config=cfg,
                                      solver_options={"max_krylov_dim": 10,
@@ -61,27 +62,30 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool):
     alg.state.tensors[c] = alg.state.tensors[c] / alg.state.norm()
     torch.cuda.synchronize()
     log(f"[{precision}] init {time.time() - t0:.2f}s")
+    E.PROFILER.phase = f"{precision}-warmup"
     for i in range(warmup):
         t1 = time.time()
         alg.run_one_time_step()
         torch.cuda.synchronize()
         log(f"[{precision}] warmup step {i} {time.time() - t1:.2f}s")
     nlog = len(alg.apply_log)
-    if profile:
-        E.PROFILER.start()
+    fl0, n0 = E.PROFILER.flops, E.PROFILER.launches
+    E.PROFILER.phase = f"{precision}-timed"
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for i in range(steps):
         alg.run_one_time_step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
-    E.PROFILER.stop()
     log(f"[{precision}] {steps} timed steps {elapsed:.3f}s")
+    fl1, n1 = E.PROFILER.flops, E.PROFILER.launches
+    E.PROFILER.phase = f"{precision}-eval"
     energy = ptn.expectation_value(alg.state, H).real
     return {"elapsed": elapsed, "energy": energy, "alg": alg,
             "applies": alg.apply_log[nlog:],
-            "gemm_flops": E.PROFILER.flops, "launches": E.PROFILER.launches,
-            "kernels": E.PROFILER.summary() if profile else []}
+            "gemm_flops": fl1 - fl0, "launches": n1 - n0,
+            "kernels": E.PROFILER.summary(phase=f"{precision}-timed", compute="fp64")
+            if profile else []}
 
 
 def cpu_baseline(chi_sample: int, steps_applies, shapes_full, wshapes_full):
@@ -148,6 +152,28 @@ def cpu_baseline(chi_sample: int, steps_applies, shapes_full, wshapes_full):
     return rate, f_step, t_env + t_apply
 
 
+def calibrate_copy():
+    """8 contiguous copies of 1 GiB (complex128) by the ptn_permute kernel.
+
+    Each launch reads 1 GiB and writes 1 GiB; the FETCH_SIZE / WRITE_SIZE
+    counters of these launches give the scale factor for the GEMM counters.
+    """
+    import torch
+    from pytreenet_amd import engine as E
+    n = (1 << 30) // 16
+    src = torch.ones(n, dtype=torch.complex128, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(8):
+        E.strided_copy([n], src, [1], dst, [1])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 8
+    print(json.dumps({"calibration": "ptn_permute contiguous copy", "bytes_read": 1 << 30,
+                      "bytes_written": 1 << 30, "ms_per_copy": dt * 1e3,
+                      "GBps_read_plus_write": 2 * (1 << 30) / dt / 1e9}), flush=True)
+
+
 def host_facts():
     model = ""
     try:
@@ -177,7 +203,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-chi", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 8 ptn_permute copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "bench.py runs on exactly one GPU: the TDVP sweep of one "
                                    "state is sequential and this build measures one GPU "
@@ -186,6 +217,9 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     dev = torch.device("cuda:0")
+    from pytreenet_amd import engine as E
+    # HIP events around every LARGE gather-GEMM launch of the process, tagged by phase
+    E.PROFILER.start("setup")
     st, H, _ = make_problem(args.chi, dev)
     r64 = run_tdvp(st, H, "fp64", args.warmup, args.steps, profile=True)
     sps64 = args.steps / r64["elapsed"]
@@ -193,7 +227,7 @@ def main():
            "fp64_gemm_tflops": r64["gemm_flops"] / r64["elapsed"] / 1e12,
            "fp64_launches_per_step": r64["launches"] / args.steps,
            "fp64_applies_per_step": sum(m for _, _, m in r64["applies"]) / args.steps}
-    kern = r64["kernels"][0] if r64["kernels"] else None
+    kern = r64["kernels"]
     mixed_ok = False
     if not args.no_mixed:
         r32 = run_tdvp(st, H, "fp32", args.warmup, args.steps, profile=False)
@@ -205,13 +239,31 @@ def main():
                     "mixed_gemm_tflops": r32["gemm_flops"] / r32["elapsed"] / 1e12})
     value = res["mixed_sweeps_per_s"] if mixed_ok else sps64
     roofline = None
-    if kern is not None:
-        ach = kern["flops_per_launch"] / (kern["avg_ms"] * 1e-3)
+    if kern:
+        # dominant kernel = the LARGE instantiation of the FP64 gather-GEMM (every
+        # launch with a tile grid >= 2048): flops per launch / mean launch time
+        n = sum(k["launches"] for k in kern)
+        fl = sum(k["flops_per_launch"] * k["launches"] for k in kern)
+        ms = sum(k["total_ms"] for k in kern)
+        ach = fl / (ms * 1e-3)
         roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12,
                     "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
-                    "kernel": "gemm_gather_kernel<F64_4M>",
-                    "shape": {"M": kern["M"], "N": kern["N"], "K": kern["K"]},
-                    "avg_ms": kern["avg_ms"], "launches": kern["launches"]}
+                    "kernel": "gemm_gather_kernel<F64_4M, c128, c128, c128, LARGE=1>",
+                    "flops_per_launch": fl / n, "avg_ms": ms / n, "launches": n,
+                    "timed_steps": args.steps,
+                    "shapes": [{"M": k["M"], "N": k["N"], "K": k["K"],
+                                "launches": k["launches"], "avg_ms": round(k["avg_ms"], 3),
+                                "tflops": round(k["flops_per_launch"] / k["avg_ms"] / 1e9, 2)}
+                               for k in kern[:6]]}
+    if roofline is not None:
+        allk = E.PROFILER.summary(compute="fp64")
+        n_all = sum(k["launches"] for k in allk)
+        roofline["all_phases"] = {"launches": n_all,
+                                  "avg_ms": sum(k["total_ms"] for k in allk) / n_all,
+                                  "note": "every LARGE FP64 launch of this process (setup, "
+                                          "warmup, timed, evaluation, QR of the mixed run): "
+                                          "the set rocprofv3 --stats averages over"}
+    E.PROFILER.stop()
     cpu = None
     if not args.no_cpu:
         alg = r64["alg"]

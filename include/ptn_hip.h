@@ -146,6 +146,24 @@ int ptn_qr(void *stream, int64_t m, int64_t n, void *A, void *R, void *tau);
 int ptn_svd(void *stream, int64_t m, int64_t n, void *A, void *S, void *U,
             void *Vh, void *E);
 
+/* Block-sparse operator application (the step of an H_eff / environment
+ * contraction where the TTNO node tensor meets a large tensor):
+ *   out[o_r[r] + o_g[g] + o_j[j]] =
+ *       sum_{t in group g} sum_i W[t][j][i] * Y[y_r[r] + y_t[t] + y_i[i]]
+ * for r < R.  Blocks t are sorted by group; group g owns t in
+ * [g_begin[g], g_begin[g+1]).  W: nt x nj x ni complex128 (device).
+ * y_r, o_r (R entries), y_t (nt), o_g (ng), g_begin (ng+1, int32) are device
+ * arrays; y_i (ni) and o_j (nj) are host arrays.  nt, ng <= 128, nj, ni <= 4.
+ * Replaces the operator-tensor np.tensordot of
+ * contractions/state_operator_contraction.py:263-272 and
+ * contraction_util.py:757-817 (contract_operator_tensor_ignoring_one_leg),
+ * skipping the TTNO's structurally zero bond-state blocks.                 */
+int ptn_block_apply(void *stream, int64_t R, int32_t nt, int32_t ng, int32_t nj, int32_t ni,
+                    const void *Y, int32_t y_dtype, const int64_t *y_r, const int64_t *y_t,
+                    const int64_t *y_i, int32_t conj_y, const int32_t *g_begin,
+                    const void *W, int32_t conj_w, void *out, int32_t out_dtype,
+                    const int64_t *o_r, const int64_t *o_g, const int64_t *o_j);
+
 const char *ptn_last_error(void);
 int ptn_version(void);
 

@@ -18,8 +18,8 @@ PTN_COMPUTE_F64_4M, PTN_COMPUTE_F32_3M = 0, 1
 EXPORTED = (
     "ptn_gemm_gather", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
     "ptn_krylov_orthogonalize", "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc",
-    "ptn_qr", "ptn_svd", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_last_error",
-    "ptn_version",
+    "ptn_qr", "ptn_svd", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
+    "ptn_last_error", "ptn_version",
 )
 
 
@@ -72,6 +72,8 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_svd": (ctypes.c_int, [vp, i64, i64, vp, vp, vp, vp, vp]),
         "ptn_potrf_lower": (ctypes.c_int, [vp, i32, vp, i64, vp, dbl, dbl, vp]),
         "ptn_trtri_lower": (ctypes.c_int, [vp, i32, vp, vp]),
+        "ptn_block_apply": (ctypes.c_int, [vp, i64, i32, i32, i32, i32, vp, i32, vp, vp, vp,
+                                           i32, vp, vp, i32, vp, i32, vp, vp, vp]),
         "ptn_last_error": (ctypes.c_char_p, []),
         "ptn_version": (ctypes.c_int, []),
     }

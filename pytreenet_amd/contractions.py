@@ -74,6 +74,18 @@ def _op_labels(ket_node: Node, op_node: Node) -> list:
     return labels
 
 
+USE_BLOCK_SPARSE = True
+
+
+def operator_tensor(operator: TreeTensorNetworkOperator, node_id: str, precision: str):
+    """TTNO node tensor as handed to the networks: block sparse over its bond-state
+    legs (structural zeros skipped exactly) unless ``USE_BLOCK_SPARSE`` is off."""
+    dt = E.storage_dtype(precision)
+    if USE_BLOCK_SPARSE:
+        return operator.block_tensor(node_id, dt)
+    return operator.tensor_in(node_id, dt)
+
+
 def contract_any(node_id: str, next_node_id: str, state: TreeTensorNetworkState,
                  operator: TreeTensorNetworkOperator, dictionary: PartialTreeCachDict,
                  bra_state: Optional[TreeTensorNetworkState] = None,
@@ -95,7 +107,7 @@ def contract_any(node_id: str, next_node_id: str, state: TreeTensorNetworkState,
         if len(missing) != 1:
             raise KeyError(f"cannot infer the open leg of {node_id} towards {next_node_id}")
         to = missing[0]
-    return env_block(ket_node, ket, op_node, operator.tensor_in(node_id, E.storage_dtype(precision)),
+    return env_block(ket_node, ket, op_node, operator_tensor(operator, node_id, precision),
                      [dictionary.get((nb, node_id)) if i != to else None
                       for i, nb in enumerate(nbs)], to,
                      bra=None if bra_state is None else bra_state.tensors[node_id],
@@ -230,7 +242,7 @@ def expectation_value(state: TreeTensorNetworkState, operator: TreeTensorNetwork
     node, ket = state[r]
     op_node = operator.nodes[r]
     k = node.nneighbours()
-    tensors = [ket, operator.tensor_in(r, E.storage_dtype(precision)), ket]
+    tensors = [ket, operator_tensor(operator, r, precision), ket]
     labels = [[("a", i) for i in range(k)] + ["p"], _op_labels(node, op_node),
               [("b", i) for i in range(k)] + ["q"]]
     for i, nb in enumerate(node.neighbouring_nodes()):

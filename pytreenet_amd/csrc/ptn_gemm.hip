@@ -32,6 +32,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
 constexpr int64_t SPLITK_MIN_K = 512;
 constexpr int64_t SPLITK_TARGET_BLOCKS = 2048;  // 4 waves of 2 blocks x 256 CUs
+constexpr int64_t LARGE_TILES = 2048;
 
 struct KArgs {
   int64_t M, N, K;
@@ -68,7 +69,10 @@ __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <int MODE, typename TA, typename TB, typename TC>
+// LARGE = 1 is the same kernel instantiated separately for launches with a
+// large tile grid (>= LARGE_TILES tiles, no split-K), so profilers report the
+// dominant H_eff / environment GEMMs apart from the many small ones.
+template <int MODE, typename TA, typename TB, typename TC, int LARGE>
 __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
   typedef typename Acc<MODE>::T T;
   typedef typename Acc<MODE>::V V;
@@ -317,8 +321,12 @@ __global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
 
 template <int MODE, typename TA, typename TB, typename TC>
 int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
-  hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC>), dim3((unsigned)tiles, ksplit),
-                     dim3(NT), 0, s, a);
+  if (ksplit == 1 && tiles >= LARGE_TILES)
+    hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
+                       dim3(NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 0>),
+                       dim3((unsigned)tiles, ksplit), dim3(NT), 0, s, a);
   PTN_LAUNCH_CHECK();
   if (ksplit > 1) {
     int64_t g = (a.M * a.N + 255) / 256;

@@ -99,13 +99,17 @@ class LaunchProfiler:
     """Executed-flop counters and HIP-event timings of gather-GEMM launches.
 
     Events are recorded on the stream the kernels are launched on (torch's
-    current stream) around every launch with at least ``min_flops``; the
-    per-shape averages give the duration of the dominant kernel.
+    current stream) around every launch that the library dispatches to its
+    LARGE instantiation (tile grid >= LARGE_TILES, never split-K): these are
+    the dominant H_eff / environment GEMMs, and profilers report them as a
+    kernel of their own.
     """
 
-    def __init__(self, min_flops: float = 1e9):
-        self.min_flops = min_flops
+    LARGE_TILES = 2048   # tile-grid size from which the library uses its LARGE instantiation
+
+    def __init__(self):
         self.active = False
+        self.phase = "default"
         self.reset()
 
     def reset(self):
@@ -113,18 +117,26 @@ class LaunchProfiler:
         self.launches = 0
         self.events: Dict[tuple, list] = {}
 
-    def start(self):
-        self.reset()
+    def start(self, phase: str = "default", reset: bool = True):
+        if reset:
+            self.reset()
+        self.phase = phase
         self.active = True
 
     def stop(self):
         self.active = False
 
-    def summary(self) -> List[dict]:
+    def summary(self, phase: Optional[str] = None, compute: Optional[str] = None) -> List[dict]:
+        """Per-shape rows for LARGE launches, optionally of one phase / precision."""
         torch.cuda.synchronize()
-        rows = []
+        agg: Dict[tuple, list] = {}
         for key, evs in self.events.items():
-            ms = [a.elapsed_time(b) for a, b in evs]
+            if (phase is not None and key[4] != phase) or (compute is not None and
+                                                           key[3] != compute):
+                continue
+            agg.setdefault(key[:4], []).extend(a.elapsed_time(b) for a, b in evs)
+        rows = []
+        for key, ms in agg.items():
             rows.append({"M": key[0], "N": key[1], "K": key[2], "compute": key[3],
                          "flops_per_launch": 8.0 * key[0] * key[1] * key[2],
                          "launches": len(ms), "total_ms": float(sum(ms)),
@@ -216,24 +228,40 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     M = math.prod(mdims)
     N = math.prod(ndims)
     K = math.prod(kdims) if kdims else 1
-    if not kdims:
-        # outer product: a single K index with zero offsets
-        pass
+    ka = _inner_is_unit(kdims, key[4]) and not _inner_is_unit(mdims, key[3])
+    kb = _inner_is_unit(kdims, key[5]) and not _inner_is_unit(ndims, key[6])
+    _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate, alpha)
+    return out
+
+
+def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, accumulate,
+            alpha):
+    """One ptn_gemm_gather launch from device offset tables
+    ptrs = (a_off_m, a_off_k, b_off_k, b_off_n, c_off_m, c_off_n).
+
+    The operand with the longer free dimension is put on the M side (64-row
+    tiles waste less on a short N than on a short M)."""
+    if M < N and N >= 64:
+        A, B = B, A
+        M, N = N, M
+        kmaj_a, kmaj_b = kmaj_b, kmaj_a
+        conj_a, conj_b = conj_b, conj_a
+        ptrs = (ptrs[3], ptrs[2], ptrs[1], ptrs[0], ptrs[5], ptrs[4])
     d = GemmDesc()
     d.M, d.N, d.K = M, N, K
     d.A = A.data_ptr()
     d.a_off_m, d.a_off_k = ptrs[0], ptrs[1]
     d.a_dtype = _DTYPE_CODE[A.dtype]
     d.a_conj = int(conj_a)
-    d.a_kmajor = int(_inner_is_unit(kdims, key[4]) and not _inner_is_unit(mdims, key[3]))
+    d.a_kmajor = int(kmaj_a)
     d.B = B.data_ptr()
     d.b_off_k, d.b_off_n = ptrs[2], ptrs[3]
     d.b_dtype = _DTYPE_CODE[B.dtype]
     d.b_conj = int(conj_b)
-    d.b_kmajor = int(_inner_is_unit(kdims, key[5]) and not _inner_is_unit(ndims, key[6]))
-    d.C = out.data_ptr()
+    d.b_kmajor = int(kmaj_b)
+    d.C = C.data_ptr()
     d.c_off_m, d.c_off_n = ptrs[4], ptrs[5]
-    d.c_dtype = _DTYPE_CODE[out.dtype]
+    d.c_dtype = _DTYPE_CODE[C.dtype]
     d.beta_one = int(accumulate)
     alpha = complex(alpha)
     d.alpha_re, d.alpha_im = alpha.real, alpha.imag
@@ -246,18 +274,18 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
         fl = 8.0 * M * N * K
         prof.flops += fl
         prof.launches += 1
-        if fl >= prof.min_flops:
+        tiles = -(-M // 64) * -(-N // 64)
+        if tiles >= prof.LARGE_TILES:
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
             _lib.check(lib.ptn_gemm_gather(ctypes.c_void_p(stream_handle()), ctypes.byref(d)),
                        "ptn_gemm_gather")
             ev1.record()
-            prof.events.setdefault((M, N, K, precision), []).append((ev0, ev1))
-            return out
+            prof.events.setdefault((M, N, K, precision, prof.phase), []).append((ev0, ev1))
+            return
     _lib.check(lib.ptn_gemm_gather(ctypes.c_void_p(stream_handle()), ctypes.byref(d)),
                "ptn_gemm_gather")
-    return out
 
 
 def permute_copy(src: torch.Tensor, perm: Sequence[int], out: Optional[torch.Tensor] = None,
@@ -310,6 +338,226 @@ def zeros(shape, dtype=torch.complex128, device=None) -> torch.Tensor:
 # --------------------------------------------------------------------------------------
 # network contraction
 # --------------------------------------------------------------------------------------
+class BlockSparse:
+    """A tensor whose ``block_axes`` only take the index tuples listed in ``tuples``.
+
+    The data are stored compactly as ``data[b, dense...]`` (block index first,
+    the remaining axes in their logical order).  A TTNO node tensor is of this
+    kind: only a few combinations of its virtual (bond-state) indices carry a
+    non-zero physical operator, e.g. 12 of the 125 at the root of a ternary XYZ
+    tree.  Contractions skip the absent blocks exactly.
+    """
+
+    def __init__(self, data: torch.Tensor, block_axes: Sequence[int], tuples: np.ndarray,
+                 shape: Sequence[int]):
+        self.data = data
+        self.block_axes = tuple(block_axes)
+        self.tuples = np.ascontiguousarray(np.asarray(tuples, dtype=np.int64).reshape(
+            len(tuples), len(self.block_axes)))
+        self.shape = tuple(shape)
+        self.tkey = hash(self.tuples.tobytes()) ^ hash(self.tuples.shape)
+
+    def dim(self) -> int:
+        return len(self.shape)
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    @staticmethod
+    def from_dense(t: torch.Tensor, block_axes: Sequence[int]) -> "BlockSparse":
+        block_axes = list(block_axes)
+        host = t.detach().cpu().numpy()
+        rest = [a for a in range(host.ndim) if a not in block_axes]
+        moved = np.transpose(host, block_axes + rest)
+        bshape = moved.shape[:len(block_axes)]
+        flat = moved.reshape(int(np.prod(bshape)) if bshape else 1, -1)
+        nz = np.nonzero(np.any(flat != 0, axis=1))[0]
+        if len(nz) == 0:
+            nz = np.array([0])
+        tuples = np.stack(np.unravel_index(nz, bshape), axis=1) if bshape else \
+            np.zeros((1, 0), dtype=np.int64)
+        data = torch.as_tensor(np.ascontiguousarray(
+            flat[nz].reshape((len(nz),) + moved.shape[len(block_axes):]))).to(t.device)
+        return BlockSparse(data.to(t.dtype), block_axes, tuples, host.shape)
+
+
+class _Blk:
+    """Block-sparse intermediate of a network contraction."""
+
+    def __init__(self, data, dense_labels, blabels, tuples, bdims, conj=False, tkey=None,
+                 source=None):
+        self.data = data
+        self.source = source      # the BlockSparse it was made from (None: intermediate)
+        self.dense_labels = tuple(dense_labels)
+        self.blabels = tuple(blabels)
+        self.tuples = tuples
+        self.bdims = dict(bdims)
+        self.conj = conj
+        self.tkey = tkey if tkey is not None else hash(tuples.tobytes()) ^ hash(tuples.shape)
+
+
+_BLK_CACHE: "OrderedDict[tuple, list]" = OrderedDict()
+
+
+def _contract_block(X: _Blk, Y: torch.Tensor, ly: Sequence[Label], conj_y: bool,
+                    res_labels: Optional[Sequence[Label]], precision: str,
+                    out: Optional[torch.Tensor], out_dtype: torch.dtype):
+    """Block-sparse X times dense Y; one gather-GEMM per output block group."""
+    ly = tuple(ly)
+    dims: Dict[Label, int] = dict(X.bdims)
+    dims.update(zip(X.dense_labels, X.data.shape[1:]))
+    for l, d in zip(ly, Y.shape):
+        if dims.setdefault(l, d) != d:
+            raise ValueError(f"dimension mismatch on label {l!r}")
+    k_blk = [l for l in X.blabels if l in ly]
+    m_blk = [l for l in X.blabels if l not in ly]
+    k_d = [l for l in X.dense_labels if l in ly]
+    m_d = [l for l in X.dense_labels if l not in ly]
+    n_l = [l for l in ly if l not in X.dense_labels and l not in X.blabels]
+    cols_k = [X.blabels.index(l) for l in k_blk]
+    cols_m = [X.blabels.index(l) for l in m_blk]
+    if cols_m:
+        G, inv = np.unique(X.tuples[:, cols_m], axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+    else:
+        G, inv = np.zeros((1, 0), dtype=np.int64), np.zeros(len(X.tuples), dtype=np.int64)
+    dev = X.data.device
+    block_out = bool(m_blk) and res_labels is None
+    if block_out:
+        odt = out_dtype
+        res = tuple(m_d + n_l)
+        data = torch.empty((len(G),) + tuple(dims[l] for l in res), dtype=odt, device=dev)
+        o_strides = dict(zip(res, data.stride()[1:]))
+        o_blk_stride = data.stride(0)
+        target = data
+    else:
+        res = tuple(res_labels) if res_labels is not None else tuple(m_d + n_l)
+        if sorted(map(repr, res)) != sorted(map(repr, m_d + m_blk + n_l)):
+            raise ValueError(f"output labels {res} do not match {m_d + m_blk + n_l}")
+        if out is None:
+            out = torch.empty(tuple(dims[l] for l in res), dtype=out_dtype, device=dev)
+        full = math.prod(dims[l] for l in m_blk) if m_blk else 1
+        if len(G) < full:
+            fill_(out, 0.0)
+        o_strides = dict(zip(res, out.stride()))
+        target = out
+    sx = dict(zip(X.dense_labels, X.data.stride()[1:]))
+    sx_blk = X.data.stride(0)
+    sy = dict(zip(ly, Y.stride()))
+    key = (X.data.shape, X.data.stride(), X.dense_labels, X.blabels, X.tkey, tuple(Y.shape),
+           tuple(Y.stride()), ly, tuple(target.shape), tuple(target.stride()), res, block_out)
+    nj = math.prod(dims[l] for l in m_d)
+    ni = math.prod(dims[l] for l in k_d)
+    R = math.prod(dims[l] for l in n_l)
+    if (nj <= 4 and ni <= 4 and len(X.tuples) <= 128 and len(G) <= 128 and R >= 4096):
+        _block_apply(X, Y, conj_y, dims, m_d, k_d, n_l, k_blk, cols_k, cols_m, G, inv, sy,
+                     target, o_strides, block_out, o_blk_stride if block_out else 0, key)
+    else:
+        groups = _BLK_CACHE.get(key)
+        if groups is None:
+            a_m = _offsets([dims[l] for l in m_d], [sx[l] for l in m_d])
+            kd_x = _offsets([dims[l] for l in k_d], [sx[l] for l in k_d])
+            kd_y = _offsets([dims[l] for l in k_d], [sy[l] for l in k_d])
+            b_n = _offsets([dims[l] for l in n_l], [sy[l] for l in n_l])
+            c_m0 = _offsets([dims[l] for l in m_d], [o_strides[l] for l in m_d])
+            c_n = _offsets([dims[l] for l in n_l], [o_strides[l] for l in n_l])
+            ystr_k = np.array([sy[l] for l in k_blk], dtype=np.int64)
+            ostr_m = np.array([o_strides[l] for l in m_blk], dtype=np.int64) if not block_out \
+                else None
+            host_groups = []
+            for g in range(len(G)):
+                P = np.nonzero(inv == g)[0].astype(np.int64)
+                a_k = (P[:, None] * sx_blk + kd_x[None, :]).reshape(-1)
+                yb = X.tuples[P][:, cols_k] @ ystr_k if cols_k else np.zeros(len(P), np.int64)
+                b_k = (yb[:, None] + kd_y[None, :]).reshape(-1)
+                shift = g * o_blk_stride if block_out else int(G[g] @ ostr_m) if cols_m else 0
+                host_groups.append((a_m, a_k, b_k, b_n, c_m0 + shift, c_n))
+            groups = []
+            for tabs in host_groups:
+                _, ptrs = _CACHE.get(("blk", key, len(groups)), lambda t=tabs: list(t), dev)
+                groups.append((ptrs, len(tabs[0]), len(tabs[1]), len(tabs[3])))
+            _BLK_CACHE[key] = groups
+            if len(_BLK_CACHE) > 4096:
+                _BLK_CACHE.popitem(last=False)
+        _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, target)
+    if block_out:
+        return _Blk(data, res, m_blk, G, {l: X.bdims[l] for l in m_blk})
+    return out
+
+
+def _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, target):
+    ka = _inner_is_unit([dims[l] for l in k_d], [sx[l] for l in k_d]) and \
+        not _inner_is_unit([dims[l] for l in m_d], [sx[l] for l in m_d])
+    kb = _inner_is_unit([dims[l] for l in k_d], [sy[l] for l in k_d]) and \
+        not _inner_is_unit([dims[l] for l in n_l], [sy[l] for l in n_l])
+    for ptrs, M, K, N in groups:
+        _launch(X.data, Y, target, ptrs, M, N, K, ka, kb, X.conj, conj_y, precision, False,
+                1.0)
+
+
+_BAPPLY_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()
+
+
+def _block_apply(X, Y, conj_y, dims, m_d, k_d, n_l, k_blk, cols_k, cols_m, G, inv, sy, target,
+                 o_strides, block_out, o_blk_stride, key):
+    """The TTNO tensor (few small blocks) meets a large tensor: one pass of
+    ``ptn_block_apply`` (each row read once, every live output block written)."""
+    dev = X.data.device
+    hit = _BAPPLY_CACHE.get(key)
+    if hit is None:
+        order = np.argsort(inv, kind="stable")
+        counts = np.bincount(inv, minlength=len(G))
+        g_begin = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        ystr_k = np.array([sy[l] for l in k_blk], dtype=np.int64)
+        y_t = (X.tuples[order][:, cols_k] @ ystr_k) if cols_k else np.zeros(len(order), np.int64)
+        if block_out:
+            o_g = np.arange(len(G), dtype=np.int64) * o_blk_stride
+        elif cols_m:
+            m_blk = [X.blabels[c] for c in cols_m]
+            o_g = G @ np.array([o_strides[l] for l in m_blk], dtype=np.int64)
+        else:
+            o_g = np.zeros(1, dtype=np.int64)
+        dl = list(X.dense_labels)
+        perm = [0] + [1 + dl.index(l) for l in m_d] + [1 + dl.index(l) for l in k_d]
+        nj = math.prod(dims[l] for l in m_d)
+        ni = math.prod(dims[l] for l in k_d)
+        y_r = _offsets([dims[l] for l in n_l], [sy[l] for l in n_l])
+        o_r = _offsets([dims[l] for l in n_l], [o_strides[l] for l in n_l])
+        small = np.concatenate([y_t.astype(np.int64), o_g.astype(np.int64),
+                                g_begin.astype(np.int64)])
+        big = torch.from_numpy(np.concatenate([y_r, o_r])).to(dev)
+        sm = torch.from_numpy(small).to(dev)
+        gb32 = torch.from_numpy(g_begin).to(dev)
+        y_i = _offsets([dims[l] for l in k_d], [sy[l] for l in k_d])
+        o_j = _offsets([dims[l] for l in m_d], [o_strides[l] for l in m_d])
+        hit = (big, sm, gb32, tuple(perm), order, len(y_r), len(order), len(G), nj, ni, y_i, o_j,
+               len(y_t))
+        _BAPPLY_CACHE[key] = hit
+        if len(_BAPPLY_CACHE) > 1024:
+            _BAPPLY_CACHE.popitem(last=False)
+    big, sm, gb32, perm, order, R, nt, ng, nj, ni, y_i, o_j, nyt = hit
+    # the operator values: cached on the persistent BlockSparse, never on structure alone
+    wkey = (perm, order.tobytes())
+    w = None
+    if X.source is not None:
+        wc = X.source.__dict__.setdefault("_apply_cache", {})
+        w = wc.get(wkey)
+    if w is None:
+        w = X.data.permute(list(perm)).reshape(len(X.tuples), nj, ni)[
+            torch.as_tensor(order).to(dev)].to(torch.complex128).contiguous()
+        if X.source is not None:
+            wc[wkey] = w
+    arr = ctypes.c_int64 * 4
+    vp = ctypes.c_void_p
+    _lib.check(_lib.load().ptn_block_apply(
+        vp(stream_handle()), R, nt, ng, nj, ni, vp(Y.data_ptr()), _DTYPE_CODE[Y.dtype],
+        vp(big.data_ptr()), vp(sm.data_ptr()), arr(*[int(v) for v in y_i] + [0] * (4 - ni)),
+        int(conj_y), vp(gb32.data_ptr()), vp(w.data_ptr()), int(X.conj), vp(target.data_ptr()),
+        _DTYPE_CODE[target.dtype], vp(big.data_ptr() + 8 * R), vp(sm.data_ptr() + 8 * nyt),
+        arr(*[int(v) for v in o_j] + [0] * (4 - nj))), "ptn_block_apply")
+
+
 @dataclass
 class NetPlan:
     steps: List[Tuple[int, int, Tuple[Label, ...]]]   # (slot_a, slot_b, result labels)
@@ -325,11 +573,17 @@ def _merge_cost(l1: frozenset, l2: frozenset, dims: Dict[Label, int]) -> int:
 
 
 def plan_network(labels: Sequence[Sequence[Label]], dims: Dict[Label, int],
-                 out_labels: Sequence[Label]) -> NetPlan:
-    """Optimal pairwise order (min total flops) by DP over subsets."""
+                 out_labels: Sequence[Label], block: Optional[tuple] = None) -> NetPlan:
+    """Optimal pairwise order (min total flops) by DP over subsets.
+
+    ``block = (i, block_labels, tuples)`` marks tensor i as block sparse: a merge
+    involving it costs 8 * M_dense * K_dense * N * (live blocks) and its
+    intermediates hold only the live blocks.
+    """
     n = len(labels)
     key = (tuple(tuple(l) for l in labels), tuple(sorted((repr(k), v) for k, v in dims.items())),
-           tuple(out_labels))
+           tuple(out_labels),
+           None if block is None else (block[0], tuple(block[1]), block[2].tobytes()))
     hit = _PLANS.get(key)
     if hit is not None:
         return hit
@@ -350,6 +604,39 @@ def plan_network(labels: Sequence[Sequence[Label]], dims: Dict[Label, int],
 
     full = (1 << n) - 1
     leg_of = {m: legs(m) for m in range(1, full + 1)}
+    bi = -1
+    bset: frozenset = frozenset()
+    if block is not None:
+        bi, blabels, btuples = block[0], list(block[1]), block[2]
+        bset = frozenset(blabels)
+        bcol = {l: i for i, l in enumerate(blabels)}
+    nb_memo: Dict[frozenset, int] = {}
+
+    def nblocks(open_blk: frozenset) -> int:
+        hit = nb_memo.get(open_blk)
+        if hit is None:
+            cols = sorted(bcol[l] for l in open_blk)
+            hit = len(np.unique(btuples[:, cols], axis=0)) if cols else 1
+            nb_memo[open_blk] = hit
+        return hit
+
+    def prod(ls) -> int:
+        return math.prod(dims[l] for l in ls)
+
+    def size_of(mask: int) -> int:
+        L = leg_of[mask]
+        if bi >= 0 and mask >> bi & 1:
+            return prod(L - bset) * nblocks(L & bset)
+        return prod(L)
+
+    def merge_cost(u: int, v: int) -> float:
+        Lu, Lv = leg_of[u], leg_of[v]
+        if bi >= 0 and (u >> bi & 1 or v >> bi & 1):
+            if v >> bi & 1:
+                Lu, Lv = Lv, Lu
+            du = Lu - bset
+            return 8.0 * prod(du - Lv) * prod(du & Lv) * prod(Lv - Lu) * nblocks(Lu & bset)
+        return 8.0 * prod(Lu | Lv)
     best: Dict[int, Tuple[float, int, Optional[tuple]]] = {}
     for i in range(n):
         best[1 << i] = (0.0, 0, None)
@@ -363,9 +650,9 @@ def plan_network(labels: Sequence[Sequence[Label]], dims: Dict[Label, int],
             if sub < other and sub in best and other in best:
                 shared = leg_of[sub] & leg_of[other]
                 if shared or mask == full:
-                    c = 8.0 * _merge_cost(leg_of[sub], leg_of[other], dims)
+                    c = merge_cost(sub, other)
                     tot = best[sub][0] + best[other][0] + c
-                    size = math.prod(dims[l] for l in leg_of[mask])
+                    size = size_of(mask)
                     peak = max(best[sub][1], best[other][1], size)
                     if cand is None or (tot, peak) < (cand[0], cand[1]):
                         cand = (tot, peak, (sub, other))
@@ -396,39 +683,67 @@ def plan_network(labels: Sequence[Sequence[Label]], dims: Dict[Label, int],
     return plan
 
 
-def contract_network(tensors: Sequence[torch.Tensor], labels: Sequence[Sequence[Label]],
+def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
                      out_labels: Sequence[Label], *, conj: Optional[Sequence[bool]] = None,
                      precision: str = FP64, out: Optional[torch.Tensor] = None,
                      out_dtype: torch.dtype = torch.complex128) -> torch.Tensor:
-    """Contract a whole network; ``conj[i]`` conjugates tensor i on load."""
+    """Contract a whole network; ``conj[i]`` conjugates tensor i on load.
+
+    At most one tensor may be a ``BlockSparse`` (the TTNO node tensor)."""
     n = len(tensors)
     conj = list(conj) if conj is not None else [False] * n
     dims: Dict[Label, int] = {}
-    for t, l in zip(tensors, labels):
+    block = None
+    for i, (t, l) in enumerate(zip(tensors, labels)):
         if len(l) != t.dim():
             raise ValueError("label count does not match tensor rank")
         for x, d in zip(l, t.shape):
             if dims.setdefault(x, d) != d:
                 raise ValueError(f"dimension mismatch on label {x!r}")
+        if isinstance(t, BlockSparse):
+            if block is not None:
+                raise ValueError("at most one block-sparse tensor per network")
+            block = (i, [l[a] for a in t.block_axes], t.tuples)
     if n == 1:
+        if block is not None:
+            raise ValueError("a lone block-sparse tensor cannot be contracted")
         perm = [list(labels[0]).index(l) for l in out_labels]
         return permute_copy(tensors[0], perm, out=out, conj=conj[0], dtype=out_dtype)
-    plan = plan_network(labels, dims, out_labels)
-    slots: List[Optional[Tuple[torch.Tensor, Tuple[Label, ...], bool]]] = [
-        (t, tuple(l), c) for t, l, c in zip(tensors, labels, conj)]
+    plan = plan_network(labels, dims, out_labels, block)
+    slots: list = []
+    for i, (t, l, c) in enumerate(zip(tensors, labels, conj)):
+        if isinstance(t, BlockSparse):
+            dl = [x for a, x in enumerate(l) if a not in t.block_axes]
+            bl = [l[a] for a in t.block_axes]
+            slots.append(_Blk(t.data, dl, bl, t.tuples, {x: dims[x] for x in bl}, conj=c,
+                              tkey=t.tkey, source=t))
+        else:
+            slots.append((t, tuple(l), c))
     inter = storage_dtype(precision)
     for idx, (sa, sb, res) in enumerate(plan.steps):
-        ta, la_, ca = slots[sa]
-        tb, lb_, cb = slots[sb]
+        A_, B_ = slots[sa], slots[sb]
         last = idx == len(plan.steps) - 1
-        if res is None:
-            res = tuple([l for l in la_ if l not in lb_] + [l for l in lb_ if l not in la_])
-        # a label shared by A and B but also needed outside is not contracted:
-        # networks here never have such hyper-edges.
-        r = contract(ta, la_, tb, lb_, res, conj_a=ca, conj_b=cb, precision=precision,
-                     out=out if last else None,
-                     out_dtype=out_dtype if last else inter)
-        slots.append((r, res, False))
+        if isinstance(B_, _Blk):
+            A_, B_ = B_, A_
+        if isinstance(A_, _Blk):
+            tb, lb_, cb = B_
+            r = _contract_block(A_, tb, lb_, cb, res if last else None, precision,
+                                out if last else None, out_dtype if last else inter)
+            slots.append(r if isinstance(r, _Blk) else
+                         (r, tuple(res) if last else
+                          tuple([l for l in A_.dense_labels if l not in lb_] +
+                                [l for l in lb_ if l not in A_.dense_labels
+                                 and l not in A_.blabels]), False))
+        else:
+            ta, la_, ca = A_
+            tb, lb_, cb = B_
+            if res is None:
+                res = tuple([l for l in la_ if l not in lb_] + [l for l in lb_ if l not in la_])
+            r = contract(ta, la_, tb, lb_, res, conj_a=ca, conj_b=cb, precision=precision,
+                         out=out if last else None,
+                         out_dtype=out_dtype if last else inter)
+            slots.append((r, res, False))
         slots[sa] = None
         slots[sb] = None
-    return slots[-1][0]
+    final = slots[-1]
+    return final[0]

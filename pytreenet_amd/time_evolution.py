@@ -31,7 +31,8 @@ import torch
 
 from . import engine as E
 from .contractions import (SandwichCache, bond_apply, contract_any,
-                           contract_ket_ham_with_envs, expectation_value, scalar_product)
+                           contract_ket_ham_with_envs, expectation_value, operator_tensor,
+                           scalar_product)
 from .krylov import EvoDirection, TimeEvoMode, krylov_exp_action
 from .splitting import (SVDParameters, SplitMode, _colmajor_panel, _cstrides, qr_colmajor,
                         svd_colmajor, truncate_singular_values)
@@ -307,7 +308,7 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
     def _update_site(self, node_id: str, time_step_factor: float = 1):
         node, tensor = self.state[node_id]
         op_node = self.hamiltonian.nodes[node_id]
-        op = self.hamiltonian.tensor_in(node_id, E.storage_dtype(self.precision))
+        op = operator_tensor(self.hamiltonian, node_id, self.precision)
         cache = self.partial_tree_cache
 
         def action(y):

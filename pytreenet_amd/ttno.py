@@ -50,6 +50,18 @@ class TreeTensorNetworkOperator(TreeTensorNetwork):
     def __init__(self):
         super().__init__()
         self._lowp: Dict[str, torch.Tensor] = {}
+        self._blocks: Dict[tuple, object] = {}
+
+    def block_tensor(self, node_id: str, dtype: torch.dtype):
+        """The node tensor in block-sparse form over its virtual (bond-state) legs."""
+        from .engine import BlockSparse
+        t = self.tensor_in(node_id, dtype)
+        hit = self._blocks.get((node_id, dtype))
+        if hit is None or hit.shape != tuple(t.shape) or hit.source is not t:
+            hit = BlockSparse.from_dense(t, range(self._nodes[node_id].nneighbours()))
+            hit.source = t
+            self._blocks[(node_id, dtype)] = hit
+        return hit
 
     def tensor_in(self, node_id: str, dtype: torch.dtype) -> torch.Tensor:
         """The node tensor in ``dtype`` (complex64 copies are cached)."""

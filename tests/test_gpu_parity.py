@@ -176,3 +176,87 @@ def test_mixed_precision_error_bound(gpu):
     e64 = ptn.expectation_value(a64.state, a64.hamiltonian)
     e32 = ptn.expectation_value(a32.state, a32.hamiltonian)
     assert abs(e64 - e32) < 1e-4 * abs(e64)
+
+
+def test_block_sparse_ttno_matches_dense(gpu):
+    """H_eff, environments and energy with the block-sparse TTNO equal the dense ones."""
+    _say("block-sparse vs dense TTNO")
+    from pytreenet_amd import contractions as PC
+    st, H, topo, tensors, W = _setup(3, 2, 6, 9)
+    c = "site1"
+    envs = {}
+    dist = topo.distance_to_node(c)
+    cache_b, cache_d = ptn.PartialTreeCachDict(), ptn.PartialTreeCachDict()
+    try:
+        for d in range(max(dist.values()), 0, -1):
+            for n in [m for m in dist if dist[m] == d]:
+                to = min(topo.neighbours(n), key=lambda x: dist[x])
+                PC.USE_BLOCK_SPARSE = True
+                cache_b.add_entry(n, to, ptn.contract_any(n, to, st, H, cache_b))
+                PC.USE_BLOCK_SPARSE = False
+                cache_d.add_entry(n, to, ptn.contract_any(n, to, st, H, cache_d))
+                eb, ed = cache_b.get_entry(n, to).cpu().numpy(), cache_d.get_entry(n, to).cpu().numpy()
+                np.testing.assert_allclose(eb, ed, rtol=0, atol=REL * np.abs(ed).max())
+        node, x = st[c]
+        yb = ptn.contract_ket_ham_with_envs(node, x, H.nodes[c], H.block_tensor(c, x.dtype),
+                                            cache_b).cpu().numpy()
+        yd = ptn.contract_ket_ham_with_envs(node, x, H.nodes[c], H.tensors[c],
+                                            cache_d).cpu().numpy()
+        np.testing.assert_allclose(yb, yd, rtol=0, atol=REL * np.abs(yd).max())
+        PC.USE_BLOCK_SPARSE = True
+        e_b = ptn.expectation_value(st, H)
+        PC.USE_BLOCK_SPARSE = False
+        e_d = ptn.expectation_value(st, H)
+        assert _rel(e_b, e_d) < REL
+    finally:
+        PC.USE_BLOCK_SPARSE = True
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_block_apply_path_matches_dense(gpu, precision):
+    """Root of a ternary tree at chi=16: the TTNO step runs through ptn_block_apply."""
+    _say(f"block apply vs dense {precision}")
+    from pytreenet_amd import contractions as PC
+    st, H, topo, tensors, W = _setup(3, 2, 16, 21)
+    c = "site0"
+    cache = ptn.PartialTreeCachDict()
+    PC.USE_BLOCK_SPARSE = False
+    for n, to in ptn.contractions.caching_path(st, c):
+        cache.add_entry(n, to, ptn.contract_any(n, to, st, H, cache))
+    PC.USE_BLOCK_SPARSE = True
+    node, x = st[c]
+    dt = E.storage_dtype(precision)
+    envs = {k: v.to(dt) for k, v in cache.items()}
+    c2 = ptn.PartialTreeCachDict(envs)
+    yb = ptn.contract_ket_ham_with_envs(node, x, H.nodes[c], H.block_tensor(c, dt), c2,
+                                        precision=precision).cpu().numpy()
+    yd = ptn.contract_ket_ham_with_envs(node, x, H.nodes[c], H.tensor_in(c, dt), c2,
+                                        precision=precision).cpu().numpy()
+    tol = REL if precision == "fp64" else 1e-5
+    np.testing.assert_allclose(yb, yd, rtol=0, atol=tol * np.abs(yd).max())
+    for to in range(3):
+        eb = PC.env_block(node, x, H.nodes[c], H.block_tensor(c, dt),
+                          [None if i == to else envs[(nb, c)] for i, nb in
+                           enumerate(node.neighbouring_nodes())], to,
+                          precision=precision).cpu().numpy()
+        ed = PC.env_block(node, x, H.nodes[c], H.tensor_in(c, dt),
+                          [None if i == to else envs[(nb, c)] for i, nb in
+                           enumerate(node.neighbouring_nodes())], to,
+                          precision=precision).cpu().numpy()
+        np.testing.assert_allclose(eb, ed, rtol=0, atol=tol * np.abs(ed).max())
+
+
+def test_tdvp_block_apply_shared_structure_matches_oracle(gpu):
+    """Binary 31-node tree at chi=32, random XYZ: both level-1 nodes (bonds 32,32,32)
+    take the block-apply TTNO step with the same block structure but different
+    couplings (regression: operator values must not be shared between them)."""
+    _say("tdvp second1 binary depth 4 chi32 xyz")
+    alg, otd, ops = _run_pair("second1", 2, 4, 32, 31, "xyz", 0.05, 1)
+    for step in range(1):
+        alg.run_one_time_step()
+        otd.run_one_time_step()
+        e = ptn.expectation_value(alg.state, alg.hamiltonian)
+        assert _rel(e, otd.energy()) < REL, step
+        for nid in ops:
+            z = alg.state.single_site_operator_expectation_value(nid, PAULI_Z)
+            assert abs(z - otd.expect_site(nid, PAULI_Z)) < REL * abs(otd.energy()), nid

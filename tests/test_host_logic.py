@@ -138,3 +138,27 @@ def test_product_refuses_cpu_tensors():
     a = torch.zeros((2, 2), dtype=torch.complex128)
     with pytest.raises(ptn.PtnBackendError):
         E.contract(a, "ij", a, "jk", "ik")
+
+
+def test_network_planner_block_sparse_root_cost():
+    """With the XYZ TTNO's 12 live bond-state blocks the root apply needs 1312.1 GFLOP."""
+    import torch
+    from pytreenet_amd.engine import BlockSparse, plan_network
+    from pytreenet_amd.random_states import bond_dims, build_state
+    ids, parent, children = complete_tree(3, 3)
+    b = bond_dims(ids, parent, children, 256)
+    tensors = {n: np.zeros(([b[n]] if parent[n] else []) + [b[x] for x in children[n]] + [2])
+               for n in ids}
+    st = build_state(ids, parent, children, tensors, device="cpu")
+    H = ptn.random_xyz_model(st, 1)
+    bs = BlockSparse.from_dense(H.tensors["site0"], range(3))
+    assert len(bs.tuples) == 12
+    labels = [[("a", i) for i in range(3)] + ["p"], [("w", i) for i in range(3)] + ["q", "p"]] + \
+             [[("a", i), ("w", i), ("b", i)] for i in range(3)]
+    dims = {("a", i): 256 for i in range(3)}
+    dims.update({("b", i): 256 for i in range(3)})
+    dims.update({("w", i): 5 for i in range(3)})
+    dims.update({"p": 2, "q": 2})
+    plan = plan_network(labels, dims, [("b", i) for i in range(3)] + ["q"],
+                        (1, [("w", i) for i in range(3)], bs.tuples))
+    assert abs(plan.flops / 1e9 - 1312.1) < 0.1
