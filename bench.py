@@ -31,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
+# measured HBM bytes per LARGE FP64 gather-GEMM launch (rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE on this command, scaled by the copy calibration; see profiles/)
+TRAFFIC_BYTES_PER_LAUNCH = 5.43e8
 METRIC = "TDVP sweeps/s & H_eff TFLOP/s at χ=256, mixed vs FP64 GPU at ΔE≤1e-8"
 
 
@@ -247,7 +250,11 @@ def main():
         ms = sum(k["total_ms"] for k in kern)
         ach = fl / (ms * 1e-3)
         roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12,
-                    "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
+                    "unit": "TFLOP/s", "frac": ach / PEAK["fp64"],
+                    # HBM bytes per launch of this kernel: rocprofv3 FETCH_SIZE x 2.0 +
+                    # WRITE_SIZE x 1.0 (scales from --calibrate-copy), averaged over its
+                    # launches in one bench.py process
+                    "traffic": TRAFFIC_BYTES_PER_LAUNCH,
                     "kernel": "gemm_gather_kernel<F64_4M, c128, c128, c128, LARGE=1>",
                     "flops_per_launch": fl / n, "avg_ms": ms / n, "launches": n,
                     "timed_steps": args.steps,

@@ -151,6 +151,19 @@ def combine(Q: torch.Tensor, coeffs, out: torch.Tensor, nvec: Optional[int] = No
     return out
 
 
+def lincomb(tensors, coeffs, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """out = sum_i coeffs[i] * tensors[i] (same shape, complex128), on the device."""
+    t0 = tensors[0]
+    n = t0.numel()
+    buf = torch.empty((len(tensors), n), dtype=torch.complex128, device=t0.device)
+    for i, t in enumerate(tensors):
+        E.strided_copy([n], _flat(t), [1], buf[i], [1])
+    if out is None:
+        out = torch.empty_like(t0, dtype=torch.complex128)
+    combine(buf, coeffs, out.reshape(-1))
+    return out
+
+
 def small_propagator(t_mat: np.ndarray, dt: float, exp_factor: complex,
                      hermitian: bool = True) -> np.ndarray:
     """First column of exp(exp_factor * dt * T) for the m x m projection T (host FP64)."""

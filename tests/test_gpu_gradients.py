@@ -1,0 +1,124 @@
+"""Differentiation form: adjoint sandwich gradients vs finite differences and closed forms."""
+import numpy as np
+import pytest
+import torch
+
+import pytreenet_amd as ptn
+from oracle import contract as OC
+from oracle import dense as OD
+from oracle.tdvp import OracleState
+from pytreenet_amd import gradients as G
+from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
+from pytreenet_amd.ttno import PAULI_X, build_edge_ttno_tensors
+from tests.helpers import host_tensors, topology_from_ttn
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+
+
+def _problem(seed=3, chi=4, depth=2):
+    ids, parent, children = complete_tree(2, depth)
+    tensors = random_state_tensors(ids, parent, children, chi, seed)
+    st = build_state(ids, parent, children, tensors, device="cuda")
+    build, n = G.tfim_coupling_builder(st, 0.5)
+    model = G.CouplingTTNO(st, build, n)
+    J = np.random.default_rng(seed).uniform(0.5, 1.5, n)
+    return st, model, J, tensors
+
+
+def _oracle_energy(topo, tensors, W):
+    return (OC.expectation_value(topo, tensors, W) / OC.overlap(topo, tensors, tensors)).real
+
+
+def test_site_and_coupling_gradients_vs_finite_differences(gpu):
+    _say("gradients vs central FD (h=1e-6)")
+    st, model, J, tensors = _problem()
+    H = model.ttno(J)
+    topo = topology_from_ttn(st)
+    energy, gsite, gop, den = G.rayleigh_quotient_gradients(st, H)
+    W = host_tensors(H)
+    assert abs(energy - _oracle_energy(topo, tensors, W)) < 1e-12 * abs(energy)
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for nid in ["site0", "site1", "site5"]:
+        g = gsite[nid].cpu().numpy()
+        for _ in range(3):
+            idx = tuple(rng.integers(0, s) for s in tensors[nid].shape)
+            for part, expect in [(1.0, 2 * g[idx].real), (1j, 2 * g[idx].imag)]:
+                tp, tm = dict(tensors), dict(tensors)
+                tp[nid] = tensors[nid].copy()
+                tm[nid] = tensors[nid].copy()
+                tp[nid][idx] += h * part
+                tm[nid][idx] -= h * part
+                fd = (_oracle_energy(topo, tp, W) - _oracle_energy(topo, tm, W)) / (2 * h)
+                assert abs(fd - expect) < 1e-6 * max(1.0, abs(expect)), (nid, idx, fd, expect)
+    gJ = model.coupling_gradient(gop)
+    for e in range(model.n):
+        Jp, Jm = J.copy(), J.copy()
+        Jp[e] += h
+        Jm[e] -= h
+        Wp = build_edge_ttno_tensors(st, *model.builder(Jp))
+        Wm = build_edge_ttno_tensors(st, *model.builder(Jm))
+        fd = (_oracle_energy(topo, tensors, Wp) - _oracle_energy(topo, tensors, Wm)) / (2 * h)
+        assert abs(fd - gJ[e]) < 1e-6 * max(1.0, abs(gJ[e])), (e, fd, gJ[e])
+
+
+def test_coupling_gradient_closed_form(gpu):
+    """TFIM: dE/dJ_e = -<X_p X_c> / <psi|psi> (dense brute force)."""
+    _say("coupling gradient closed form")
+    st, model, J, tensors = _problem(seed=5)
+    H = model.ttno(J)
+    _, _, gop, _ = G.rayleigh_quotient_gradients(st, H)
+    gJ = model.coupling_gradient(gop)
+    topo = topology_from_ttn(st)
+    psi = OD.state_to_vector(topo, tensors)
+    ids = topo.node_ids
+    edges = [(n.parent, nid) for nid, n in st.nodes.items() if n.parent is not None]
+    for e, (p, c) in enumerate(edges):
+        XX = OD.kron_op(len(ids), {ids.index(p): PAULI_X, ids.index(c): PAULI_X})
+        ref = -(np.vdot(psi, XX @ psi) / np.vdot(psi, psi)).real
+        assert abs(gJ[e] - ref) < 1e-11 * max(1, abs(ref))
+
+
+def test_site_gradient_equals_heff_reference_at_centre(gpu):
+    """Normalised canonical state: at the orthogonality centre N_eff = I, so
+    dE/d conj(A_c) = H_eff A_c - E A_c."""
+    _say("gradient vs H_eff A reference")
+    st, model, J, tensors = _problem(seed=7)
+    H = model.ttno(J)
+    c = "site1"
+    st.canonical_form(c)
+    st.normalise()
+    energy, gsite, _, den = G.rayleigh_quotient_gradients(st, H, with_operator=False)
+    assert abs(den - 1) < 1e-12
+    topo = topology_from_ttn(st)
+    A = host_tensors(st)
+    W = host_tensors(H)
+    ost = OracleState(topo, A)
+    envs = {}
+    dist = topo.distance_to_node(c)
+    for d in range(max(dist.values()), 0, -1):
+        for n in [m for m in dist if dist[m] == d]:
+            to = min(topo.neighbours(n), key=lambda x: dist[x])
+            envs[(n, to)] = OC.env_update(topo, n, to, A[n], W[n], envs)
+    ref = OC.heff_apply(topo, c, A[c], W[c], envs) - energy * A[c]
+    np.testing.assert_allclose(gsite[c].cpu().numpy(), ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_autograd_function_and_descent(gpu):
+    _say("autograd Function + gradient descent")
+    st, model, J, tensors = _problem(seed=11, chi=4)
+    Jt = torch.tensor(J, dtype=torch.float64, device="cuda", requires_grad=True)
+    ts = [st.tensors[n].clone().requires_grad_(True) for n in st.nodes]
+    e = G.RayleighQuotient.apply(model, st, Jt, *ts)
+    e.backward()
+    energy, gsite, gop, _ = G.rayleigh_quotient_gradients(st, model.ttno(J))
+    assert abs(e.item() - energy) < 1e-12 * abs(energy)
+    for t, nid in zip(ts, st.nodes):
+        torch.testing.assert_close(t.grad, 2 * gsite[nid], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(Jt.grad.cpu().numpy(), model.coupling_gradient(gop), atol=1e-12)
+    hist = G.ground_state_descent(st, model.ttno(J), steps=20, lr=0.05)
+    assert hist[-1] < hist[0] - 1e-3
