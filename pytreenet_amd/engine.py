@@ -41,8 +41,25 @@ def storage_dtype(precision: str) -> torch.dtype:
     return torch.complex128 if precision == FP64 else torch.complex64
 
 
+_STREAM: Optional[int] = None
+
+
 def stream_handle() -> int:
-    return torch.cuda.current_stream().cuda_stream
+    """HIP stream the library launches on: torch's current stream, cached.
+
+    Call ``refresh_stream()`` after switching torch streams; the time-evolution
+    drivers refresh it at the start of every step.
+    """
+    global _STREAM
+    if _STREAM is None:
+        _STREAM = torch.cuda.current_stream().cuda_stream
+    return _STREAM
+
+
+def refresh_stream() -> int:
+    global _STREAM
+    _STREAM = torch.cuda.current_stream().cuda_stream
+    return _STREAM
 
 
 def require_device(*ts: torch.Tensor):
@@ -234,6 +251,16 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     return out
 
 
+class _Recorder:
+    """Collects the launches of one network contraction for later replay."""
+
+    def __init__(self):
+        self.ops: list = []
+
+
+_RECORDER: Optional[_Recorder] = None
+
+
 def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, accumulate,
             alpha):
     """One ptn_gemm_gather launch from device offset tables
@@ -268,6 +295,12 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     d.compute = _COMPUTE[precision]
     ws = _splitk_workspace(A.device)
     d.ws, d.ws_bytes = ws.data_ptr(), ws.numel() * ws.element_size()
+    if _RECORDER is not None:
+        _RECORDER.ops.append(("gemm", d, A, B, C, M, N, K, precision))
+    _run_gemm(d, M, N, K, precision)
+
+
+def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
     lib = _lib.load()
     prof = PROFILER
     if prof.active:
@@ -322,6 +355,8 @@ def fill_(t: torch.Tensor, value: complex = 0.0) -> torch.Tensor:
     if not t.is_contiguous():
         raise ValueError("fill_ needs a contiguous tensor")
     v = complex(value)
+    if _RECORDER is not None:
+        _RECORDER.ops.append(("fill", t, v))
     _lib.check(_lib.load().ptn_fill(ctypes.c_void_p(stream_handle()),
                                     ctypes.c_void_p(t.data_ptr()), t.numel(),
                                     _DTYPE_CODE[t.dtype], v.real, v.imag), "ptn_fill")
@@ -398,6 +433,7 @@ class _Blk:
 
 
 _BLK_CACHE: "OrderedDict[tuple, list]" = OrderedDict()
+_GROUPS: Dict[tuple, tuple] = {}
 
 
 def _contract_block(X: _Blk, Y: torch.Tensor, ly: Sequence[Label], conj_y: bool,
@@ -417,11 +453,16 @@ def _contract_block(X: _Blk, Y: torch.Tensor, ly: Sequence[Label], conj_y: bool,
     n_l = [l for l in ly if l not in X.dense_labels and l not in X.blabels]
     cols_k = [X.blabels.index(l) for l in k_blk]
     cols_m = [X.blabels.index(l) for l in m_blk]
-    if cols_m:
-        G, inv = np.unique(X.tuples[:, cols_m], axis=0, return_inverse=True)
-        inv = inv.reshape(-1)
-    else:
-        G, inv = np.zeros((1, 0), dtype=np.int64), np.zeros(len(X.tuples), dtype=np.int64)
+    gk = (X.tkey, tuple(cols_m))
+    hitg = _GROUPS.get(gk)
+    if hitg is None:
+        if cols_m:
+            G, inv = np.unique(X.tuples[:, cols_m], axis=0, return_inverse=True)
+            inv = inv.reshape(-1)
+        else:
+            G, inv = np.zeros((1, 0), dtype=np.int64), np.zeros(len(X.tuples), dtype=np.int64)
+        _GROUPS[gk] = hitg = (G, inv)
+    G, inv = hitg
     dev = X.data.device
     block_out = bool(m_blk) and res_labels is None
     if block_out:
@@ -549,13 +590,24 @@ def _block_apply(X, Y, conj_y, dims, m_d, k_d, n_l, k_blk, cols_k, cols_m, G, in
         if X.source is not None:
             wc[wkey] = w
     arr = ctypes.c_int64 * 4
-    vp = ctypes.c_void_p
-    _lib.check(_lib.load().ptn_block_apply(
-        vp(stream_handle()), R, nt, ng, nj, ni, vp(Y.data_ptr()), _DTYPE_CODE[Y.dtype],
-        vp(big.data_ptr()), vp(sm.data_ptr()), arr(*[int(v) for v in y_i] + [0] * (4 - ni)),
-        int(conj_y), vp(gb32.data_ptr()), vp(w.data_ptr()), int(X.conj), vp(target.data_ptr()),
-        _DTYPE_CODE[target.dtype], vp(big.data_ptr() + 8 * R), vp(sm.data_ptr() + 8 * nyt),
-        arr(*[int(v) for v in o_j] + [0] * (4 - nj))), "ptn_block_apply")
+    args = [R, nt, ng, nj, ni, None, _DTYPE_CODE[Y.dtype], ctypes.c_void_p(big.data_ptr()),
+            ctypes.c_void_p(sm.data_ptr()), arr(*[int(v) for v in y_i] + [0] * (4 - ni)),
+            int(conj_y), ctypes.c_void_p(gb32.data_ptr()), ctypes.c_void_p(w.data_ptr()),
+            int(X.conj), None, _DTYPE_CODE[target.dtype],
+            ctypes.c_void_p(big.data_ptr() + 8 * R), ctypes.c_void_p(sm.data_ptr() + 8 * nyt),
+            arr(*[int(v) for v in o_j] + [0] * (4 - nj))]
+    keep = (big, sm, gb32, w)
+    if _RECORDER is not None:
+        _RECORDER.ops.append(("block", args, Y, target, keep))
+    _run_block(args, Y, target)
+
+
+def _run_block(args, Y, target):
+    args = list(args)
+    args[5] = ctypes.c_void_p(Y.data_ptr())
+    args[14] = ctypes.c_void_p(target.data_ptr())
+    _lib.check(_lib.load().ptn_block_apply(ctypes.c_void_p(stream_handle()), *args),
+               "ptn_block_apply")
 
 
 @dataclass
@@ -747,3 +799,96 @@ def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
         slots[sb] = None
     final = slots[-1]
     return final[0]
+
+
+class CompiledNetwork:
+    """A network contraction recorded once and replayed with new variable inputs.
+
+    The first call runs ``contract_network`` while recording every launch
+    (gather-GEMM descriptors, block applies, fills) together with the tensors
+    they touch.  Later calls allocate only the output, patch the pointers of the
+    variable inputs and the output into the recorded descriptors, and re-issue
+    the launches: no planning, labels or offset-table lookups on the host.
+    Intermediates are kept and reused, so calls must be ordered on one stream
+    (the Krylov loop calls one apply after another).
+    """
+
+    def __init__(self, tensors: Sequence, labels, out_labels, *, conj=None,
+                 precision: str = FP64, out_dtype: torch.dtype = torch.complex128,
+                 variable: Sequence[int] = (0,)):
+        self.tensors = list(tensors)
+        self.labels = labels
+        self.out_labels = out_labels
+        self.conj = conj
+        self.precision = precision
+        self.out_dtype = out_dtype
+        self.variable = tuple(variable)
+        self.ops = None
+        self.sig = None
+
+    @staticmethod
+    def _sig(ts):
+        return tuple((tuple(t.shape), tuple(t.stride()), t.dtype) for t in ts)
+
+    def __call__(self, *var: torch.Tensor) -> torch.Tensor:
+        sig = self._sig(var)
+        if self.ops is None or sig != self.sig:
+            return self._record(var, sig)
+        out = torch.empty(self.out_shape, dtype=self.out_dtype, device=var[0].device)
+        for op, roles in zip(self.ops, self.op_roles):
+            kind = op[0]
+            if kind == "gemm":
+                d = op[1]
+                for field, role in roles:
+                    t = var[role[1]] if role[0] == "var" else out
+                    setattr(d, field, t.data_ptr() + role[2])
+                _run_gemm(d, op[5], op[6], op[7], op[8])
+            elif kind == "block":
+                Y = op[2] if roles[0] is None else (var[roles[0][1]] if roles[0][0] == "var"
+                                                    else out)
+                T = op[3] if roles[1] is None else (var[roles[1][1]] if roles[1][0] == "var"
+                                                    else out)
+                _run_block(op[1], Y, T)
+            else:
+                T = op[1] if roles[0] is None else out
+                fill_(T, op[2])
+        return out
+
+    def _record(self, var, sig) -> torch.Tensor:
+        global _RECORDER
+        ts = list(self.tensors)
+        for i, v in zip(self.variable, var):
+            ts[i] = v
+        rec = _Recorder()
+        prev, _RECORDER = _RECORDER, rec
+        try:
+            out = contract_network(ts, self.labels, self.out_labels, conj=self.conj,
+                                   precision=self.precision, out_dtype=self.out_dtype)
+        finally:
+            _RECORDER = prev
+        self.out_shape = tuple(out.shape)
+        self.sig = sig
+        self.ops = rec.ops
+
+        def role(t):
+            for k, v in enumerate(var):
+                if t is v:
+                    return ("var", k, 0)
+            if t is out:
+                return ("out", 0, 0)
+            return None
+
+        self.op_roles = []
+        for op in self.ops:
+            if op[0] == "gemm":
+                roles = []
+                for field, t in (("A", op[2]), ("B", op[3]), ("C", op[4])):
+                    r = role(t)
+                    if r is not None:
+                        roles.append((field, r))
+                self.op_roles.append(roles)
+            elif op[0] == "block":
+                self.op_roles.append((role(op[2]), role(op[3])))
+            else:
+                self.op_roles.append((role(op[1]),))
+        return out

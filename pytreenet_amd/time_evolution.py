@@ -31,8 +31,8 @@ import torch
 
 from . import engine as E
 from .contractions import (SandwichCache, bond_apply, contract_any,
-                           contract_ket_ham_with_envs, expectation_value, operator_tensor,
-                           scalar_product)
+                           contract_ket_ham_with_envs, expectation_value, heff_network,
+                           operator_tensor, scalar_product)
 from .krylov import EvoDirection, TimeEvoMode, krylov_exp_action
 from .splitting import (SVDParameters, SplitMode, _colmajor_panel, _cstrides, qr_colmajor,
                         svd_colmajor, truncate_singular_values)
@@ -310,10 +310,11 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         op_node = self.hamiltonian.nodes[node_id]
         op = operator_tensor(self.hamiltonian, node_id, self.precision)
         cache = self.partial_tree_cache
-
-        def action(y):
-            return contract_ket_ham_with_envs(node, y, op_node, op, cache,
-                                              precision=self.precision)
+        envs = [cache.get_entry(nb, node_id) for nb in node.neighbouring_nodes()]
+        tensors, labels, out_l = heff_network(node, op_node, op, envs)
+        tensors[0] = tensor
+        # recorded on the first Krylov apply, replayed for the others
+        action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision)
 
         from .krylov import STATS
         n0 = STATS.applies
@@ -344,8 +345,11 @@ class OneSiteTDVP(TDVPAlgorithm):
         env_b = self.partial_tree_cache.get_entry(next_node_id, node_id)
         from .krylov import STATS
         n0 = STATS.applies
-        r = self._krylov(r, lambda y: bond_apply(y, env_a, env_b, precision=self.precision),
-                         self.time_step_size * time_step_factor, EvoDirection.BACKWARD)
+        bond = E.CompiledNetwork([r, env_a, env_b],
+                                 [["a0", "a1"], ["a0", "w", "b0"], ["a1", "w", "b1"]],
+                                 ["b0", "b1"], precision=self.precision)
+        r = self._krylov(r, bond, self.time_step_size * time_step_factor,
+                         EvoDirection.BACKWARD)
         self.apply_log.append(("bond", (node_id, next_node_id), STATS.applies - n0))
         absorb_into_neighbour(self.state, next_node_id, node_id, r)
         self.state.orthogonality_center_id = next_node_id
@@ -365,6 +369,7 @@ class FirstOrderOneSiteTDVP(OneSiteTDVP):
         self.partial_tree_cache = self._init_partial_tree_cache()
 
     def run_one_time_step(self, **kwargs):
+        E.refresh_stream()
         up, op = self.update_path, self.orthogonalization_path
         for i, node_id in enumerate(up):
             if i == len(up) - 1:
@@ -419,6 +424,7 @@ class SecondOrderOneSiteTDVP(OneSiteTDVP):
         self._update_site(bp[-1], time_step_factor=0.5)
 
     def run_one_time_step(self, **kwargs):
+        E.refresh_stream()
         self.forward_sweep()
         self._final_forward_update()
         self.backward_sweep()
@@ -461,12 +467,8 @@ class TwoSiteTDVP(TDVPAlgorithm):
             tensors.append(cache.get_entry(nb, n2))
             labels.append([("a", k1 + i), ("w", k1 + i), ("b", k1 + i)])
         out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
-
-        def action(y):
-            tensors[0] = y
-            return E.contract_network(tensors, labels, out, precision=self.precision)
-
-        return action, k1, k2
+        return (lambda y: E.CompiledNetwork([y] + tensors[1:], labels, out,
+                                            precision=self.precision)), k1, k2
 
     def _update_two_site_nodes(self, target_node_id: str, next_node_id: str,
                                time_step_factor: float = 1):
@@ -485,10 +487,10 @@ class TwoSiteTDVP(TDVPAlgorithm):
         outer1 = [x for x in l1[:-1] if x != "bond"]
         outer2 = [x for x in l2[:-1] if x != "bond"]
         x = E.contract(A1, l1, A2, l2, outer1 + outer2 + ["p1", "p2"])
-        action, _, _ = self._two_site_action(n1, n2)
+        make, _, _ = self._two_site_action(n1, n2)
         from .krylov import STATS
         a0 = STATS.applies
-        x = self._krylov(x, action, self.time_step_size * time_step_factor,
+        x = self._krylov(x, make(x), self.time_step_size * time_step_factor,
                          EvoDirection.FORWARD)
         self.apply_log.append(("two_site", (n1, n2), STATS.applies - a0))
         # truncated SVD, S absorbed into V (VCONTR)
@@ -562,5 +564,6 @@ class SecondOrderTwoSiteTDVP(TwoSiteTDVP):
                 self._update_two_site_nodes(target, bp[i + 1], time_step_factor=0.5)
 
     def run_one_time_step(self, **kwargs):
+        E.refresh_stream()
         self.forward_sweep()
         self.backwards_sweep()
