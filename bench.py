@@ -31,9 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
-# measured HBM bytes per LARGE FP64 gather-GEMM launch (rocprofv3 --pmc FETCH_SIZE /
-# WRITE_SIZE on this command, scaled by the copy calibration; see profiles/)
-TRAFFIC_BYTES_PER_LAUNCH = 5.43e8
+# measured HBM bytes per LARGE FP64 launch of each backend (rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE on this command, scaled by the copy calibration; the
+# summaries are under profiles/); None where not yet measured
+TRAFFIC_BYTES_PER_LAUNCH = {"gather": 5.43e8, "blas": None}
+KERNEL_NAME = {"gather": "gemm_gather_kernel<F64_4M, c128, c128, c128, LARGE=1>",
+               "blas": "rocblas_zgemm (ptn_gemm_strided)"}
 METRIC = "TDVP sweeps/s & H_eff TFLOP/s at χ=256, mixed vs FP64 GPU at ΔE≤1e-8"
 
 
@@ -243,33 +246,45 @@ def main():
     value = res["mixed_sweeps_per_s"] if mixed_ok else sps64
     roofline = None
     if kern:
-        # dominant kernel = the LARGE instantiation of the FP64 gather-GEMM (every
-        # launch with a tile grid >= 2048): flops per launch / mean launch time
-        n = sum(k["launches"] for k in kern)
-        fl = sum(k["flops_per_launch"] * k["launches"] for k in kern)
-        ms = sum(k["total_ms"] for k in kern)
-        ach = fl / (ms * 1e-3)
-        roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12,
+        # dominant kernel: of the LARGE FP64 launches (tile grid >= 2048) in the timed
+        # steps, the backend with the larger total time -- rocBLAS strided GEMM
+        # ("blas") or the hand-written gather-GEMM's LARGE instantiation ("gather");
+        # the other one is reported beside it
+        by = {}
+        for k in kern:
+            by.setdefault(k["backend"], []).append(k)
+        tot = {b: sum(k["total_ms"] for k in ks) for b, ks in by.items()}
+        order = sorted(by, key=lambda b: -tot[b])
+
+        def block(b):
+            ks = by[b]
+            n = sum(k["launches"] for k in ks)
+            fl = sum(k["flops_per_launch"] * k["launches"] for k in ks)
+            ms = sum(k["total_ms"] for k in ks)
+            ach = fl / (ms * 1e-3)
+            return {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12,
                     "unit": "TFLOP/s", "frac": ach / PEAK["fp64"],
-                    # HBM bytes per launch of this kernel: rocprofv3 FETCH_SIZE x 2.0 +
-                    # WRITE_SIZE x 1.0 (scales from --calibrate-copy), averaged over its
-                    # launches in one bench.py process
-                    "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                    "kernel": "gemm_gather_kernel<F64_4M, c128, c128, c128, LARGE=1>",
-                    "flops_per_launch": fl / n, "avg_ms": ms / n, "launches": n,
-                    "timed_steps": args.steps,
+                    # HBM bytes per launch: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+                    # (scales from --calibrate-copy), averaged over the launches
+                    "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(b),
+                    "kernel": KERNEL_NAME[b], "flops_per_launch": fl / n, "avg_ms": ms / n,
+                    "launches": n, "timed_steps": args.steps,
+                    "share_of_step": ms * 1e-3 / r64["elapsed"],
                     "shapes": [{"M": k["M"], "N": k["N"], "K": k["K"],
                                 "launches": k["launches"], "avg_ms": round(k["avg_ms"], 3),
                                 "tflops": round(k["flops_per_launch"] / k["avg_ms"] / 1e9, 2)}
-                               for k in kern[:6]]}
-    if roofline is not None:
-        allk = E.PROFILER.summary(compute="fp64")
+                               for k in ks[:6]]}
+        roofline = block(order[0])
+        if len(order) > 1:
+            roofline["other"] = block(order[1])
+        allk = E.PROFILER.summary(compute="fp64", backend=order[0])
         n_all = sum(k["launches"] for k in allk)
         roofline["all_phases"] = {"launches": n_all,
                                   "avg_ms": sum(k["total_ms"] for k in allk) / n_all,
-                                  "note": "every LARGE FP64 launch of this process (setup, "
-                                          "warmup, timed, evaluation, QR of the mixed run): "
-                                          "the set rocprofv3 --stats averages over"}
+                                  "note": "every LARGE FP64 launch of this backend in this "
+                                          "process (setup, warmup, timed, evaluation, QR of "
+                                          "the mixed run): the set rocprofv3 --stats "
+                                          "averages over"}
     E.PROFILER.stop()
     cpu = None
     if not args.no_cpu:

@@ -77,6 +77,36 @@ typedef struct ptn_gemm_desc {
 
 int ptn_gemm_gather(void *stream, const ptn_gemm_desc *desc);
 
+/* The same contraction when every offset table is an arithmetic progression
+ * (the legs of each index space are already grouped in memory):
+ *
+ *   C[m c_sm + n c_sn] (beta_one ? += : =)
+ *       alpha * sum_k opA(A[m a_sm + k a_sk]) * opB(B[k b_sk + n b_sn])
+ *
+ * run as one rocblas_zgemm (dtype PTN_C128) or rocblas_cgemm (PTN_C64) call
+ * on `stream`.  A, B, C point at element (0, 0).  Each operand and C needs a
+ * unit stride on one of its two index spaces, and a conjugated operand must
+ * end up transposed in the column-major call; ptn_gemm_strided_ok() tells
+ * whether a descriptor qualifies (1) or not (0) without launching anything,
+ * and ptn_gemm_strided returns PTN_EINVAL for one that does not.  Replaces
+ * the same np.tensordot calls as ptn_gemm_gather.                            */
+typedef struct ptn_gemm_strided_desc {
+  int64_t M, N, K;
+  const void *A;
+  int64_t a_sm, a_sk;
+  int32_t a_conj, _pad0;
+  const void *B;
+  int64_t b_sk, b_sn;
+  int32_t b_conj, _pad1;
+  void *C;
+  int64_t c_sm, c_sn;
+  int32_t dtype, beta_one;
+  double alpha_re, alpha_im;
+} ptn_gemm_strided_desc;
+
+int ptn_gemm_strided_ok(const ptn_gemm_strided_desc *desc);
+int ptn_gemm_strided(void *stream, const ptn_gemm_strided_desc *desc);
+
 /* Strided copy with optional conjugation and dtype conversion:
  *   dst[sum_i idx_i * dst_strides[i]] = op(src[sum_i idx_i * src_strides[i]])
  * over the box dims[0..rank-1] (rank <= 8).  Strides are in elements.
@@ -104,6 +134,19 @@ int ptn_fill(void *stream, void *dst, int64_t n, int32_t dtype, double re, doubl
 int64_t ptn_krylov_work_bytes(int64_t n, int32_t nvec);
 int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, int64_t ldq,
                              int32_t nvec, void *w, void *out, void *work);
+
+/* ptn_krylov_orthogonalize followed by the Arnoldi normalisation
+ * q_{j+1} = w / ||w|| (time_evolution/time_evolution.py:818) written to
+ * `qnext` (n complex128; not written when NULL; zero when ||w|| = 0), so the
+ * next H_eff application can be queued before the host reads `out`.  Short
+ * vectors (n <= 8192) take one single-workgroup launch.                     */
+int ptn_krylov_orthonormalize(void *stream, int64_t n, const void *Q, int64_t ldq,
+                              int32_t nvec, void *w, void *qnext, void *out, void *work);
+
+/* y = x / ||x|| (y may alias x; zero when ||x|| = 0) and out[0] = (||x||^2, 0)
+ * (complex128, device): the start vector q_0 = psi / ||psi|| of
+ * krylov_exp_action (time_evolution/time_evolution.py:775-781).             */
+int ptn_normalize(void *stream, int64_t n, const void *x, void *y, void *out, void *work);
 
 /* out = sum_i coeffs[i] * Q[i]   (coeffs: device complex128[nvec]);
  * also used for q_{j+1} = w / beta with nvec = 1.                           */

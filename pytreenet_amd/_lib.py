@@ -17,14 +17,28 @@ PTN_COMPUTE_F64_4M, PTN_COMPUTE_F32_3M = 0, 1
 
 EXPORTED = (
     "ptn_gemm_gather", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
-    "ptn_krylov_orthogonalize", "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc",
-    "ptn_qr", "ptn_svd", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
+    "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize", "ptn_normalize",
+    "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc",
+    "ptn_gemm_strided", "ptn_gemm_strided_ok", "ptn_qr", "ptn_svd", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
     "ptn_last_error", "ptn_version",
 )
 
 
 class PtnBackendError(RuntimeError):
     """The HIP backend is unavailable or a kernel call failed."""
+
+
+class StridedDesc(ctypes.Structure):
+    _fields_ = [
+        ("M", ctypes.c_int64), ("N", ctypes.c_int64), ("K", ctypes.c_int64),
+        ("A", ctypes.c_void_p), ("a_sm", ctypes.c_int64), ("a_sk", ctypes.c_int64),
+        ("a_conj", ctypes.c_int32), ("_pad0", ctypes.c_int32),
+        ("B", ctypes.c_void_p), ("b_sk", ctypes.c_int64), ("b_sn", ctypes.c_int64),
+        ("b_conj", ctypes.c_int32), ("_pad1", ctypes.c_int32),
+        ("C", ctypes.c_void_p), ("c_sm", ctypes.c_int64), ("c_sn", ctypes.c_int64),
+        ("dtype", ctypes.c_int32), ("beta_one", ctypes.c_int32),
+        ("alpha_re", ctypes.c_double), ("alpha_im", ctypes.c_double),
+    ]
 
 
 class GemmDesc(ctypes.Structure):
@@ -61,10 +75,14 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
     sig = {
         "ptn_gemm_gather": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc)]),
+        "ptn_gemm_strided": (ctypes.c_int, [vp, ctypes.POINTER(StridedDesc)]),
+        "ptn_gemm_strided_ok": (ctypes.c_int, [ctypes.POINTER(StridedDesc)]),
         "ptn_permute": (ctypes.c_int, [vp, i32, vp, vp, vp, i32, vp, vp, i32, i32]),
         "ptn_fill": (ctypes.c_int, [vp, vp, i64, i32, dbl, dbl]),
         "ptn_krylov_work_bytes": (i64, [i64, i32]),
         "ptn_krylov_orthogonalize": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp]),
+        "ptn_krylov_orthonormalize": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp, vp]),
+        "ptn_normalize": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
         "ptn_krylov_combine": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp]),
         "ptn_norm_sq": (ctypes.c_int, [vp, i64, vp, vp, vp]),
         "ptn_dotc": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),

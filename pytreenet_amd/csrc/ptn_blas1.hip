@@ -42,29 +42,38 @@ __device__ __forceinline__ double2 cmul_conj_a(double2 a, double2 b) {
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
-// block-wide sum of `cnt` complex values per thread held in LDS rows
-template <int CNT>
-__device__ void block_reduce_store(double2 (&acc)[CNT], int cnt, double2 *out_row) {
-  __shared__ double2 red[RT / 64][CNT];
+// block-wide sum of `cnt` complex values per thread; thread v < cnt stores the
+// v-th sum to dst[v].  NT is the block size (a multiple of 64).
+template <int NT, int CNT>
+__device__ void block_sum(double2 (&acc)[CNT], int cnt, double2 *dst) {
+  __shared__ double2 red[NT / 64][CNT];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int v = 0; v < cnt; ++v) {
-    double x = acc[v].x, y = acc[v].y;
-    for (int off = 32; off > 0; off >>= 1) {
-      x += __shfl_down(x, off, 64);
-      y += __shfl_down(y, off, 64);
+#pragma unroll
+  for (int v = 0; v < CNT; ++v) {
+    if (v < cnt) {  // uniform: cnt is the same in every thread
+      double x = acc[v].x, y = acc[v].y;
+      for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_down(x, off, 64);
+        y += __shfl_down(y, off, 64);
+      }
+      if (lane == 0) red[wave][v] = make_double2(x, y);
     }
-    if (lane == 0) red[wave][v] = make_double2(x, y);
   }
   __syncthreads();
   if (threadIdx.x < cnt) {
     double2 s = make_double2(0, 0);
-    for (int w = 0; w < RT / 64; ++w) {
+    for (int w = 0; w < NT / 64; ++w) {
       s.x += red[w][threadIdx.x].x;
       s.y += red[w][threadIdx.x].y;
     }
-    out_row[threadIdx.x] = s;
+    dst[threadIdx.x] = s;
   }
   __syncthreads();
+}
+
+template <int CNT>
+__device__ void block_reduce_store(double2 (&acc)[CNT], int cnt, double2 *out_row) {
+  block_sum<RT, CNT>(acc, cnt, out_row);
 }
 
 // partial[b][v] = sum over block b's range of conj(Q[v0+v]) * w, v < cnt;
@@ -92,27 +101,141 @@ __global__ __launch_bounds__(RT) void dots_partial(int64_t n, const double2 *Q, 
   }
   double2 *row = partial + (int64_t)blockIdx.x * pstride;
   // move the self term next to the dots so one reduction covers both
-  if (self_slot >= 0) acc[cnt] = acc[KV];
+  if (self_slot >= 0) {
+#pragma unroll
+    for (int v = 0; v < KV; ++v)
+      if (v == cnt) acc[v] = acc[KV];
+  }
   block_reduce_store<KV + 1>(acc, cnt + (self_slot >= 0 ? 1 : 0), row + v0);
 }
 
-// out[v] = sum_b partial[b][v] for v < count (fixed order), optional add-into
-__global__ __launch_bounds__(RT) void reduce_partials(const double2 *partial, int pstride,
-                                                      int nblk, int count, double2 *out,
-                                                      int accumulate) {
-  for (int v = threadIdx.x; v < count; v += RT) {
-    double2 s = make_double2(0, 0);
-    for (int b = 0; b < nblk; ++b) {
-      const double2 t = partial[(int64_t)b * pstride + v];
-      s.x += t.x;
-      s.y += t.y;
+// out[v] = sum_b partial[b][v]; one block per v, fixed summation order
+__global__ __launch_bounds__(RT) void reduce_cols(const double2 *partial, int pstride, int nblk,
+                                                  double2 *out) {
+  const int v = blockIdx.x;
+  double2 acc[1] = {make_double2(0, 0)};
+  for (int b = threadIdx.x; b < nblk; b += RT) {
+    const double2 t = partial[(int64_t)b * pstride + v];
+    acc[0].x += t.x;
+    acc[0].y += t.y;
+  }
+  block_sum<RT, 1>(acc, 1, out + v);
+}
+
+// bsq = sum_b partial[b][0].x; writes out[0..nvec) = h1 + h2 and
+// out[nvec] = (w0sq, bsq) (nvec > 0), or out[0] = (bsq, 0) (nvec == 0), and
+// inv = 1 / sqrt(bsq) (0 when bsq <= 0)
+__global__ __launch_bounds__(RT) void finish_kernel(const double2 *partial, int pstride,
+                                                    int nblk, int nvec, const double2 *h1,
+                                                    const double2 *h2, double2 *out,
+                                                    double2 *inv) {
+  __shared__ double2 tot;
+  double2 acc[1] = {make_double2(0, 0)};
+  for (int b = threadIdx.x; b < nblk; b += RT) acc[0].x += partial[(int64_t)b * pstride].x;
+  block_sum<RT, 1>(acc, 1, &tot);
+  const double bsq = tot.x;
+  for (int v = threadIdx.x; v < nvec; v += RT)
+    out[v] = make_double2(h1[v].x + h2[v].x, h1[v].y + h2[v].y);
+  if (threadIdx.x == 0) {
+    out[nvec] = nvec > 0 ? make_double2(h1[nvec].x, bsq) : make_double2(bsq, 0.0);
+    *inv = make_double2(bsq > 0.0 ? 1.0 / sqrt(bsq) : 0.0, 0.0);
+  }
+}
+
+// y = x * inv.x
+__global__ __launch_bounds__(RT) void scale_kernel(int64_t n, const double2 *x,
+                                                   const double2 *inv, double2 *y) {
+  const double c = inv->x;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    const double2 v = x[i];
+    y[i] = make_double2(c * v.x, c * v.y);
+  }
+}
+
+// ---- one-workgroup forms for short vectors (n <= SMALL_N) --------------------
+// Whole CGS2 step in one launch: h1 = Q^H w, w -= Q h1, h2 = Q^H w, w -= Q h2,
+// out = (h1 + h2, (|w_in|^2, |w_out|^2)), qnext = w / |w| when given.  Every
+// thread owns the same elements of w in every phase, so w needs no barrier;
+// the coefficients pass through LDS.
+constexpr int ST = 1024;
+constexpr int64_t SMALL_N = 8192;
+
+__global__ __launch_bounds__(ST) void orth_small(int64_t n, const double2 *Q, int64_t ldq,
+                                                 int nvec, double2 *w, double2 *qnext,
+                                                 double2 *out) {
+  __shared__ double2 h[2][MAXV + 1];
+  __shared__ double2 bsq;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int v0 = 0; v0 < nvec; v0 += KV) {
+      const int cnt = (nvec - v0) < KV ? (nvec - v0) : KV;
+      const bool self = pass == 0 && v0 + KV >= nvec;
+      double2 acc[KV + 1];
+#pragma unroll
+      for (int v = 0; v <= KV; ++v) acc[v] = make_double2(0, 0);
+      for (int64_t i = threadIdx.x; i < n; i += ST) {
+        const double2 wi = w[i];
+#pragma unroll
+        for (int v = 0; v < KV; ++v) {
+          if (v < cnt) {
+            const double2 t = cmul_conj_a(Q[(int64_t)(v0 + v) * ldq + i], wi);
+            acc[v].x += t.x;
+            acc[v].y += t.y;
+          }
+        }
+        if (self) acc[KV].x += wi.x * wi.x + wi.y * wi.y;
+      }
+      if (self) {
+#pragma unroll
+        for (int v = 0; v < KV; ++v)
+          if (v == cnt) acc[v] = acc[KV];
+      }
+      block_sum<ST, KV + 1>(acc, cnt + (self ? 1 : 0), &h[pass][v0]);
     }
-    if (accumulate) {
-      out[v].x += s.x;
-      out[v].y += s.y;
-    } else {
-      out[v] = s;
+    double nrm = 0;
+    for (int64_t i = threadIdx.x; i < n; i += ST) {
+      double2 wi = w[i];
+      for (int v = 0; v < nvec; ++v) {
+        const double2 q = Q[(int64_t)v * ldq + i];
+        const double2 hv = h[pass][v];
+        wi.x -= hv.x * q.x - hv.y * q.y;
+        wi.y -= hv.x * q.y + hv.y * q.x;
+      }
+      w[i] = wi;
+      nrm += wi.x * wi.x + wi.y * wi.y;
     }
+    if (pass == 1) {
+      double2 acc[1] = {make_double2(nrm, 0)};
+      block_sum<ST, 1>(acc, 1, &bsq);
+    }
+  }
+  for (int v = threadIdx.x; v < nvec; v += ST)
+    out[v] = make_double2(h[0][v].x + h[1][v].x, h[0][v].y + h[1][v].y);
+  if (threadIdx.x == 0) out[nvec] = make_double2(h[0][nvec].x, bsq.x);
+  if (qnext) {
+    const double c = bsq.x > 0.0 ? 1.0 / sqrt(bsq.x) : 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += ST) {
+      const double2 wi = w[i];
+      qnext[i] = make_double2(c * wi.x, c * wi.y);
+    }
+  }
+}
+
+// y = x / |x|, out[0] = (|x|^2, 0)
+__global__ __launch_bounds__(ST) void normalize_small(int64_t n, const double2 *x, double2 *y,
+                                                      double2 *out) {
+  __shared__ double2 tot;
+  double2 acc[1] = {make_double2(0, 0)};
+  for (int64_t i = threadIdx.x; i < n; i += ST) {
+    const double2 v = x[i];
+    acc[0].x += v.x * v.x + v.y * v.y;
+  }
+  block_sum<ST, 1>(acc, 1, &tot);
+  const double c = tot.x > 0.0 ? 1.0 / sqrt(tot.x) : 0.0;
+  if (threadIdx.x == 0) out[0] = make_double2(tot.x, 0.0);
+  for (int64_t i = threadIdx.x; i < n; i += ST) {
+    const double2 v = x[i];
+    y[i] = make_double2(c * v.x, c * v.y);
   }
 }
 
@@ -140,14 +263,6 @@ __global__ __launch_bounds__(RT) void subtract_proj(int64_t n, const double2 *Q,
     double2 acc[1] = {make_double2(nrm, 0)};
     block_reduce_store<1>(acc, 1, partial + (int64_t)blockIdx.x * pstride);
   }
-}
-
-// final packing: out[0..nvec) = h1 + h2, out[nvec] = (w0sq, beta_sq)
-__global__ void pack_orth(int nvec, const double2 *h1, const double2 *h2, const double2 *w0sq,
-                          const double2 *bsq, double2 *out) {
-  for (int v = threadIdx.x; v < nvec; v += blockDim.x)
-    out[v] = make_double2(h1[v].x + h2[v].x, h1[v].y + h2[v].y);
-  if (threadIdx.x == 0) out[nvec] = make_double2(w0sq->x, bsq->x);
 }
 
 __global__ __launch_bounds__(RT) void combine_kernel(int64_t n, const double2 *Q, int64_t ldq,
@@ -213,57 +328,96 @@ inline unsigned grid_for(int64_t total) {
 }  // namespace
 
 extern "C" int64_t ptn_krylov_work_bytes(int64_t n, int32_t nvec) {
-  // partial rows: MAXBLK x (MAXV + 2) complex, plus 2*MAXV + 4 complex scratch
+  // partial rows: MAXBLK x (MAXV + 2) complex, plus 2 (MAXV + 1) + 1 complex scratch
   (void)n;
   (void)nvec;
   return (int64_t)sizeof(double2) * ((int64_t)MAXBLK * (MAXV + 2) + 2 * MAXV + 8);
 }
 
-extern "C" int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Qv, int64_t ldq,
-                                        int32_t nvec, void *wv, void *outv, void *work) {
+extern "C" int ptn_krylov_orthonormalize(void *stream, int64_t n, const void *Qv, int64_t ldq,
+                                         int32_t nvec, void *wv, void *qnextv, void *outv,
+                                         void *work) {
   if (n <= 0 || nvec < 1 || nvec > MAXV || ldq < n || !Qv || !wv || !outv || !work) {
-    ptn::set_error("ptn_krylov_orthogonalize: bad arguments (n=%lld nvec=%d)", (long long)n,
+    ptn::set_error("ptn_krylov_orthonormalize: bad arguments (n=%lld nvec=%d)", (long long)n,
                    nvec);
     return nvec > MAXV ? PTN_ERANGE : PTN_EINVAL;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double2 *Q = static_cast<const double2 *>(Qv);
   double2 *w = static_cast<double2 *>(wv);
+  double2 *qnext = static_cast<double2 *>(qnextv);
   double2 *out = static_cast<double2 *>(outv);
+  if (n <= SMALL_N) {
+    hipLaunchKernelGGL(orth_small, dim3(1), dim3(ST), 0, s, n, Q, ldq, nvec, w, qnext, out);
+    PTN_LAUNCH_CHECK();
+    return PTN_OK;
+  }
   double2 *partial = static_cast<double2 *>(work);
   const int pstride = MAXV + 2;
   double2 *scratch = partial + (int64_t)MAXBLK * pstride;
-  double2 *h1 = scratch, *h2 = scratch + MAXV, *w0 = scratch + 2 * MAXV,
-          *bsq = scratch + 2 * MAXV + 1;
+  double2 *h1 = scratch, *h2 = scratch + (MAXV + 1), *inv = scratch + 2 * (MAXV + 1);
   const int nb = nblocks_for(n);
   for (int pass = 0; pass < 2; ++pass) {
     double2 *h = pass == 0 ? h1 : h2;
     for (int v0 = 0; v0 < nvec; v0 += KV) {
       const int cnt = (nvec - v0) < KV ? (nvec - v0) : KV;
       const bool last = v0 + KV >= nvec;
-      // the self-norm of w (pass 0 only) lands in slot nvec
+      // pass 0: |w|^2 lands in column nvec of the partial rows
       const int self_slot = (pass == 0 && last) ? 1 : -1;
       hipLaunchKernelGGL(dots_partial, dim3(nb), dim3(RT), 0, s, n, Q, ldq, v0, cnt, w,
                          partial, pstride, self_slot);
       PTN_LAUNCH_CHECK();
     }
-    const int count = nvec + (pass == 0 ? 1 : 0);
-    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(RT), 0, s, partial, pstride, nb, count,
-                       h, 0);
+    // h1[0..nvec) = Q^H w, h1[nvec] = |w_in|^2;  h2[0..nvec) = Q^H w (second pass)
+    hipLaunchKernelGGL(reduce_cols, dim3(nvec + (pass == 0 ? 1 : 0)), dim3(RT), 0, s, partial,
+                       pstride, nb, h);
     PTN_LAUNCH_CHECK();
-    if (pass == 0) {
-      // copy w0sq out of h1[nvec] before h1 is used as coefficients
-      hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(RT), 0, s, partial + nvec, pstride,
-                         nb, 1, w0, 0);
-      PTN_LAUNCH_CHECK();
-    }
     hipLaunchKernelGGL(subtract_proj, dim3(nb), dim3(RT), 0, s, n, Q, ldq, nvec, h, w,
                        pass == 1 ? partial : nullptr, pstride);
     PTN_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(RT), 0, s, partial, pstride, nb, 1, bsq, 0);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(RT), 0, s, partial, pstride, nb, nvec, h1, h2,
+                     out, inv);
   PTN_LAUNCH_CHECK();
-  hipLaunchKernelGGL(pack_orth, dim3(1), dim3(64), 0, s, nvec, h1, h2, w0, bsq, out);
+  if (qnext) {
+    hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(RT), 0, s, n, w, inv, qnext);
+    PTN_LAUNCH_CHECK();
+  }
+  return PTN_OK;
+}
+
+extern "C" int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, int64_t ldq,
+                                        int32_t nvec, void *w, void *out, void *work) {
+  return ptn_krylov_orthonormalize(stream, n, Q, ldq, nvec, w, nullptr, out, work);
+}
+
+extern "C" int ptn_normalize(void *stream, int64_t n, const void *xv, void *yv, void *outv,
+                             void *work) {
+  if (n <= 0 || !xv || !yv || !outv || !work) {
+    ptn::set_error("ptn_normalize: bad arguments");
+    return PTN_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double2 *x = static_cast<const double2 *>(xv);
+  double2 *y = static_cast<double2 *>(yv);
+  double2 *out = static_cast<double2 *>(outv);
+  if (n <= SMALL_N) {
+    hipLaunchKernelGGL(normalize_small, dim3(1), dim3(ST), 0, s, n, x, y, out);
+    PTN_LAUNCH_CHECK();
+    return PTN_OK;
+  }
+  double2 *partial = static_cast<double2 *>(work);
+  const int pstride = MAXV + 2;
+  double2 *inv = partial + (int64_t)MAXBLK * pstride + 2 * (MAXV + 1);
+  const int nb = nblocks_for(n);
+  // cnt = 0 dots: the self norm lands in column 0
+  hipLaunchKernelGGL(dots_partial, dim3(nb), dim3(RT), 0, s, n, x, n, 0, 0, x, partial, pstride,
+                     1);
+  PTN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(RT), 0, s, partial, pstride, nb, 0, nullptr,
+                     nullptr, out, inv);
+  PTN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(RT), 0, s, n, x, inv, y);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }
@@ -291,13 +445,13 @@ extern "C" int ptn_norm_sq(void *stream, int64_t n, const void *x, void *out, vo
   const int nb = nblocks_for(n);
   double2 *partial = static_cast<double2 *>(work);
   const int pstride = MAXV + 2;
-  // cnt = 0 dots, self norm only -> lands in slot 0
+  // cnt = 0 dots, self norm only -> lands in column 0
   hipLaunchKernelGGL(dots_partial, dim3(nb), dim3(RT), 0, s, n,
                      static_cast<const double2 *>(x), n, 0, 0,
                      static_cast<const double2 *>(x), partial, pstride, 1);
   PTN_LAUNCH_CHECK();
-  hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(RT), 0, s, partial, pstride, nb, 1,
-                     static_cast<double2 *>(out), 0);
+  hipLaunchKernelGGL(reduce_cols, dim3(1), dim3(RT), 0, s, partial, pstride, nb,
+                     static_cast<double2 *>(out));
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }
@@ -316,8 +470,8 @@ extern "C" int ptn_dotc(void *stream, int64_t n, const void *x, const void *y, v
                      static_cast<const double2 *>(x), n, 0, 1,
                      static_cast<const double2 *>(y), partial, pstride, -1);
   PTN_LAUNCH_CHECK();
-  hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(RT), 0, s, partial, pstride, nb, 1,
-                     static_cast<double2 *>(out), 0);
+  hipLaunchKernelGGL(reduce_cols, dim3(1), dim3(RT), 0, s, partial, pstride, nb,
+                     static_cast<double2 *>(out));
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

@@ -19,6 +19,7 @@ namespace {
 std::mutex g_mu;
 rocblas_handle g_handles[64] = {nullptr};
 
+// callers hold g_mu: it guards the handle table and each handle's stream binding
 int get_handle(hipStream_t s, rocblas_handle *out) {
   int dev = 0;
   PTN_HIP_CHECK(hipGetDevice(&dev));
@@ -38,6 +39,68 @@ int get_handle(hipStream_t s, rocblas_handle *out) {
   }
   *out = g_handles[dev];
   return PTN_OK;
+}
+
+// ---- strided complex GEMM through rocBLAS ------------------------------------
+struct OpLd {
+  rocblas_operation op;
+  rocblas_int ld;
+};
+
+// operand X, logically R x Cc with element strides (sr, sc): as a column-major
+// matrix (op none, ld = sc) or as the transpose of one (op T / C, ld = sr)
+bool pick_op(int64_t R, int64_t Cc, int64_t sr, int64_t sc, bool conj, OpLd *o) {
+  const int64_t r1 = R > 1 ? R : 1, c1 = Cc > 1 ? Cc : 1;
+  if (!conj && (sr == 1 || R == 1)) {
+    const int64_t ld = Cc > 1 ? sc : r1;
+    if (ld >= r1 && ld <= INT32_MAX) {
+      o->op = rocblas_operation_none;
+      o->ld = (rocblas_int)ld;
+      return true;
+    }
+  }
+  if (sc == 1 || Cc == 1) {
+    const int64_t ld = R > 1 ? sr : c1;
+    if (ld >= c1 && ld <= INT32_MAX) {
+      o->op = conj ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+      o->ld = (rocblas_int)ld;
+      return true;
+    }
+  }
+  return false;
+}
+
+struct StridedPlan {
+  bool swap;  // false: C = op(A) op(B) column-major; true: C^T = op(B^T) op(A^T)
+  OpLd a, b;
+  rocblas_int ldc;
+};
+
+bool plan_strided(const ptn_gemm_strided_desc *d, StridedPlan *p) {
+  if (!d || d->M < 1 || d->N < 1 || d->K < 1 || d->M > INT32_MAX || d->N > INT32_MAX ||
+      d->K > INT32_MAX || !d->A || !d->B || !d->C ||
+      (d->dtype != PTN_C128 && d->dtype != PTN_C64))
+    return false;
+  const int64_t M = d->M, N = d->N, K = d->K;
+  if (d->c_sm == 1 || M == 1) {
+    const int64_t ldc = N > 1 ? d->c_sn : M;
+    if (ldc >= M && ldc <= INT32_MAX && pick_op(M, K, d->a_sm, d->a_sk, d->a_conj, &p->a) &&
+        pick_op(K, N, d->b_sk, d->b_sn, d->b_conj, &p->b)) {
+      p->swap = false;
+      p->ldc = (rocblas_int)ldc;
+      return true;
+    }
+  }
+  if (d->c_sn == 1 || N == 1) {
+    const int64_t ldc = M > 1 ? d->c_sm : N;
+    if (ldc >= N && ldc <= INT32_MAX && pick_op(N, K, d->b_sn, d->b_sk, d->b_conj, &p->b) &&
+        pick_op(K, M, d->a_sk, d->a_sm, d->a_conj, &p->a)) {
+      p->swap = true;
+      p->ldc = (rocblas_int)ldc;
+      return true;
+    }
+  }
+  return false;
 }
 
 // R (k x n col-major) = upper triangle of A (m x n col-major, lda = m), k = min(m, n)
@@ -252,6 +315,52 @@ extern "C" int ptn_svd(void *stream, int64_t m, int64_t n, void *A, void *S, voi
                        static_cast<rocblas_double_complex *>(Vh), (rocblas_int)k, e,
                        rocblas_outofplace, info) != rocblas_status_success) {
     ptn::set_error("rocsolver_zgesvd failed");
+    return PTN_ESOLVER;
+  }
+  return PTN_OK;
+}
+
+extern "C" int ptn_gemm_strided_ok(const ptn_gemm_strided_desc *d) {
+  StridedPlan p;
+  return plan_strided(d, &p) ? 1 : 0;
+}
+
+extern "C" int ptn_gemm_strided(void *stream, const ptn_gemm_strided_desc *d) {
+  StridedPlan p;
+  if (!plan_strided(d, &p)) {
+    ptn::set_error("ptn_gemm_strided: layout not expressible as one strided GEMM");
+    return PTN_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(g_mu);
+  rocblas_handle h;
+  int rc = get_handle(s, &h);
+  if (rc) return rc;
+  const rocblas_int M = (rocblas_int)d->M, N = (rocblas_int)d->N, K = (rocblas_int)d->K;
+  rocblas_status st;
+  if (d->dtype == PTN_C128) {
+    const rocblas_double_complex alpha(d->alpha_re, d->alpha_im);
+    const rocblas_double_complex beta(d->beta_one ? 1.0 : 0.0, 0.0);
+    auto *A = static_cast<const rocblas_double_complex *>(d->A);
+    auto *B = static_cast<const rocblas_double_complex *>(d->B);
+    auto *C = static_cast<rocblas_double_complex *>(d->C);
+    st = p.swap ? rocblas_zgemm(h, p.b.op, p.a.op, N, M, K, &alpha, B, p.b.ld, A, p.a.ld, &beta,
+                                C, p.ldc)
+                : rocblas_zgemm(h, p.a.op, p.b.op, M, N, K, &alpha, A, p.a.ld, B, p.b.ld, &beta,
+                                C, p.ldc);
+  } else {
+    const rocblas_float_complex alpha((float)d->alpha_re, (float)d->alpha_im);
+    const rocblas_float_complex beta(d->beta_one ? 1.0f : 0.0f, 0.0f);
+    auto *A = static_cast<const rocblas_float_complex *>(d->A);
+    auto *B = static_cast<const rocblas_float_complex *>(d->B);
+    auto *C = static_cast<rocblas_float_complex *>(d->C);
+    st = p.swap ? rocblas_cgemm(h, p.b.op, p.a.op, N, M, K, &alpha, B, p.b.ld, A, p.a.ld, &beta,
+                                C, p.ldc)
+                : rocblas_cgemm(h, p.a.op, p.b.op, M, N, K, &alpha, A, p.a.ld, B, p.b.ld, &beta,
+                                C, p.ldc);
+  }
+  if (st != rocblas_status_success) {
+    ptn::set_error("ptn_gemm_strided: rocblas gemm failed (%d)", (int)st);
     return PTN_ESOLVER;
   }
   return PTN_OK;

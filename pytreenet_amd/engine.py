@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import GemmDesc, PtnBackendError
+from ._lib import GemmDesc, PtnBackendError, StridedDesc
 
 Label = Hashable
 
@@ -113,13 +113,13 @@ _CACHE = _OffsetCache()
 
 
 class LaunchProfiler:
-    """Executed-flop counters and HIP-event timings of gather-GEMM launches.
+    """Executed-flop counters and HIP-event timings of contraction launches.
 
     Events are recorded on the stream the kernels are launched on (torch's
-    current stream) around every launch that the library dispatches to its
-    LARGE instantiation (tile grid >= LARGE_TILES, never split-K): these are
-    the dominant H_eff / environment GEMMs, and profilers report them as a
-    kernel of their own.
+    current stream) around every launch with a tile grid >= LARGE_TILES: the
+    dominant H_eff / environment GEMMs.  A gather-GEMM of that size runs the
+    library's LARGE instantiation (never split-K), which profilers report as a
+    kernel of its own; a strided one runs as a rocBLAS GEMM.
     """
 
     LARGE_TILES = 2048   # tile-grid size from which the library uses its LARGE instantiation
@@ -143,18 +143,22 @@ class LaunchProfiler:
     def stop(self):
         self.active = False
 
-    def summary(self, phase: Optional[str] = None, compute: Optional[str] = None) -> List[dict]:
-        """Per-shape rows for LARGE launches, optionally of one phase / precision."""
+    def summary(self, phase: Optional[str] = None, compute: Optional[str] = None,
+                backend: Optional[str] = None) -> List[dict]:
+        """Per-shape rows for LARGE launches, optionally of one phase / precision /
+        backend ("gather": ptn_gemm_gather, "blas": ptn_gemm_strided)."""
         torch.cuda.synchronize()
         agg: Dict[tuple, list] = {}
         for key, evs in self.events.items():
             if (phase is not None and key[4] != phase) or (compute is not None and
-                                                           key[3] != compute):
+                                                           key[3] != compute) or \
+                    (backend is not None and key[5] != backend):
                 continue
-            agg.setdefault(key[:4], []).extend(a.elapsed_time(b) for a, b in evs)
+            agg.setdefault(key[:4] + (key[5],), []).extend(a.elapsed_time(b) for a, b in evs)
         rows = []
         for key, ms in agg.items():
             rows.append({"M": key[0], "N": key[1], "K": key[2], "compute": key[3],
+                         "backend": key[4],
                          "flops_per_launch": 8.0 * key[0] * key[1] * key[2],
                          "launches": len(ms), "total_ms": float(sum(ms)),
                          "avg_ms": float(sum(ms) / len(ms))})
@@ -168,6 +172,7 @@ PROFILER = LaunchProfiler()
 def clear_caches():
     _CACHE.d.clear()
     _PLANS.clear()
+    _AFFINE.clear()
 
 
 _SPLITK_WS: Dict[torch.device, torch.Tensor] = {}
@@ -188,6 +193,28 @@ def _inner_is_unit(dims, strides) -> bool:
         if d > 1:
             return s == 1
     return False
+
+
+def _affine_modes(dims, strides) -> Optional[int]:
+    """Stride of the flattened index space if its modes are nested (one stride), else None."""
+    nz = [(d, s) for d, s in zip(dims, strides) if d > 1]
+    if not nz:
+        return 1
+    for (_, s0), (d1, s1) in zip(nz[:-1], nz[1:]):
+        if s0 != s1 * d1:
+            return None
+    return nz[-1][1] if nz[-1][1] > 0 else None
+
+
+def _affine_table(t: np.ndarray) -> Optional[Tuple[int, int]]:
+    """(base, stride) if the offset table is an arithmetic progression."""
+    t = np.asarray(t, dtype=np.int64)
+    if len(t) == 1:
+        return int(t[0]), 1
+    st = int(t[1] - t[0])
+    if st <= 0 or not np.array_equal(t, t[0] + st * np.arange(len(t), dtype=np.int64)):
+        return None
+    return int(t[0]), st
 
 
 # --------------------------------------------------------------------------------------
@@ -247,8 +274,19 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     K = math.prod(kdims) if kdims else 1
     ka = _inner_is_unit(kdims, key[4]) and not _inner_is_unit(mdims, key[3])
     kb = _inner_is_unit(kdims, key[5]) and not _inner_is_unit(ndims, key[6])
-    _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate, alpha)
+    aff = _AFFINE.get(key)
+    if aff is None:
+        st = [_affine_modes(d, sx) for d, sx in ((mdims, key[3]), (kdims, key[4]),
+                                                (kdims, key[5]), (ndims, key[6]),
+                                                (mdims, key[7]), (ndims, key[8]))]
+        aff = _AFFINE[key] = (False if any(x is None for x in st)
+                              else tuple((0, x) for x in st))
+    _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate, alpha,
+            aff or None)
     return out
+
+
+_AFFINE: Dict[tuple, object] = {}
 
 
 class _Recorder:
@@ -261,13 +299,81 @@ class _Recorder:
 _RECORDER: Optional[_Recorder] = None
 
 
+# strided GEMMs go to rocBLAS when the tile grid fills the chip; the gather-GEMM
+# keeps the gathered, short and split-K (long K, few tiles) cases
+BLAS_MIN_TILES = 256
+BLAS_MIN_DIM = 64
+BLAS_MAX_K_PER_TILE = 1 << 14
+USE_BLAS = True
+
+
+def _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine):
+    if not USE_BLAS or K < 1 or min(M, N) < BLAS_MIN_DIM:
+        return None
+    tiles = -(-M // 64) * -(-N // 64)
+    if tiles < BLAS_MIN_TILES or K > BLAS_MAX_K_PER_TILE:
+        return None
+    dt = A.dtype
+    if B.dtype != dt or C.dtype != dt:
+        return None
+    if not ((precision == FP64 and dt == torch.complex128) or
+            (precision == FP32 and dt == torch.complex64)):
+        return None
+    (am0, asm), (ak0, ask), (bk0, bsk), (bn0, bsn), (cm0, csm), (cn0, csn) = affine
+    sd = StridedDesc()
+    sd.M, sd.N, sd.K = M, N, K
+    es = A.element_size()
+    offs = ((am0 + ak0) * es, (bk0 + bn0) * es, (cm0 + cn0) * es)
+    sd.A, sd.a_sm, sd.a_sk, sd.a_conj = A.data_ptr() + offs[0], asm, ask, int(conj_a)
+    sd.B, sd.b_sk, sd.b_sn, sd.b_conj = B.data_ptr() + offs[1], bsk, bsn, int(conj_b)
+    sd.C, sd.c_sm, sd.c_sn = C.data_ptr() + offs[2], csm, csn
+    sd.dtype = _DTYPE_CODE[dt]
+    sd.beta_one = int(accumulate)
+    alpha = complex(alpha)
+    sd.alpha_re, sd.alpha_im = alpha.real, alpha.imag
+    if not _lib.load().ptn_gemm_strided_ok(ctypes.byref(sd)):
+        return None
+    return sd, offs
+
+
+def _run_blas(sd: StridedDesc, M: int, N: int, K: int, precision: str):
+    lib = _lib.load()
+    prof = PROFILER
+    if prof.active:
+        prof.flops += 8.0 * M * N * K
+        prof.launches += 1
+        if -(-M // 64) * -(-N // 64) >= prof.LARGE_TILES:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            _lib.check(lib.ptn_gemm_strided(ctypes.c_void_p(stream_handle()), ctypes.byref(sd)),
+                       "ptn_gemm_strided")
+            ev1.record()
+            prof.events.setdefault((M, N, K, precision, prof.phase, "blas"), []).append(
+                (ev0, ev1))
+            return
+    _lib.check(lib.ptn_gemm_strided(ctypes.c_void_p(stream_handle()), ctypes.byref(sd)),
+               "ptn_gemm_strided")
+
+
 def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, accumulate,
-            alpha):
-    """One ptn_gemm_gather launch from device offset tables
+            alpha, affine=None):
+    """One contraction launch from device offset tables
     ptrs = (a_off_m, a_off_k, b_off_k, b_off_n, c_off_m, c_off_n).
 
-    The operand with the longer free dimension is put on the M side (64-row
-    tiles waste less on a short N than on a short M)."""
+    ``affine`` holds (base, stride) of each table when all six are arithmetic
+    progressions; such a launch with a full tile grid runs as one strided
+    rocBLAS GEMM.  Otherwise ptn_gemm_gather runs, with the operand of the
+    longer free dimension on the M side (64-row tiles waste less on a short N
+    than on a short M)."""
+    if affine is not None:
+        hit = _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine)
+        if hit is not None:
+            sd, offs = hit
+            if _RECORDER is not None:
+                _RECORDER.ops.append(("blas", sd, A, B, C, M, N, K, precision, offs))
+            _run_blas(sd, M, N, K, precision)
+            return
     if M < N and N >= 64:
         A, B = B, A
         M, N = N, M
@@ -315,7 +421,8 @@ def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
             _lib.check(lib.ptn_gemm_gather(ctypes.c_void_p(stream_handle()), ctypes.byref(d)),
                        "ptn_gemm_gather")
             ev1.record()
-            prof.events.setdefault((M, N, K, precision, prof.phase), []).append((ev0, ev1))
+            prof.events.setdefault((M, N, K, precision, prof.phase, "gather"), []).append(
+                (ev0, ev1))
             return
     _lib.check(lib.ptn_gemm_gather(ctypes.c_void_p(stream_handle()), ctypes.byref(d)),
                "ptn_gemm_gather")
@@ -517,7 +624,9 @@ def _contract_block(X: _Blk, Y: torch.Tensor, ly: Sequence[Label], conj_y: bool,
             groups = []
             for tabs in host_groups:
                 _, ptrs = _CACHE.get(("blk", key, len(groups)), lambda t=tabs: list(t), dev)
-                groups.append((ptrs, len(tabs[0]), len(tabs[1]), len(tabs[3])))
+                aff = [_affine_table(t) for t in tabs]
+                groups.append((ptrs, len(tabs[0]), len(tabs[1]), len(tabs[3]),
+                               None if any(a is None for a in aff) else tuple(aff)))
             _BLK_CACHE[key] = groups
             if len(_BLK_CACHE) > 4096:
                 _BLK_CACHE.popitem(last=False)
@@ -532,9 +641,9 @@ def _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, t
         not _inner_is_unit([dims[l] for l in m_d], [sx[l] for l in m_d])
     kb = _inner_is_unit([dims[l] for l in k_d], [sy[l] for l in k_d]) and \
         not _inner_is_unit([dims[l] for l in n_l], [sy[l] for l in n_l])
-    for ptrs, M, K, N in groups:
+    for ptrs, M, K, N, aff in groups:
         _launch(X.data, Y, target, ptrs, M, N, K, ka, kb, X.conj, conj_y, precision, False,
-                1.0)
+                1.0, aff)
 
 
 _BAPPLY_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()
@@ -837,12 +946,15 @@ class CompiledNetwork:
         out = torch.empty(self.out_shape, dtype=self.out_dtype, device=var[0].device)
         for op, roles in zip(self.ops, self.op_roles):
             kind = op[0]
-            if kind == "gemm":
+            if kind == "gemm" or kind == "blas":
                 d = op[1]
                 for field, role in roles:
                     t = var[role[1]] if role[0] == "var" else out
                     setattr(d, field, t.data_ptr() + role[2])
-                _run_gemm(d, op[5], op[6], op[7], op[8])
+                if kind == "gemm":
+                    _run_gemm(d, op[5], op[6], op[7], op[8])
+                else:
+                    _run_blas(d, op[5], op[6], op[7], op[8])
             elif kind == "block":
                 Y = op[2] if roles[0] is None else (var[roles[0][1]] if roles[0][0] == "var"
                                                     else out)
@@ -870,20 +982,22 @@ class CompiledNetwork:
         self.sig = sig
         self.ops = rec.ops
 
-        def role(t):
+        def role(t, off=0):
             for k, v in enumerate(var):
                 if t is v:
-                    return ("var", k, 0)
+                    return ("var", k, off)
             if t is out:
-                return ("out", 0, 0)
+                return ("out", 0, off)
             return None
 
         self.op_roles = []
         for op in self.ops:
-            if op[0] == "gemm":
+            if op[0] == "gemm" or op[0] == "blas":
+                offs = op[9] if op[0] == "blas" else (0, 0, 0)
                 roles = []
-                for field, t in (("A", op[2]), ("B", op[3]), ("C", op[4])):
-                    r = role(t)
+                for field, t, off in (("A", op[2], offs[0]), ("B", op[3], offs[1]),
+                                      ("C", op[4], offs[2])):
+                    r = role(t, off)
                     if r is not None:
                         roles.append((field, r))
                 self.op_roles.append(roles)

@@ -7,13 +7,18 @@ breakdown at beta <= max(eps*n, 1e-12*||A q_j||), the Saad estimate
 ||psi|| * beta * |[exp(f dt T_m) e_1]_{m-1}| < krylov_tol, and the small
 propagator from eigh((T+T^H)/2) (hermitian) or expm(T).
 
-MI355X form: the basis lives in one device buffer; each step's two projection
-passes run as block classical Gram-Schmidt (CGS2 -- two passes of Q^H w /
-w -= Q h, each streaming the basis once) inside ``ptn_krylov_orthogonalize``,
-which also returns ||w||^2 before and after.  One device->host copy of the
-(j+2) scalars per step feeds the host-side m x m propagator (m <= max dim,
-FP64), as in the reference.  The two-pass CGS and the reference's two-pass MGS
-differ only in rounding.
+MI355X form: the basis lives in one device buffer.  q_0 = psi / ||psi|| is
+formed on the device (``ptn_normalize``).  Each step's two projection passes
+run as block classical Gram-Schmidt (CGS2 -- two passes of Q^H w / w -= Q h,
+each streaming the basis once) inside ``ptn_krylov_orthonormalize``, which
+also returns ||w||^2 before and after and writes q_{j+1} = w / ||w||, so the
+next H_eff application does not wait for the host.  One asynchronous
+device->host copy of the (j+3) scalars per step feeds the host-side m x m
+propagator (m <= max dim, FP64), as in the reference.  For short vectors
+(n <= LOOKAHEAD_N) the next application is queued before that copy is read:
+if the stop rule then ends the recurrence, that one application is discarded
+-- the accepted q_0..q_{m-1} and T_m are the same either way.  The two-pass CGS
+and the reference's two-pass MGS differ only in rounding.
 """
 from __future__ import annotations
 
@@ -144,7 +149,8 @@ def norm_sq(x: torch.Tensor) -> float:
 def combine(Q: torch.Tensor, coeffs, out: torch.Tensor, nvec: Optional[int] = None):
     """out = sum_i coeffs[i] Q[i] over the first nvec rows of Q."""
     nvec = len(coeffs) if nvec is None else nvec
-    c = torch.as_tensor(np.asarray(coeffs, dtype=np.complex128)).to(Q.device)
+    c = torch.as_tensor(np.asarray(coeffs, dtype=np.complex128)).pin_memory().to(
+        Q.device, non_blocking=True)
     _lib.check(_lib.load().ptn_krylov_combine(ctypes.c_void_p(E.stream_handle()),
                                               Q.shape[1], _vp(Q), Q.stride(0), nvec, _vp(c),
                                               _vp(out)), "ptn_krylov_combine")
@@ -189,6 +195,18 @@ class KrylovStats:
 STATS = KrylovStats()
 
 
+LOOKAHEAD_N = 1 << 16
+_PINNED: Dict[int, torch.Tensor] = {}
+
+
+def _pinned(count: int) -> torch.Tensor:
+    buf = _PINNED.get(count)
+    if buf is None:
+        buf = torch.empty(count, dtype=torch.complex128, pin_memory=True)
+        _PINNED[count] = buf
+    return buf
+
+
 def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.Tensor],
                       time_difference: float, exp_factor: complex,
                       krylov_tol: float = 1e-10, max_krylov_dim: Optional[int] = None,
@@ -199,31 +217,50 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     dev = psi.device
     v0 = _flat(psi.to(torch.complex128))
     n = v0.numel()
-    beta0 = math.sqrt(norm_sq(v0))
-    if beta0 == 0.0:
-        return psi.clone()
     max_steps = n if max_krylov_dim is None else min(n, int(max_krylov_dim))
-    cap = min(max_steps + 1, 16)
-    Q = torch.empty((cap, n), dtype=torch.complex128, device=dev)
-    combine(v0.view(1, n), [1.0 / beta0], Q[0])
-    hess = np.zeros((max_steps + 1, max_steps + 1), dtype=complex)
-    breakdown = np.finfo(float).eps * max(n, 1)
-    packed = torch.empty(max_steps + 2, dtype=torch.complex128, device=dev)
     lib = _lib.load()
     work = _work(dev)
+    stream = ctypes.c_void_p(E.stream_handle())
+    Q = torch.empty((min(max_steps + 1, 16), n), dtype=torch.complex128, device=dev)
+    # packed[0] = (||psi||^2, 0); packed[1..j+2) = T[:j+1, j]; packed[j+2] = (w0sq, beta^2)
+    packed = torch.empty(max_steps + 3, dtype=torch.complex128, device=dev)
+    _lib.check(lib.ptn_normalize(stream, n, _vp(v0), _vp(Q[0]), _vp(packed), _vp(work)),
+               "ptn_normalize")
+    hess = np.zeros((max_steps + 1, max_steps + 1), dtype=complex)
+    breakdown = np.finfo(float).eps * max(n, 1)
+    host_buf = _pinned(max_steps + 3)
+    done = torch.cuda.Event()
+    lookahead = n <= LOOKAHEAD_N
+    beta0 = 0.0
+    w_next = None
     m = 0
     for j in range(max_steps):
-        w = action(Q[j].view(shape))
+        if j + 2 > Q.shape[0]:
+            grown = torch.empty((min(2 * Q.shape[0], max_steps + 1), n),
+                                dtype=torch.complex128, device=dev)
+            grown[: Q.shape[0]].copy_(Q)
+            Q = grown
+        w = w_next if w_next is not None else action(Q[j].view(shape))
+        w_next = None
         w = _flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
         if w.data_ptr() == Q[j].data_ptr():
             w = w.clone()
-        _lib.check(lib.ptn_krylov_orthogonalize(ctypes.c_void_p(E.stream_handle()), n, _vp(Q),
-                                                Q.stride(0), j + 1, _vp(w), _vp(packed),
-                                                _vp(work)), "ptn_krylov_orthogonalize")
-        host = packed[: j + 2].cpu().numpy()
-        hess[: j + 1, j] = host[: j + 1]
-        w0_norm = math.sqrt(max(host[j + 1].real, 0.0))
-        beta = math.sqrt(max(host[j + 1].imag, 0.0))
+        _lib.check(lib.ptn_krylov_orthonormalize(stream, n, _vp(Q), Q.stride(0), j + 1,
+                                                 _vp(w), _vp(Q[j + 1]), _vp(packed[1:]),
+                                                 _vp(work)), "ptn_krylov_orthonormalize")
+        host_buf[: j + 3].copy_(packed[: j + 3], non_blocking=True)
+        done.record()
+        if lookahead and j + 1 < max_steps:
+            w_next = action(Q[j + 1].view(shape))
+        done.synchronize()
+        host = host_buf[: j + 3].numpy()
+        if j == 0:
+            beta0 = math.sqrt(max(host[0].real, 0.0))
+            if beta0 == 0.0:
+                return psi.clone()
+        hess[: j + 1, j] = host[1: j + 2]
+        w0_norm = math.sqrt(max(host[j + 2].real, 0.0))
+        beta = math.sqrt(max(host[j + 2].imag, 0.0))
         hess[j + 1, j] = beta
         m = j + 1
         if beta <= max(breakdown, 1e-12 * w0_norm):
@@ -232,12 +269,6 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
             col = small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
             if beta0 * beta * abs(col[m - 1]) < krylov_tol:
                 break
-        if j + 1 >= Q.shape[0]:
-            grown = torch.empty((min(2 * Q.shape[0], max_steps + 1), n),
-                                dtype=torch.complex128, device=dev)
-            grown[: Q.shape[0]].copy_(Q)
-            Q = grown
-        combine(w.view(1, n), [1.0 / beta], Q[j + 1])
     coeffs = beta0 * small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
     out = torch.empty(n, dtype=torch.complex128, device=dev)
     combine(Q, coeffs, out, nvec=m)

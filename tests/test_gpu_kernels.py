@@ -120,6 +120,71 @@ def test_krylov_orthogonalize(gpu, n, nvec):
     np.testing.assert_allclose(tw.cpu().numpy(), w2, atol=1e-10 * np.linalg.norm(w))
 
 
+@pytest.mark.parametrize("n,nvec,pad", [(4, 2, 0), (8192, 9, 3), (8193, 3, 0), (100000, 11, 5)])
+def test_krylov_orthonormalize(gpu, n, nvec, pad):
+    """One-workgroup (n <= 8192) and multi-block forms, basis rows with stride > n."""
+    _say(f"krylov orthonormalize n={n} nvec={nvec} pad={pad}")
+    import ctypes
+    from pytreenet_amd import _lib
+    rng = np.random.default_rng(7 * n + nvec)
+    nv = min(nvec, n)
+    q, _ = np.linalg.qr(_crand(rng, (n, nv)))
+    ld = n + pad
+    Qbuf = np.zeros((nv + 1, ld), dtype=complex)
+    Qbuf[:nv, :n] = q.T
+    w = _crand(rng, n)
+    tQ, tw = _dev(Qbuf), _dev(w)
+    packed = torch.empty(nv + 1, dtype=torch.complex128, device="cuda")
+    vp = ctypes.c_void_p
+    _lib.check(_lib.load().ptn_krylov_orthonormalize(
+        vp(E.stream_handle()), n, vp(tQ.data_ptr()), ld, nv, vp(tw.data_ptr()),
+        vp(tQ[nv].data_ptr()), vp(packed.data_ptr()), vp(K._work(tQ.device).data_ptr())),
+        "orthonormalize")
+    h = packed.cpu().numpy()
+    Q = q.T
+    h1 = Q.conj() @ w
+    w1 = w - Q.T @ h1
+    h2 = Q.conj() @ w1
+    w2 = w1 - Q.T @ h2
+    scale = np.linalg.norm(w)
+    np.testing.assert_allclose(h[:nv], h1 + h2, atol=1e-12 * scale)
+    assert abs(h[nv].real - scale ** 2) < 1e-12 * scale ** 2
+    assert abs(h[nv].imag - np.vdot(w2, w2).real) < 1e-12 * scale ** 2
+    np.testing.assert_allclose(tw.cpu().numpy(), w2, atol=1e-12 * scale)
+    got = tQ.cpu().numpy()
+    np.testing.assert_allclose(got[nv, :n], w2 / np.linalg.norm(w2), atol=1e-12)
+    np.testing.assert_array_equal(got[:nv], Qbuf[:nv])       # basis untouched
+    # the new vector is orthonormal to the basis
+    assert np.max(np.abs(Q.conj() @ got[nv, :n])) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 8192, 8193, 300001])
+def test_normalize(gpu, n):
+    _say(f"normalize n={n}")
+    import ctypes
+    from pytreenet_amd import _lib
+    rng = np.random.default_rng(n)
+    x = _crand(rng, n)
+    tx = _dev(x)
+    y = torch.empty_like(tx)
+    out = torch.empty(1, dtype=torch.complex128, device="cuda")
+    vp = ctypes.c_void_p
+    lib, work = _lib.load(), K._work(tx.device)
+    _lib.check(lib.ptn_normalize(vp(E.stream_handle()), n, vp(tx.data_ptr()), vp(y.data_ptr()),
+                                 vp(out.data_ptr()), vp(work.data_ptr())), "normalize")
+    nrm2 = np.vdot(x, x).real
+    assert abs(out.cpu().item() - nrm2) < 1e-12 * nrm2
+    np.testing.assert_allclose(y.cpu().numpy(), x / np.sqrt(nrm2), atol=1e-14)
+    # in place, and the zero vector maps to zero
+    _lib.check(lib.ptn_normalize(vp(E.stream_handle()), n, vp(tx.data_ptr()), vp(tx.data_ptr()),
+                                 vp(out.data_ptr()), vp(work.data_ptr())), "normalize")
+    np.testing.assert_allclose(tx.cpu().numpy(), x / np.sqrt(nrm2), atol=1e-14)
+    z = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    _lib.check(lib.ptn_normalize(vp(E.stream_handle()), n, vp(z.data_ptr()), vp(y.data_ptr()),
+                                 vp(out.data_ptr()), vp(work.data_ptr())), "normalize")
+    assert out.cpu().item() == 0 and not torch.any(y != 0)
+
+
 def test_dot_norm_combine(gpu):
     _say("dot/norm/combine")
     rng = np.random.default_rng(21)
@@ -216,3 +281,90 @@ def test_potrf_trtri(gpu):
     np.testing.assert_allclose(L @ L.conj().T, g, atol=1e-10 * np.abs(g).max())
     assert np.abs(np.triu(L, 1)).max() == 0
     np.testing.assert_allclose(X @ L, np.eye(200), atol=1e-11)
+
+
+def _layout(rng, rows, cols, colmajor, dtype):
+    """A rows x cols complex matrix stored column-major (unit row stride) or row-major."""
+    x = _crand(rng, (rows, cols)).astype(dtype)
+    if colmajor:
+        t = torch.from_numpy(np.ascontiguousarray(x.T)).to("cuda").t()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+    return x, t
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_gemm_strided_layouts(gpu, dtype):
+    """ptn_gemm_strided over every (A, B, C) layout and conjugation, against numpy;
+    ptn_gemm_strided_ok against the rule that a conjugated operand must be transposed
+    in the column-major call."""
+    import ctypes
+    from itertools import product
+    from pytreenet_amd import _lib
+    lib = _lib.load()
+    M, N, K = 96, 80, 72
+    rng = np.random.default_rng(5)
+    tol = 1e-12 if dtype == np.complex128 else 2e-5
+    code = _lib.PTN_C128 if dtype == np.complex128 else _lib.PTN_C64
+    ran = 0
+    for acol, bcol, ccol, ca, cb, acc in product([0, 1], repeat=6):
+        a, tA = _layout(rng, M, K, acol, dtype)
+        b, tB = _layout(rng, K, N, bcol, dtype)
+        c0, tC = _layout(rng, M, N, ccol, dtype)
+        alpha = 0.75 - 0.5j
+        sd = _lib.StridedDesc()
+        sd.M, sd.N, sd.K = M, N, K
+        sd.A, sd.a_sm, sd.a_sk, sd.a_conj = tA.data_ptr(), tA.stride(0), tA.stride(1), ca
+        sd.B, sd.b_sk, sd.b_sn, sd.b_conj = tB.data_ptr(), tB.stride(0), tB.stride(1), cb
+        sd.C, sd.c_sm, sd.c_sn = tC.data_ptr(), tC.stride(0), tC.stride(1)
+        sd.dtype, sd.beta_one = code, acc
+        sd.alpha_re, sd.alpha_im = alpha.real, alpha.imag
+        # column-major C: A is op N (col-major, no conj) or op T/C (row-major);
+        # row-major C runs C^T = B^T A^T: the roles of the layouts flip
+        f1 = ccol and (not ca or not acol) and (not cb or not bcol)
+        f2 = (not ccol) and (not cb or bcol) and (not ca or acol)
+        ok = lib.ptn_gemm_strided_ok(ctypes.byref(sd))
+        assert ok == int(bool(f1 or f2)), (acol, bcol, ccol, ca, cb)
+        if not ok:
+            continue
+        _lib.check(lib.ptn_gemm_strided(ctypes.c_void_p(E.stream_handle()), ctypes.byref(sd)),
+                   "strided")
+        ref = alpha * ((a.conj() if ca else a) @ (b.conj() if cb else b)) + (c0 if acc else 0)
+        got = tC.cpu().numpy()
+        np.testing.assert_allclose(got, ref, atol=tol * np.abs(ref).max())
+        ran += 1
+    assert ran == 36
+
+
+def test_contract_blas_matches_gather(gpu):
+    """Engine contractions routed to rocBLAS equal the gather-GEMM ones."""
+    rng = np.random.default_rng(17)
+    x = _dev(_crand(rng, (64, 64, 64, 2)))
+    env = _dev(_crand(rng, (64, 5, 64)))
+    cases = [((x, "abcp"), (env, "awB"), "bcpwB", False),
+             ((x, "abcp"), (env, "cwB"), "abpwB", True),
+             ((env, "awB"), (x, "abcp"), "wBbcp", False)]
+    for (A, la), (B, lb), lo, cj in cases:
+        try:
+            E.USE_BLAS = False
+            ref = E.contract(A, la, B, lb, lo, conj_b=cj).cpu().numpy()
+            E.USE_BLAS = True
+            got = E.contract(A, la, B, lb, lo, conj_b=cj).cpu().numpy()
+        finally:
+            E.USE_BLAS = True
+        np.testing.assert_allclose(got, ref, atol=1e-11 * np.abs(ref).max())
+    # the compiled replay of a network that routes through rocBLAS
+    net = E.CompiledNetwork([x, env, env], [["a", "b", "c", "p"], ["a", "w", "A"],
+                                            ["b", "v", "B"]], ["A", "w", "B", "v", "c", "p"])
+    first = net(x).cpu().numpy()
+    y = _dev(_crand(rng, (64, 64, 64, 2)))
+    again = net(y).cpu().numpy()
+    E.USE_BLAS = False
+    try:
+        ref = E.contract_network([y, env, env], [["a", "b", "c", "p"], ["a", "w", "A"],
+                                                 ["b", "v", "B"]],
+                                 ["A", "w", "B", "v", "c", "p"]).cpu().numpy()
+    finally:
+        E.USE_BLAS = True
+    np.testing.assert_allclose(again, ref, atol=1e-11 * np.abs(ref).max())
+    assert not np.allclose(first, again)
