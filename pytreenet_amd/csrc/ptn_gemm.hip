@@ -7,15 +7,18 @@
 // leg permutation of a tensordot is absorbed by the global->LDS tile loads and
 // the epilogue stores.  No transposed copy of an operand is written to HBM.
 //
-// Tiling: 256 threads = 4 wave64s in a 2x2 grid, block tile 64x64 (complex),
-// K step 16, two LDS stages.  Each wave owns a 32x32 sub-tile = 2x2 MFMA
-// blocks of 16x16.
-//   F64_4M: v_mfma_f64_16x16x4_f64, complex product as 4 real MFMAs into
-//           (re, im) accumulators.
-//   F32_3M: v_mfma_f32_16x16x4_f32 with the 3-multiply (Gauss) split:
-//           T1 = ar*br, T2 = ai*bi, T3 = (ar+ai)(br+bi); re = T1-T2,
-//           im = T3-T1-T2; operands are converted to float while staging
-//           (complex128 inputs are read as double and rounded once).
+// Tiling: 256 threads = 4 wave64s in a 2x2 grid, block tile 64x64 (complex).
+// Each wave owns a 32x32 sub-tile = 2x2 MFMA blocks of 16x16.
+//   F64_4M (gemm_direct_f64_kernel): v_mfma_f64_16x16x4_f64, complex product
+//           as 4 real MFMAs into (re, im) accumulators.  K step 8, tiles copied
+//           global->LDS by global_load_lds_dwordx4 (lane l copies element
+//           (k, m0 + l) of a tile row, whatever the operand's layout), three
+//           LDS stages, two workgroups per CU.
+//   F32_3M (gemm_gather_kernel): v_mfma_f32_16x16x4_f32 with the 3-multiply
+//           (Gauss) split: T1 = ar*br, T2 = ai*bi, T3 = (ar+ai)(br+bi);
+//           re = T1-T2, im = T3-T1-T2; operands are converted to float while
+//           staging through registers (complex128 inputs are read as double and
+//           rounded once).  K step 8, two LDS stages, padded LDS rows.
 // MFMA operand layout (16x16x4): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; accumulator register r of lane l holds column l%16 and row
 // 4*r + l/16 for the f64 instruction, row 4*(l/16) + r for the f32 one.
@@ -29,7 +32,21 @@ namespace {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+// register-staged kernel (the F32_3M path): K step, LDS row padding (in
+// elements; breaks the bank conflicts of k-major tile stores) and the
+// occupancy target in workgroups per CU
+#ifndef PTN_GEMM_BK
+#define PTN_GEMM_BK 8
+#endif
+#ifndef PTN_GEMM_PAD
+#define PTN_GEMM_PAD 1
+#endif
+#ifndef PTN_GEMM_OCC
+#define PTN_GEMM_OCC 3
+#endif
+constexpr int BM = 64, BN = 64, BK = PTN_GEMM_BK, NT = 256;
+constexpr int LA = BM * BK / NT;  // A (and B) tile elements per thread per stage
+static_assert(LA >= 1 && BM * BK % NT == 0, "tile / thread count mismatch");
 constexpr int64_t SPLITK_MIN_K = 512;
 constexpr int64_t SPLITK_TARGET_BLOCKS = 2048;  // 4 waves of 2 blocks x 256 CUs
 constexpr int64_t LARGE_TILES = 2048;
@@ -69,17 +86,18 @@ __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// LARGE = 1 is the same kernel instantiated separately for launches with a
-// large tile grid (>= LARGE_TILES tiles, no split-K), so profilers report the
-// dominant H_eff / environment GEMMs apart from the many small ones.
+// LARGE = 1 (here and in gemm_direct_f64_kernel) is the same kernel
+// instantiated separately for launches with a large tile grid (>= LARGE_TILES
+// tiles, no split-K), so profilers report the dominant H_eff / environment
+// GEMMs apart from the many small ones.
 template <int MODE, typename TA, typename TB, typename TC, int LARGE>
-__global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
+__global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) {
   typedef typename Acc<MODE>::T T;
   typedef typename Acc<MODE>::V V;
-  __shared__ T sAr[2][BK][BM];
-  __shared__ T sAi[2][BK][BM];
-  __shared__ T sBr[2][BK][BN];
-  __shared__ T sBi[2][BK][BN];
+  __shared__ T sAr[2][BK][BM + PTN_GEMM_PAD];
+  __shared__ T sAi[2][BK][BM + PTN_GEMM_PAD];
+  __shared__ T sBr[2][BK][BN + PTN_GEMM_PAD];
+  __shared__ T sBi[2][BK][BN + PTN_GEMM_PAD];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -100,15 +118,15 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
   const TB *B = static_cast<const TB *>(p.B);
 
   // ---- per-thread load coordinates ----------------------------------------
-  // A: m-major -> (ml = tid%64, kl = tid/64 + 4i); k-major -> (kl = tid%16, ml = tid/16 + 16i)
-  int64_t a_moff[4];
-  bool a_mval[4];
-  int a_kl[4], a_ml[4];
+  // A: m-major -> (ml = tid%64, kl = tid/64 + 4i); k-major -> (kl = tid%BK, ml = tid/BK + (NT/BK)i)
+  int64_t a_moff[LA];
+  bool a_mval[LA];
+  int a_kl[LA], a_ml[LA];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < LA; ++i) {
     if (p.akmaj) {
-      a_kl[i] = tid & 15;
-      a_ml[i] = (tid >> 4) + 16 * i;
+      a_kl[i] = tid % BK;
+      a_ml[i] = tid / BK + (NT / BK) * i;
     } else {
       a_kl[i] = (tid >> 6) + 4 * i;
       a_ml[i] = tid & 63;
@@ -117,14 +135,14 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
     a_mval[i] = gm < p.M;
     a_moff[i] = a_mval[i] ? p.aom[gm] : 0;
   }
-  int64_t b_noff[4];
-  bool b_nval[4];
-  int b_kl[4], b_nl[4];
+  int64_t b_noff[LA];
+  bool b_nval[LA];
+  int b_kl[LA], b_nl[LA];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < LA; ++i) {
     if (p.bkmaj) {
-      b_kl[i] = tid & 15;
-      b_nl[i] = (tid >> 4) + 16 * i;
+      b_kl[i] = tid % BK;
+      b_nl[i] = tid / BK + (NT / BK) * i;
     } else {
       b_kl[i] = (tid >> 6) + 4 * i;
       b_nl[i] = tid & 63;
@@ -137,42 +155,46 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
   // Software pipeline: the k-offsets of stage s+1 are fetched while stage s's
   // data loads are in flight, and the loaded values are only consumed (sign of
   // the conjugation applied) when they are written to LDS after the MFMA block.
-  T ra_r[4], ra_i[4], rb_r[4], rb_i[4];
-  int64_t a_koff[4], b_koff[4];
+  // Loads are branch-free: an element outside the matrix reads element 0 of the
+  // operand and is replaced by zero.
+  T ra_r[LA], ra_i[LA], rb_r[LA], rb_i[LA];
+  int64_t a_koff[LA], b_koff[LA];
   const T asg = p.aconj ? T(-1) : T(1);
   const T bsg = p.bconj ? T(-1) : T(1);
 
   auto kload = [&](int64_t k0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < LA; ++i) {
       const int64_t gk = k0 + a_kl[i];
       a_koff[i] = (gk < kend) ? p.aok[gk] : -1;
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < LA; ++i) {
       const int64_t gk = k0 + b_kl[i];
       b_koff[i] = (gk < kend) ? p.bok[gk] : -1;
     }
   };
   auto gload = [&]() {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      T re = 0, im = 0;
-      if (a_mval[i] && a_koff[i] >= 0) ptn::cload(A + a_moff[i] + a_koff[i], re, im);
-      ra_r[i] = re;
-      ra_i[i] = im;
+    for (int i = 0; i < LA; ++i) {
+      const bool ok = a_mval[i] && a_koff[i] >= 0;
+      T re, im;
+      ptn::cload(A + (ok ? a_moff[i] + a_koff[i] : 0), re, im);
+      ra_r[i] = ok ? re : T(0);
+      ra_i[i] = ok ? im : T(0);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      T re = 0, im = 0;
-      if (b_nval[i] && b_koff[i] >= 0) ptn::cload(B + b_koff[i] + b_noff[i], re, im);
-      rb_r[i] = re;
-      rb_i[i] = im;
+    for (int i = 0; i < LA; ++i) {
+      const bool ok = b_nval[i] && b_koff[i] >= 0;
+      T re, im;
+      ptn::cload(B + (ok ? b_koff[i] + b_noff[i] : 0), re, im);
+      rb_r[i] = ok ? re : T(0);
+      rb_i[i] = ok ? im : T(0);
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < LA; ++i) {
       sAr[buf][a_kl[i]][a_ml[i]] = ra_r[i];
       sAi[buf][a_kl[i]][a_ml[i]] = asg * ra_i[i];
       sBr[buf][b_kl[i]][b_nl[i]] = rb_r[i];
@@ -205,40 +227,45 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
       gload();                                   // data of stage kt+1
       if (kt + 2 < KT) kload(kbeg + (kt + 2) * BK);  // offsets of stage kt+2
     }
+    // all MFMA operand fragments of this K step are read from LDS first, so the
+    // reads of step kk+1 are in flight while step kk's MFMAs issue
+    T ar[BK / 4][2], ai[BK / 4][2], br[BK / 4][2], bi[BK / 4][2];
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       const int kr = kk * 4 + fk;
-      T ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        ar[i] = sAr[buf][kr][wm * 32 + i * 16 + fr];
-        ai[i] = sAi[buf][kr][wm * 32 + i * 16 + fr];
-        br[i] = sBr[buf][kr][wn * 32 + i * 16 + fr];
-        bi[i] = sBi[buf][kr][wn * 32 + i * 16 + fr];
+        ar[kk][i] = sAr[buf][kr][wm * 32 + i * 16 + fr];
+        ai[kk][i] = sAi[buf][kr][wm * 32 + i * 16 + fr];
+        br[kk][i] = sBr[buf][kr][wn * 32 + i * 16 + fr];
+        bi[kk][i] = sBi[buf][kr][wn * 32 + i * 16 + fr];
       }
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
       if constexpr (MODE == PTN_COMPUTE_F64_4M) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
-            c0[i][j] = mfma(ar[i], br[j], c0[i][j]);
-            c1[i][j] = mfma(ar[i], bi[j], c1[i][j]);
-            c0[i][j] = mfma(-ai[i], bi[j], c0[i][j]);
-            c1[i][j] = mfma(ai[i], br[j], c1[i][j]);
+            c0[i][j] = mfma(ar[kk][i], br[kk][j], c0[i][j]);
+            c1[i][j] = mfma(ar[kk][i], bi[kk][j], c1[i][j]);
+            c0[i][j] = mfma(-ai[kk][i], bi[kk][j], c0[i][j]);
+            c1[i][j] = mfma(ai[kk][i], br[kk][j], c1[i][j]);
           }
       } else {
         T as[2], bs[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-          as[i] = ar[i] + ai[i];
-          bs[i] = br[i] + bi[i];
+          as[i] = ar[kk][i] + ai[kk][i];
+          bs[i] = br[kk][i] + bi[kk][i];
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
-            c0[i][j] = mfma(ar[i], br[j], c0[i][j]);
-            c1[i][j] = mfma(ai[i], bi[j], c1[i][j]);
+            c0[i][j] = mfma(ar[kk][i], br[kk][j], c0[i][j]);
+            c1[i][j] = mfma(ai[kk][i], bi[kk][j], c1[i][j]);
             c2[i][j] = mfma(as[i], bs[j], c2[i][j]);
           }
       }
@@ -292,6 +319,168 @@ __global__ __launch_bounds__(NT, 2) void gemm_gather_kernel(KArgs p) {
   }
 }
 
+// ---- FP64 kernel with direct global->LDS tile loads ---------------------------
+// Each wave fills whole LDS rows of a tile with global_load_lds_dwordx4: lane l
+// copies element (k, m0 + l) straight into LDS, no register staging.  With m
+// (n) contiguous in memory a row is one coalesced 1 KB read; with k contiguous
+// the eight rows of a tile read the same cache lines in turn.  Three LDS stages form a ring: the
+// loads of stage kt+2 are issued right after the barrier that retires stage
+// kt-1, so each load has two K steps of MFMA work to arrive.  The k-offsets of
+// a tile row are wave-uniform (scalar loads).  Elements outside the matrix or
+// past the K range are read from a zero page.
+constexpr int DBK = 8, DST = 3;
+__device__ double2 g_zero_page[64];
+
+__device__ __forceinline__ void lds_dma16(const double2 *src, double2 *lds_row) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                   (__attribute__((address_space(3))) void *)lds_row, 16, 0, 0);
+}
+
+template <int LARGE>
+__global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
+    KArgs p, const int64_t *__restrict__ aok, const int64_t *__restrict__ bok) {
+  // one LDS array per ring stage, so every LDS read names the stage it reads
+  // and the compiler need not wait for the DMA of the other stages
+  __shared__ double2 sA0[DBK][BM], sA1[DBK][BM], sA2[DBK][BM];
+  __shared__ double2 sB0[DBK][BN], sB1[DBK][BN], sB2[DBK][BN];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 1, wn = wave & 1;
+  int64_t bid = blockIdx.x;
+  if (p.remap) {
+    const int64_t per = gridDim.x >> 3;
+    bid = (bid & 7) * per + (bid >> 3);
+  }
+  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = (int64_t)blockIdx.y * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const double2 *A = static_cast<const double2 *>(p.A);
+  const double2 *B = static_cast<const double2 *>(p.B);
+  const int64_t gm = m0 + lane, gn = n0 + lane;
+  const bool mval = gm < p.M, nval = gn < p.N;
+  const int64_t moff = mval ? p.aom[gm] : 0;
+  const int64_t noff = nval ? p.bon[gn] : 0;
+  const double2 *zero = g_zero_page + lane;
+  const int64_t klast = kend > kbeg ? kend - 1 : kbeg;
+
+  // rows wave and wave + 4 of the A and B tiles of stage kt into (sa, sb)
+  auto issue = [&](int64_t kt, double2 (*sa)[BM], double2 (*sb)[BN]) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int kl = wave + 4 * r;
+      const int64_t gk = kbeg + kt * DBK + kl;
+      const bool kval = gk < kend;
+      const int64_t gkc = kval ? gk : klast;
+      const int64_t ka = aok[gkc];
+      const int64_t kb = bok[gkc];
+      lds_dma16((kval && mval) ? A + moff + ka : zero, &sa[kl][0]);
+      lds_dma16((kval && nval) ? B + kb + noff : zero, &sb[kl][0]);
+    }
+  };
+
+  typedef double v4d_t __attribute__((ext_vector_type(4)));
+  v4d_t c0[2][2], c1[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      c0[i][j] = v4d_t{0, 0, 0, 0};
+      c1[i][j] = v4d_t{0, 0, 0, 0};
+    }
+  const double asg = p.aconj ? -1.0 : 1.0, bsg = p.bconj ? -1.0 : 1.0;
+  const int fr = lane & 15, fk = lane >> 4;
+
+  auto compute = [&](double2 (*sa)[BM], double2 (*sb)[BN]) {
+#pragma unroll
+    for (int kk = 0; kk < DBK / 4; ++kk) {
+      const int kr = kk * 4 + fk;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const double2 a = sa[kr][wm * 32 + i * 16 + fr];
+        const double2 b = sb[kr][wn * 32 + i * 16 + fr];
+        ar[i] = a.x;
+        ai[i] = asg * a.y;
+        br[i] = b.x;
+        bi[i] = bsg * b.y;
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], c0[i][j], 0, 0, 0);
+          c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], c1[i][j], 0, 0, 0);
+          c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], c0[i][j], 0, 0, 0);
+          c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], c1[i][j], 0, 0, 0);
+        }
+    }
+  };
+
+  // one K step: wait for this wave's loads of stage kt (the 4 loads of stage
+  // kt+1 may still fly), barrier (all rows of stage kt visible, stage kt-1
+  // retired), refill the retired buffer with stage kt+2, compute stage kt
+  const int64_t KT = kend > kbeg ? (kend - kbeg + DBK - 1) / DBK : 0;
+  auto step = [&](int64_t kt, double2 (*ca)[BM], double2 (*cb)[BN], double2 (*na)[BM],
+                  double2 (*nb)[BN]) {
+    if (kt + 1 < KT)
+      __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
+    else
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    // a plain s_barrier: __syncthreads() would add a workgroup fence, which
+    // waits for every load in flight (vmcnt(0)) including stage kt+1.  Each
+    // wave has waited for its own stage-kt rows above and its LDS reads of
+    // stage kt-1 completed before their MFMAs; the empty asm statements keep
+    // the compiler from moving LDS accesses across the barrier.
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (kt + 2 < KT) issue(kt + 2, na, nb);
+    compute(ca, cb);
+  };
+  if (KT > 0) issue(0, sA0, sB0);
+  if (KT > 1) issue(1, sA1, sB1);
+  for (int64_t kt = 0; kt < KT; kt += 3) {
+    step(kt, sA0, sB0, sA2, sB2);
+    if (kt + 1 >= KT) break;
+    step(kt + 1, sA1, sB1, sA0, sB0);
+    if (kt + 2 >= KT) break;
+    step(kt + 2, sA2, sB2, sA1, sB1);
+  }
+
+  // ---- epilogue (accumulator row of (lane, r) = 4*r + lane/16) ----------------
+  double2 *C = static_cast<double2 *>(p.C);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t cn = n0 + wn * 32 + j * 16 + fr;
+    if (cn >= p.N) continue;
+    const int64_t coff = p.con[cn];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t cm = m0 + wm * 32 + i * 16 + r * 4 + fk;
+        if (cm >= p.M) continue;
+        const double re = c0[i][j][r], im = c1[i][j][r];
+        if (gridDim.y > 1) {
+          p.ws[((int64_t)blockIdx.y * p.M + cm) * p.N + cn] = make_double2(re, im);
+          continue;
+        }
+        double ore = p.are * re - p.aim * im;
+        double oim = p.are * im + p.aim * re;
+        double2 *dst = C + p.com[cm] + coff;
+        if (p.beta_one) {
+          const double2 x = *dst;
+          ore += x.x;
+          oim += x.y;
+        }
+        *dst = make_double2(ore, oim);
+      }
+    }
+  }
+}
+
 // sum the split-K partials in split order, then alpha / beta / scatter store
 template <typename TC>
 __global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
@@ -321,12 +510,22 @@ __global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
 
 template <int MODE, typename TA, typename TB, typename TC>
 int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
-  if (ksplit == 1 && tiles >= LARGE_TILES)
-    hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
-                       dim3(NT), 0, s, a);
-  else
-    hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 0>),
-                       dim3((unsigned)tiles, ksplit), dim3(NT), 0, s, a);
+  const bool large = ksplit == 1 && tiles >= LARGE_TILES;
+  if constexpr (MODE == PTN_COMPUTE_F64_4M) {
+    if (large)
+      hipLaunchKernelGGL((gemm_direct_f64_kernel<1>), dim3((unsigned)tiles, 1), dim3(NT), 0, s,
+                         a, a.aok, a.bok);
+    else
+      hipLaunchKernelGGL((gemm_direct_f64_kernel<0>), dim3((unsigned)tiles, ksplit), dim3(NT), 0,
+                         s, a, a.aok, a.bok);
+  } else {
+    if (large)
+      hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
+                         dim3(NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 0>),
+                         dim3((unsigned)tiles, ksplit), dim3(NT), 0, s, a);
+  }
   PTN_LAUNCH_CHECK();
   if (ksplit > 1) {
     int64_t g = (a.M * a.N + 255) / 256;
