@@ -107,6 +107,48 @@ typedef struct ptn_gemm_strided_desc {
 int ptn_gemm_strided_ok(const ptn_gemm_strided_desc *desc);
 int ptn_gemm_strided(void *stream, const ptn_gemm_strided_desc *desc);
 
+/* ---- batched forms: independent problem instances in one launch ------------
+ * Many instances (disorder realisations, parameter sweeps) run the same step
+ * of the same algorithm at once.  The reference runs such instances one after
+ * another, each through the single-instance functions cited above; these
+ * entry points replace that loop for one step.  Per-instance descriptors and
+ * pointers are host arrays; the library stages them into the caller's device
+ * `work` buffer (size from the matching *_work_bytes function) on `stream`,
+ * so a `work` buffer may be reused by the next call on the same stream.
+ *
+ * ptn_gemm_gather_grouped: descs[0..count) as for ptn_gemm_gather (shapes
+ * may differ; compute mode and dtypes must agree; ws is ignored, no split-K);
+ * workgroup b of the launch runs one output tile of the descriptor whose tile
+ * range holds b.                                                             */
+int64_t ptn_gemm_grouped_work_bytes(int32_t count);
+int ptn_gemm_gather_grouped(void *stream, const ptn_gemm_desc *descs, int32_t count,
+                            void *work);
+
+/* ptn_krylov_orthonormalize / ptn_normalize / ptn_krylov_combine over
+ * `count` instances of one vector length n: instance b uses Q[b] (row stride
+ * ldq[b]), w[b], qnext[b] (qnext itself may be NULL), out[b]; combine takes
+ * coeffs as a host array of count x nvec complex128 (instance-major).      */
+int64_t ptn_krylov_batched_work_bytes(int32_t count);
+int ptn_krylov_orthonormalize_batched(void *stream, int64_t n, int32_t nvec, int32_t count,
+                                      const void *const *Q, const int64_t *ldq,
+                                      void *const *w, void *const *qnext, void *const *out,
+                                      void *work);
+int ptn_normalize_batched(void *stream, int64_t n, int32_t count, const void *const *x,
+                          void *const *y, void *const *out, void *work);
+int ptn_krylov_combine_batched(void *stream, int64_t n, int32_t nvec, int32_t count,
+                               const void *const *Q, const int64_t *ldq, const void *coeffs,
+                               void *const *out, void *work);
+
+/* ptn_fill / ptn_permute over `count` instances of one shape (dst[b], src[b]
+ * per instance).                                                            */
+int64_t ptn_ptr_batch_work_bytes(int32_t count);
+int ptn_fill_batched(void *stream, int32_t count, void *const *dst, int64_t n, int32_t dtype,
+                     double re, double im, void *work);
+int ptn_permute_batched(void *stream, int32_t count, int32_t rank, const int64_t *dims,
+                        const void *const *src, const int64_t *src_strides, int32_t src_dtype,
+                        void *const *dst, const int64_t *dst_strides, int32_t dst_dtype,
+                        int32_t conj, void *work);
+
 /* Strided copy with optional conjugation and dtype conversion:
  *   dst[sum_i idx_i * dst_strides[i]] = op(src[sum_i idx_i * src_strides[i]])
  * over the box dims[0..rank-1] (rank <= 8).  Strides are in elements.

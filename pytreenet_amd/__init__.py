@@ -21,7 +21,8 @@ from .splitting import (SplitMode, SVDParameters, ContractionMode, tensor_qr_dec
                         renormalise_singular_values)
 from .time_evolution import (TDVPConfig, TwoSiteTDVPConfig, TDVPAlgorithm, OneSiteTDVP,
                              FirstOrderOneSiteTDVP, SecondOrderOneSiteTDVP, TwoSiteTDVP,
-                             SecondOrderTwoSiteTDVP, Results, find_orthogonalisation_path)
+                             SecondOrderTwoSiteTDVP, Results, find_orthogonalisation_path,
+                             BatchedTDVP)
 from .random_states import random_tree_state, complete_tree, build_state
 from . import engine
 

@@ -19,7 +19,10 @@ EXPORTED = (
     "ptn_gemm_gather", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
     "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize", "ptn_normalize",
     "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc",
-    "ptn_gemm_strided", "ptn_gemm_strided_ok", "ptn_qr", "ptn_svd", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
+    "ptn_gemm_strided", "ptn_gemm_strided_ok", "ptn_qr", "ptn_svd",
+    "ptn_gemm_grouped_work_bytes", "ptn_gemm_gather_grouped", "ptn_krylov_batched_work_bytes",
+    "ptn_krylov_orthonormalize_batched", "ptn_normalize_batched", "ptn_krylov_combine_batched",
+    "ptn_ptr_batch_work_bytes", "ptn_fill_batched", "ptn_permute_batched", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
     "ptn_last_error", "ptn_version",
 )
 
@@ -92,6 +95,17 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_trtri_lower": (ctypes.c_int, [vp, i32, vp, vp]),
         "ptn_block_apply": (ctypes.c_int, [vp, i64, i32, i32, i32, i32, vp, i32, vp, vp, vp,
                                            i32, vp, vp, i32, vp, i32, vp, vp, vp]),
+        "ptn_gemm_grouped_work_bytes": (i64, [i32]),
+        "ptn_gemm_gather_grouped": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc), i32, vp]),
+        "ptn_krylov_batched_work_bytes": (i64, [i32]),
+        "ptn_krylov_orthonormalize_batched": (ctypes.c_int, [vp, i64, i32, i32, vp, vp, vp, vp,
+                                                             vp, vp]),
+        "ptn_normalize_batched": (ctypes.c_int, [vp, i64, i32, vp, vp, vp, vp]),
+        "ptn_krylov_combine_batched": (ctypes.c_int, [vp, i64, i32, i32, vp, vp, vp, vp, vp]),
+        "ptn_ptr_batch_work_bytes": (i64, [i32]),
+        "ptn_fill_batched": (ctypes.c_int, [vp, i32, vp, i64, i32, dbl, dbl, vp]),
+        "ptn_permute_batched": (ctypes.c_int, [vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, i32,
+                                               vp]),
         "ptn_last_error": (ctypes.c_char_p, []),
         "ptn_version": (ctypes.c_int, []),
     }

@@ -77,11 +77,13 @@ def _op_labels(ket_node: Node, op_node: Node) -> list:
 USE_BLOCK_SPARSE = True
 
 
-def operator_tensor(operator: TreeTensorNetworkOperator, node_id: str, precision: str):
+def operator_tensor(operator: TreeTensorNetworkOperator, node_id: str, precision: str,
+                    block: Optional[bool] = None):
     """TTNO node tensor as handed to the networks: block sparse over its bond-state
-    legs (structural zeros skipped exactly) unless ``USE_BLOCK_SPARSE`` is off."""
+    legs (structural zeros skipped exactly) unless ``block`` (default
+    ``USE_BLOCK_SPARSE``) is off."""
     dt = E.storage_dtype(precision)
-    if USE_BLOCK_SPARSE:
+    if USE_BLOCK_SPARSE if block is None else block:
         return operator.block_tensor(node_id, dt)
     return operator.tensor_in(node_id, dt)
 

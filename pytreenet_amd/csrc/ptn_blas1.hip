@@ -10,6 +10,9 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <mutex>
+#include <vector>
+
 namespace ptn {
 static thread_local char g_err[512];
 void set_error(const char *fmt, ...) {
@@ -17,6 +20,46 @@ void set_error(const char *fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
+}
+}  // namespace ptn
+
+namespace ptn {
+namespace {
+struct StageSlot {
+  void *host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+constexpr int NSLOT = 8;
+std::mutex g_stage_mu;
+StageSlot g_slots[NSLOT];
+int g_next_slot = 0;
+}  // namespace
+
+int stage_to_device(hipStream_t s, const void *src, size_t bytes, void *dst) {
+  if (bytes == 0) return PTN_OK;
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  StageSlot &sl = g_slots[g_next_slot];
+  g_next_slot = (g_next_slot + 1) % NSLOT;
+  if (sl.pending) {  // the copy that last read this slot has finished
+    PTN_HIP_CHECK(hipEventSynchronize(sl.ev));
+    sl.pending = false;
+  }
+  if (sl.cap < bytes) {
+    if (sl.host) PTN_HIP_CHECK(hipHostFree(sl.host));
+    sl.host = nullptr;
+    sl.cap = 0;
+    const size_t cap = ((bytes + 65535) / 65536) * 65536;
+    PTN_HIP_CHECK(hipHostMalloc(&sl.host, cap, hipHostMallocDefault));
+    sl.cap = cap;
+  }
+  if (!sl.ev) PTN_HIP_CHECK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+  memcpy(sl.host, src, bytes);
+  PTN_HIP_CHECK(hipMemcpyAsync(dst, sl.host, bytes, hipMemcpyHostToDevice, s));
+  PTN_HIP_CHECK(hipEventRecord(sl.ev, s));
+  sl.pending = true;
+  return PTN_OK;
 }
 }  // namespace ptn
 
@@ -78,15 +121,15 @@ __device__ void block_reduce_store(double2 (&acc)[CNT], int cnt, double2 *out_ro
 
 // partial[b][v] = sum over block b's range of conj(Q[v0+v]) * w, v < cnt;
 // if self_slot >= 0 the slot `self_slot` gets sum |w|^2 (imag 0).
-__global__ __launch_bounds__(RT) void dots_partial(int64_t n, const double2 *Q, int64_t ldq,
-                                                   int v0, int cnt, const double2 *w,
-                                                   double2 *partial, int pstride,
-                                                   int self_slot) {
+__device__ __forceinline__ void dots_partial_body(int64_t n, const double2 *Q, int64_t ldq,
+                                                  int v0, int cnt, const double2 *w,
+                                                  double2 *partial, int pstride, int self_slot,
+                                                  int blk, int nblk) {
   double2 acc[KV + 1];
 #pragma unroll
   for (int v = 0; v <= KV; ++v) acc[v] = make_double2(0, 0);
-  const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+  const int64_t stride = (int64_t)nblk * RT;
+  for (int64_t i = (int64_t)blk * RT + threadIdx.x; i < n; i += stride) {
     const double2 wi = w[i];
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
@@ -99,7 +142,7 @@ __global__ __launch_bounds__(RT) void dots_partial(int64_t n, const double2 *Q, 
     }
     if (self_slot >= 0) acc[KV].x += wi.x * wi.x + wi.y * wi.y;
   }
-  double2 *row = partial + (int64_t)blockIdx.x * pstride;
+  double2 *row = partial + (int64_t)blk * pstride;
   // move the self term next to the dots so one reduction covers both
   if (self_slot >= 0) {
 #pragma unroll
@@ -109,10 +152,16 @@ __global__ __launch_bounds__(RT) void dots_partial(int64_t n, const double2 *Q, 
   block_reduce_store<KV + 1>(acc, cnt + (self_slot >= 0 ? 1 : 0), row + v0);
 }
 
+__global__ __launch_bounds__(RT) void dots_partial(int64_t n, const double2 *Q, int64_t ldq,
+                                                   int v0, int cnt, const double2 *w,
+                                                   double2 *partial, int pstride,
+                                                   int self_slot) {
+  dots_partial_body(n, Q, ldq, v0, cnt, w, partial, pstride, self_slot, blockIdx.x, gridDim.x);
+}
+
 // out[v] = sum_b partial[b][v]; one block per v, fixed summation order
-__global__ __launch_bounds__(RT) void reduce_cols(const double2 *partial, int pstride, int nblk,
-                                                  double2 *out) {
-  const int v = blockIdx.x;
+__device__ __forceinline__ void reduce_cols_body(const double2 *partial, int pstride, int nblk,
+                                                 double2 *out, int v) {
   double2 acc[1] = {make_double2(0, 0)};
   for (int b = threadIdx.x; b < nblk; b += RT) {
     const double2 t = partial[(int64_t)b * pstride + v];
@@ -122,13 +171,17 @@ __global__ __launch_bounds__(RT) void reduce_cols(const double2 *partial, int ps
   block_sum<RT, 1>(acc, 1, out + v);
 }
 
+__global__ __launch_bounds__(RT) void reduce_cols(const double2 *partial, int pstride, int nblk,
+                                                  double2 *out) {
+  reduce_cols_body(partial, pstride, nblk, out, blockIdx.x);
+}
+
 // bsq = sum_b partial[b][0].x; writes out[0..nvec) = h1 + h2 and
 // out[nvec] = (w0sq, bsq) (nvec > 0), or out[0] = (bsq, 0) (nvec == 0), and
 // inv = 1 / sqrt(bsq) (0 when bsq <= 0)
-__global__ __launch_bounds__(RT) void finish_kernel(const double2 *partial, int pstride,
-                                                    int nblk, int nvec, const double2 *h1,
-                                                    const double2 *h2, double2 *out,
-                                                    double2 *inv) {
+__device__ __forceinline__ void finish_body(const double2 *partial, int pstride, int nblk,
+                                            int nvec, const double2 *h1, const double2 *h2,
+                                            double2 *out, double2 *inv) {
   __shared__ double2 tot;
   double2 acc[1] = {make_double2(0, 0)};
   for (int b = threadIdx.x; b < nblk; b += RT) acc[0].x += partial[(int64_t)b * pstride].x;
@@ -142,15 +195,27 @@ __global__ __launch_bounds__(RT) void finish_kernel(const double2 *partial, int 
   }
 }
 
+__global__ __launch_bounds__(RT) void finish_kernel(const double2 *partial, int pstride,
+                                                    int nblk, int nvec, const double2 *h1,
+                                                    const double2 *h2, double2 *out,
+                                                    double2 *inv) {
+  finish_body(partial, pstride, nblk, nvec, h1, h2, out, inv);
+}
+
 // y = x * inv.x
-__global__ __launch_bounds__(RT) void scale_kernel(int64_t n, const double2 *x,
-                                                   const double2 *inv, double2 *y) {
+__device__ __forceinline__ void scale_body(int64_t n, const double2 *x, const double2 *inv,
+                                           double2 *y, int blk, int nblk) {
   const double c = inv->x;
-  const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+  const int64_t stride = (int64_t)nblk * RT;
+  for (int64_t i = (int64_t)blk * RT + threadIdx.x; i < n; i += stride) {
     const double2 v = x[i];
     y[i] = make_double2(c * v.x, c * v.y);
   }
+}
+
+__global__ __launch_bounds__(RT) void scale_kernel(int64_t n, const double2 *x,
+                                                   const double2 *inv, double2 *y) {
+  scale_body(n, x, inv, y, blockIdx.x, gridDim.x);
 }
 
 // ---- one-workgroup forms for short vectors (n <= SMALL_N) --------------------
@@ -161,9 +226,9 @@ __global__ __launch_bounds__(RT) void scale_kernel(int64_t n, const double2 *x,
 constexpr int ST = 1024;
 constexpr int64_t SMALL_N = 8192;
 
-__global__ __launch_bounds__(ST) void orth_small(int64_t n, const double2 *Q, int64_t ldq,
-                                                 int nvec, double2 *w, double2 *qnext,
-                                                 double2 *out) {
+__device__ __forceinline__ void orth_small_body(int64_t n, const double2 *Q, int64_t ldq,
+                                                int nvec, double2 *w, double2 *qnext,
+                                                double2 *out) {
   __shared__ double2 h[2][MAXV + 1];
   __shared__ double2 bsq;
   for (int pass = 0; pass < 2; ++pass) {
@@ -221,9 +286,15 @@ __global__ __launch_bounds__(ST) void orth_small(int64_t n, const double2 *Q, in
   }
 }
 
+__global__ __launch_bounds__(ST) void orth_small(int64_t n, const double2 *Q, int64_t ldq,
+                                                 int nvec, double2 *w, double2 *qnext,
+                                                 double2 *out) {
+  orth_small_body(n, Q, ldq, nvec, w, qnext, out);
+}
+
 // y = x / |x|, out[0] = (|x|^2, 0)
-__global__ __launch_bounds__(ST) void normalize_small(int64_t n, const double2 *x, double2 *y,
-                                                      double2 *out) {
+__device__ __forceinline__ void normalize_small_body(int64_t n, const double2 *x, double2 *y,
+                                                     double2 *out) {
   __shared__ double2 tot;
   double2 acc[1] = {make_double2(0, 0)};
   for (int64_t i = threadIdx.x; i < n; i += ST) {
@@ -239,16 +310,22 @@ __global__ __launch_bounds__(ST) void normalize_small(int64_t n, const double2 *
   }
 }
 
+__global__ __launch_bounds__(ST) void normalize_small(int64_t n, const double2 *x, double2 *y,
+                                                      double2 *out) {
+  normalize_small_body(n, x, y, out);
+}
+
 // w -= sum_v h[v] Q[v];  optionally partial |w|^2 into partial[b][0]
-__global__ __launch_bounds__(RT) void subtract_proj(int64_t n, const double2 *Q, int64_t ldq,
-                                                    int nvec, const double2 *h, double2 *w,
-                                                    double2 *partial, int pstride) {
+__device__ __forceinline__ void subtract_proj_body(int64_t n, const double2 *Q, int64_t ldq,
+                                                   int nvec, const double2 *h, double2 *w,
+                                                   double2 *partial, int pstride, int blk,
+                                                   int nblk) {
   __shared__ double2 hs[MAXV];
   for (int v = threadIdx.x; v < nvec; v += RT) hs[v] = h[v];
   __syncthreads();
   double nrm = 0;
-  const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+  const int64_t stride = (int64_t)nblk * RT;
+  for (int64_t i = (int64_t)blk * RT + threadIdx.x; i < n; i += stride) {
     double2 wi = w[i];
     for (int v = 0; v < nvec; ++v) {
       const double2 q = Q[(int64_t)v * ldq + i];
@@ -261,17 +338,23 @@ __global__ __launch_bounds__(RT) void subtract_proj(int64_t n, const double2 *Q,
   }
   if (partial) {
     double2 acc[1] = {make_double2(nrm, 0)};
-    block_reduce_store<1>(acc, 1, partial + (int64_t)blockIdx.x * pstride);
+    block_reduce_store<1>(acc, 1, partial + (int64_t)blk * pstride);
   }
 }
 
-__global__ __launch_bounds__(RT) void combine_kernel(int64_t n, const double2 *Q, int64_t ldq,
-                                                     int nvec, const double2 *c, double2 *out) {
+__global__ __launch_bounds__(RT) void subtract_proj(int64_t n, const double2 *Q, int64_t ldq,
+                                                    int nvec, const double2 *h, double2 *w,
+                                                    double2 *partial, int pstride) {
+  subtract_proj_body(n, Q, ldq, nvec, h, w, partial, pstride, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void combine_body(int64_t n, const double2 *Q, int64_t ldq, int nvec,
+                                             const double2 *c, double2 *out, int blk, int nblk) {
   __shared__ double2 cs[MAXV];
   for (int v = threadIdx.x; v < nvec; v += RT) cs[v] = c[v];
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * RT;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+  const int64_t stride = (int64_t)nblk * RT;
+  for (int64_t i = (int64_t)blk * RT + threadIdx.x; i < n; i += stride) {
     double2 s = make_double2(0, 0);
     for (int v = 0; v < nvec; ++v) {
       const double2 q = Q[(int64_t)v * ldq + i];
@@ -280,6 +363,11 @@ __global__ __launch_bounds__(RT) void combine_kernel(int64_t n, const double2 *Q
     }
     out[i] = s;
   }
+}
+
+__global__ __launch_bounds__(RT) void combine_kernel(int64_t n, const double2 *Q, int64_t ldq,
+                                                     int nvec, const double2 *c, double2 *out) {
+  combine_body(n, Q, ldq, nvec, c, out, blockIdx.x, gridDim.x);
 }
 
 // ---- strided permutation copy ------------------------------------------------
@@ -323,6 +411,119 @@ inline unsigned grid_for(int64_t total) {
   if (g < 1) g = 1;
   if (g > 65536) g = 65536;
   return (unsigned)g;
+}
+
+// ---- batched forms: one launch over the instances of a shape group ------------
+// Every instance has its own vectors (same length n); per-instance pointers
+// come from a device array of BVec.  Short vectors: one workgroup per
+// instance.  Long vectors: blockIdx.y is the instance, blockIdx.x one of BNB
+// slices, with per-instance partial rows and scratch in the work buffer.
+struct BVec {
+  const double2 *Q;  // basis rows (orthonormalize / combine) or x (normalize)
+  int64_t ldq;
+  double2 *w;        // vector to orthogonalise (orthonormalize) or y (normalize)
+  double2 *qnext;    // may be null
+  double2 *out;      // packed scalars
+  const double2 *c;  // combine coefficients
+};
+constexpr int BNB = 64;                      // slices per instance (long vectors)
+constexpr int BPSTRIDE = MAXV + 2;           // partial row length
+constexpr int BSCR = 2 * (MAXV + 1) + 2;     // h1, h2, inv per instance
+
+__device__ __forceinline__ double2 *bpart(double2 *work, int b) {
+  return work + (int64_t)b * (BNB * BPSTRIDE + BSCR);
+}
+
+__global__ __launch_bounds__(ST) void orth_small_batched(const BVec *__restrict__ v, int64_t n,
+                                                         int nvec) {
+  const BVec d = v[blockIdx.x];
+  orth_small_body(n, d.Q, d.ldq, nvec, d.w, d.qnext, d.out);
+}
+
+__global__ __launch_bounds__(ST) void normalize_small_batched(const BVec *__restrict__ v,
+                                                              int64_t n) {
+  const BVec d = v[blockIdx.x];
+  normalize_small_body(n, d.Q, d.w, d.out);
+}
+
+__global__ __launch_bounds__(RT) void dots_partial_batched(const BVec *__restrict__ v, int64_t n,
+                                                           int v0, int cnt, int self_slot,
+                                                           int use_q, double2 *work) {
+  const BVec d = v[blockIdx.y];
+  const double2 *x = d.w ? d.w : d.Q;
+  dots_partial_body(n, use_q ? d.Q : x, use_q ? d.ldq : n, v0, cnt, x, bpart(work, blockIdx.y),
+                    BPSTRIDE, self_slot, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(RT) void reduce_cols_batched(int nblk, int which, double2 *work) {
+  double2 *p = bpart(work, blockIdx.y);
+  double2 *h = p + BNB * BPSTRIDE + (which ? (MAXV + 1) : 0);
+  reduce_cols_body(p, BPSTRIDE, nblk, h, blockIdx.x);
+}
+
+__global__ __launch_bounds__(RT) void subtract_proj_batched(const BVec *__restrict__ v, int64_t n,
+                                                            int nvec, int which, int keep_norm,
+                                                            double2 *work) {
+  const BVec d = v[blockIdx.y];
+  double2 *p = bpart(work, blockIdx.y);
+  const double2 *h = p + BNB * BPSTRIDE + (which ? (MAXV + 1) : 0);
+  subtract_proj_body(n, d.Q, d.ldq, nvec, h, d.w, keep_norm ? p : nullptr, BPSTRIDE, blockIdx.x,
+                     gridDim.x);
+}
+
+__global__ __launch_bounds__(RT) void finish_batched(const BVec *__restrict__ v, int nblk, int nvec,
+                                                     double2 *work) {
+  const BVec d = v[blockIdx.x];
+  double2 *p = bpart(work, blockIdx.x);
+  double2 *h1 = p + BNB * BPSTRIDE, *h2 = h1 + (MAXV + 1), *inv = h2 + (MAXV + 1);
+  finish_body(p, BPSTRIDE, nblk, nvec, h1, h2, d.out, inv);
+}
+
+// y = x * inv: orthonormalize (x = w, y = qnext) or normalize (x = Q, y = w)
+__global__ __launch_bounds__(RT) void scale_batched(const BVec *__restrict__ v, int64_t n,
+                                                    int normalize, double2 *work) {
+  const BVec d = v[blockIdx.y];
+  const double2 *inv = bpart(work, blockIdx.y) + BNB * BPSTRIDE + 2 * (MAXV + 1);
+  if (normalize)
+    scale_body(n, d.Q, inv, d.w, blockIdx.x, gridDim.x);
+  else if (d.qnext)
+    scale_body(n, d.w, inv, d.qnext, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(RT) void combine_batched(const BVec *__restrict__ v, int64_t n,
+                                                      int nvec) {
+  const BVec d = v[blockIdx.y];
+  combine_body(n, d.Q, d.ldq, nvec, d.c, d.out, blockIdx.x, gridDim.x);
+}
+
+template <typename TD>
+__global__ void fill_batched(TD *const *__restrict__ dst, int64_t n, double re, double im) {
+  TD *p = dst[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    ptn::cstore(p + i, re, im);
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void permute_batched(PermArgs a, const TS *const *__restrict__ src,
+                                                       TD *const *__restrict__ dst) {
+  const TS *sp = src[blockIdx.y];
+  TD *dp = dst[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < a.total;
+       lin += stride) {
+    int64_t rem = lin, so = 0, dof = 0;
+    for (int d = a.rank - 1; d >= 0; --d) {
+      const int64_t idx = rem % a.dims[d];
+      rem /= a.dims[d];
+      so += idx * a.ss[d];
+      dof += idx * a.ds[d];
+    }
+    double re, im;
+    ptn::cload(sp + so, re, im);
+    if (a.conj) im = -im;
+    ptn::cstore(dp + dof, re, im);
+  }
 }
 
 }  // namespace
@@ -531,6 +732,267 @@ extern "C" int ptn_fill(void *stream, void *dst, int64_t n, int32_t dtype, doubl
   else
     hipLaunchKernelGGL((fill_kernel<float2>), dim3(grid_for(n)), dim3(256), 0, s,
                        static_cast<float2 *>(dst), n, re, im);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+// ---- batched entry points ---------------------------------------------------------
+namespace {
+constexpr int BMAX = 65535;  // instances per batched call (grid y extent)
+
+int64_t bvec_bytes(int32_t count) { return (((int64_t)count * sizeof(BVec) + 255) / 256) * 256; }
+
+int check_batch(int32_t count, const char *who) {
+  if (count < 1 || count > BMAX) {
+    ptn::set_error("%s: batch count %d outside [1, %d]", who, count, BMAX);
+    return PTN_ERANGE;
+  }
+  return PTN_OK;
+}
+
+// stage the per-instance descriptors; returns the device copy
+int stage_bvec(hipStream_t s, const std::vector<BVec> &h, void *work, BVec **dev) {
+  *dev = static_cast<BVec *>(work);
+  return ptn::stage_to_device(s, h.data(), h.size() * sizeof(BVec), work);
+}
+
+double2 *batch_scratch(void *work, int32_t count) {
+  return reinterpret_cast<double2 *>(static_cast<char *>(work) + bvec_bytes(count));
+}
+}  // namespace
+
+extern "C" int64_t ptn_krylov_batched_work_bytes(int32_t count) {
+  if (count < 1) return 0;
+  return bvec_bytes(count) + (int64_t)count * (BNB * BPSTRIDE + BSCR) * sizeof(double2) +
+         (int64_t)count * MAXV * sizeof(double2);
+}
+
+extern "C" int ptn_krylov_orthonormalize_batched(void *stream, int64_t n, int32_t nvec,
+                                                 int32_t count, const void *const *Q,
+                                                 const int64_t *ldq, void *const *w,
+                                                 void *const *qnext, void *const *out,
+                                                 void *work) {
+  int rc = check_batch(count, "ptn_krylov_orthonormalize_batched");
+  if (rc) return rc;
+  if (n <= 0 || nvec < 1 || nvec > MAXV || !Q || !ldq || !w || !out || !work) {
+    ptn::set_error("ptn_krylov_orthonormalize_batched: bad arguments (n=%lld nvec=%d)",
+                   (long long)n, nvec);
+    return nvec > MAXV ? PTN_ERANGE : PTN_EINVAL;
+  }
+  std::vector<BVec> h(count);
+  for (int b = 0; b < count; ++b) {
+    if (!Q[b] || !w[b] || !out[b] || ldq[b] < n) {
+      ptn::set_error("ptn_krylov_orthonormalize_batched: bad instance %d", b);
+      return PTN_EINVAL;
+    }
+    h[b] = BVec{static_cast<const double2 *>(Q[b]), ldq[b], static_cast<double2 *>(w[b]),
+                qnext ? static_cast<double2 *>(qnext[b]) : nullptr,
+                static_cast<double2 *>(out[b]), nullptr};
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  BVec *dv;
+  if ((rc = stage_bvec(s, h, work, &dv))) return rc;
+  if (n <= SMALL_N) {
+    hipLaunchKernelGGL(orth_small_batched, dim3(count), dim3(ST), 0, s, dv, n, nvec);
+    PTN_LAUNCH_CHECK();
+    return PTN_OK;
+  }
+  double2 *scr = batch_scratch(work, count);
+  int nb = nblocks_for(n);
+  if (nb > BNB) nb = BNB;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int v0 = 0; v0 < nvec; v0 += KV) {
+      const int cnt = (nvec - v0) < KV ? (nvec - v0) : KV;
+      const int self_slot = (pass == 0 && v0 + KV >= nvec) ? 1 : -1;
+      hipLaunchKernelGGL(dots_partial_batched, dim3(nb, count), dim3(RT), 0, s, dv, n, v0, cnt,
+                         self_slot, 1, scr);
+      PTN_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(reduce_cols_batched, dim3(nvec + (pass == 0 ? 1 : 0), count), dim3(RT), 0,
+                       s, nb, pass, scr);
+    PTN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(subtract_proj_batched, dim3(nb, count), dim3(RT), 0, s, dv, n, nvec, pass,
+                       pass, scr);
+    PTN_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(finish_batched, dim3(count), dim3(RT), 0, s, dv, nb, nvec, scr);
+  PTN_LAUNCH_CHECK();
+  if (qnext) {
+    hipLaunchKernelGGL(scale_batched, dim3(nb, count), dim3(RT), 0, s, dv, n, 0, scr);
+    PTN_LAUNCH_CHECK();
+  }
+  return PTN_OK;
+}
+
+extern "C" int ptn_normalize_batched(void *stream, int64_t n, int32_t count,
+                                     const void *const *x, void *const *y, void *const *out,
+                                     void *work) {
+  int rc = check_batch(count, "ptn_normalize_batched");
+  if (rc) return rc;
+  if (n <= 0 || !x || !y || !out || !work) {
+    ptn::set_error("ptn_normalize_batched: bad arguments");
+    return PTN_EINVAL;
+  }
+  std::vector<BVec> h(count);
+  for (int b = 0; b < count; ++b) {
+    if (!x[b] || !y[b] || !out[b]) {
+      ptn::set_error("ptn_normalize_batched: bad instance %d", b);
+      return PTN_EINVAL;
+    }
+    h[b] = BVec{static_cast<const double2 *>(x[b]), n, static_cast<double2 *>(y[b]), nullptr,
+                static_cast<double2 *>(out[b]), nullptr};
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  BVec *dv;
+  if ((rc = stage_bvec(s, h, work, &dv))) return rc;
+  if (n <= SMALL_N) {
+    hipLaunchKernelGGL(normalize_small_batched, dim3(count), dim3(ST), 0, s, dv, n);
+    PTN_LAUNCH_CHECK();
+    return PTN_OK;
+  }
+  double2 *scr = batch_scratch(work, count);
+  int nb = nblocks_for(n);
+  if (nb > BNB) nb = BNB;
+  // self norm of x (= Q of the descriptor, w unused): dots with cnt = 0, column 0
+  std::vector<BVec> hx(h);
+  for (auto &e : hx) e.w = nullptr;
+  BVec *dx = reinterpret_cast<BVec *>(reinterpret_cast<char *>(scr) +
+                                      (int64_t)count * (BNB * BPSTRIDE + BSCR) * sizeof(double2));
+  // the x-only descriptors go to the coefficient area (count * MAXV complex >= count BVec)
+  static_assert(sizeof(BVec) <= MAXV * sizeof(double2), "descriptor size");
+  if ((rc = ptn::stage_to_device(s, hx.data(), hx.size() * sizeof(BVec), dx))) return rc;
+  hipLaunchKernelGGL(dots_partial_batched, dim3(nb, count), dim3(RT), 0, s, dx, n, 0, 0, 1, 0,
+                     scr);
+  PTN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(finish_batched, dim3(count), dim3(RT), 0, s, dv, nb, 0, scr);
+  PTN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scale_batched, dim3(nb, count), dim3(RT), 0, s, dv, n, 1, scr);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+extern "C" int ptn_krylov_combine_batched(void *stream, int64_t n, int32_t nvec, int32_t count,
+                                          const void *const *Q, const int64_t *ldq,
+                                          const void *coeffs, void *const *out, void *work) {
+  int rc = check_batch(count, "ptn_krylov_combine_batched");
+  if (rc) return rc;
+  if (n <= 0 || nvec < 1 || nvec > MAXV || !Q || !ldq || !coeffs || !out || !work) {
+    ptn::set_error("ptn_krylov_combine_batched: bad arguments");
+    return nvec > MAXV ? PTN_ERANGE : PTN_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double2 *scr = batch_scratch(work, count);
+  double2 *cdev = scr + (int64_t)count * (BNB * BPSTRIDE + BSCR);
+  if ((rc = ptn::stage_to_device(s, coeffs, (size_t)count * nvec * sizeof(double2), cdev)))
+    return rc;
+  std::vector<BVec> h(count);
+  for (int b = 0; b < count; ++b) {
+    if (!Q[b] || !out[b] || ldq[b] < n) {
+      ptn::set_error("ptn_krylov_combine_batched: bad instance %d", b);
+      return PTN_EINVAL;
+    }
+    h[b] = BVec{static_cast<const double2 *>(Q[b]), ldq[b], nullptr, nullptr,
+                static_cast<double2 *>(out[b]), cdev + (int64_t)b * nvec};
+  }
+  BVec *dv;
+  if ((rc = stage_bvec(s, h, work, &dv))) return rc;
+  int nb = nblocks_for(n);
+  if (nb > BNB) nb = BNB;
+  hipLaunchKernelGGL(combine_batched, dim3(nb, count), dim3(RT), 0, s, dv, n, nvec);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+extern "C" int64_t ptn_ptr_batch_work_bytes(int32_t count) {
+  return count < 1 ? 0 : 2 * ((((int64_t)count * 8) + 255) / 256) * 256;
+}
+
+extern "C" int ptn_fill_batched(void *stream, int32_t count, void *const *dst, int64_t n,
+                                int32_t dtype, double re, double im, void *work) {
+  int rc = check_batch(count, "ptn_fill_batched");
+  if (rc) return rc;
+  if (n < 0 || !dst || (unsigned)dtype > 1 || !work) {
+    ptn::set_error("ptn_fill_batched: bad arguments");
+    return PTN_EINVAL;
+  }
+  if (n == 0) return PTN_OK;
+  for (int b = 0; b < count; ++b)
+    if (!dst[b]) {
+      ptn::set_error("ptn_fill_batched: null destination %d", b);
+      return PTN_EINVAL;
+    }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = ptn::stage_to_device(s, dst, (size_t)count * sizeof(void *), work))) return rc;
+  unsigned g = grid_for(n);
+  if (g > 1024) g = 1024;
+  if (dtype == PTN_C128)
+    hipLaunchKernelGGL((fill_batched<double2>), dim3(g, count), dim3(256), 0, s,
+                       static_cast<double2 *const *>(work), n, re, im);
+  else
+    hipLaunchKernelGGL((fill_batched<float2>), dim3(g, count), dim3(256), 0, s,
+                       static_cast<float2 *const *>(work), n, re, im);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+extern "C" int ptn_permute_batched(void *stream, int32_t count, int32_t rank, const int64_t *dims,
+                                   const void *const *src, const int64_t *src_strides,
+                                   int32_t src_dtype, void *const *dst,
+                                   const int64_t *dst_strides, int32_t dst_dtype, int32_t conj,
+                                   void *work) {
+  int rc = check_batch(count, "ptn_permute_batched");
+  if (rc) return rc;
+  if (rank < 0 || rank > 8 || !src || !dst || !work ||
+      (rank > 0 && (!dims || !src_strides || !dst_strides)) || (unsigned)src_dtype > 1 ||
+      (unsigned)dst_dtype > 1) {
+    ptn::set_error("ptn_permute_batched: bad arguments");
+    return PTN_EINVAL;
+  }
+  PermArgs a;
+  memset(&a, 0, sizeof(a));
+  a.rank = rank;
+  a.conj = conj;
+  a.total = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] < 0) {
+      ptn::set_error("ptn_permute_batched: negative dim");
+      return PTN_EINVAL;
+    }
+    a.dims[i] = dims[i];
+    a.ss[i] = src_strides[i];
+    a.ds[i] = dst_strides[i];
+    a.total *= dims[i];
+  }
+  if (a.total == 0) return PTN_OK;
+  for (int b = 0; b < count; ++b)
+    if (!src[b] || !dst[b]) {
+      ptn::set_error("ptn_permute_batched: null pointer in instance %d", b);
+      return PTN_EINVAL;
+    }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char *wb = static_cast<char *>(work);
+  const int64_t half = ptn_ptr_batch_work_bytes(count) / 2;
+  if ((rc = ptn::stage_to_device(s, src, (size_t)count * sizeof(void *), wb))) return rc;
+  if ((rc = ptn::stage_to_device(s, dst, (size_t)count * sizeof(void *), wb + half))) return rc;
+  unsigned g = grid_for(a.total);
+  if (g > 1024) g = 1024;
+  const dim3 grid(g, count);
+  if (src_dtype == PTN_C128 && dst_dtype == PTN_C128)
+    hipLaunchKernelGGL((permute_batched<double2, double2>), grid, dim3(256), 0, s, a,
+                       reinterpret_cast<const double2 *const *>(wb),
+                       reinterpret_cast<double2 *const *>(wb + half));
+  else if (src_dtype == PTN_C128)
+    hipLaunchKernelGGL((permute_batched<double2, float2>), grid, dim3(256), 0, s, a,
+                       reinterpret_cast<const double2 *const *>(wb),
+                       reinterpret_cast<float2 *const *>(wb + half));
+  else if (dst_dtype == PTN_C128)
+    hipLaunchKernelGGL((permute_batched<float2, double2>), grid, dim3(256), 0, s, a,
+                       reinterpret_cast<const float2 *const *>(wb),
+                       reinterpret_cast<double2 *const *>(wb + half));
+  else
+    hipLaunchKernelGGL((permute_batched<float2, float2>), grid, dim3(256), 0, s, a,
+                       reinterpret_cast<const float2 *const *>(wb),
+                       reinterpret_cast<float2 *const *>(wb + half));
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

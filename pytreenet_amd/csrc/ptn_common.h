@@ -10,6 +10,11 @@ namespace ptn {
 
 void set_error(const char *fmt, ...);
 
+// Copy `bytes` of host data to device memory `dst`, ordered on stream `s`,
+// through a ring of pinned staging buffers (the host data may be freed when
+// the call returns).  Used for the per-instance descriptors of grouped launches.
+int stage_to_device(hipStream_t s, const void *src, size_t bytes, void *dst);
+
 #define PTN_HIP_CHECK(expr)                                                    \
   do {                                                                         \
     hipError_t _e = (expr);                                                    \

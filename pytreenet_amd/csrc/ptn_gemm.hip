@@ -27,6 +27,8 @@
 // sharing an A row panel share one L2.
 #include "ptn_common.h"
 
+#include <vector>
+
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -90,8 +92,9 @@ __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
 // instantiated separately for launches with a large tile grid (>= LARGE_TILES
 // tiles, no split-K), so profilers report the dominant H_eff / environment
 // GEMMs apart from the many small ones.
-template <int MODE, typename TA, typename TB, typename TC, int LARGE>
-__global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) {
+template <int MODE, typename TA, typename TB, typename TC>
+__device__ __forceinline__ void gather_tile(const KArgs &p, int64_t bid, int64_t split,
+                                            bool splitk) {
   typedef typename Acc<MODE>::T T;
   typedef typename Acc<MODE>::V V;
   __shared__ T sAr[2][BK][BM + PTN_GEMM_PAD];
@@ -104,15 +107,10 @@ __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) 
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  int64_t bid = blockIdx.x;
-  if (p.remap) {
-    const int64_t per = gridDim.x >> 3;
-    bid = (bid & 7) * per + (bid >> 3);
-  }
   const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
   const int64_t m0 = tm * BM, n0 = tn * BN;
   // split-K: this block covers k in [kbeg, kend)
-  const int64_t kbeg = (int64_t)blockIdx.y * p.kchunk;
+  const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
   const TA *A = static_cast<const TA *>(p.A);
   const TB *B = static_cast<const TB *>(p.B);
@@ -299,9 +297,9 @@ __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) 
           re = (double)(t1 - t2);
           im = (double)(t3 - t1 - t2);
         }
-        if (gridDim.y > 1) {
+        if (splitk) {
           // raw partial sum; the reduction kernel applies alpha, beta and the scatter
-          p.ws[((int64_t)blockIdx.y * p.M + gm) * p.N + gn] = make_double2(re, im);
+          p.ws[(split * p.M + gm) * p.N + gn] = make_double2(re, im);
           continue;
         }
         double ore = p.are * re - p.aim * im;
@@ -317,6 +315,21 @@ __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) 
       }
     }
   }
+}
+
+// tile index of a launch with the XCD-aware remap applied
+__device__ __forceinline__ int64_t remapped_tile(const KArgs &p) {
+  int64_t bid = blockIdx.x;
+  if (p.remap) {
+    const int64_t per = gridDim.x >> 3;
+    bid = (bid & 7) * per + (bid >> 3);
+  }
+  return bid;
+}
+
+template <int MODE, typename TA, typename TB, typename TC, int LARGE>
+__global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) {
+  gather_tile<MODE, TA, TB, TC>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
 }
 
 // ---- FP64 kernel with direct global->LDS tile loads ---------------------------
@@ -336,9 +349,9 @@ __device__ __forceinline__ void lds_dma16(const double2 *src, double2 *lds_row) 
                                    (__attribute__((address_space(3))) void *)lds_row, 16, 0, 0);
 }
 
-template <int LARGE>
-__global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
-    KArgs p, const int64_t *__restrict__ aok, const int64_t *__restrict__ bok) {
+__device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__restrict__ aok,
+                                            const int64_t *__restrict__ bok, int64_t bid,
+                                            int64_t split, bool splitk) {
   // one LDS array per ring stage, so every LDS read names the stage it reads
   // and the compiler need not wait for the DMA of the other stages
   __shared__ double2 sA0[DBK][BM], sA1[DBK][BM], sA2[DBK][BM];
@@ -347,14 +360,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
-  int64_t bid = blockIdx.x;
-  if (p.remap) {
-    const int64_t per = gridDim.x >> 3;
-    bid = (bid & 7) * per + (bid >> 3);
-  }
   const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
   const int64_t m0 = tm * BM, n0 = tn * BN;
-  const int64_t kbeg = (int64_t)blockIdx.y * p.kchunk;
+  const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
   const double2 *A = static_cast<const double2 *>(p.A);
   const double2 *B = static_cast<const double2 *>(p.B);
@@ -463,8 +471,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
         const int64_t cm = m0 + wm * 32 + i * 16 + r * 4 + fk;
         if (cm >= p.M) continue;
         const double re = c0[i][j][r], im = c1[i][j][r];
-        if (gridDim.y > 1) {
-          p.ws[((int64_t)blockIdx.y * p.M + cm) * p.N + cn] = make_double2(re, im);
+        if (splitk) {
+          p.ws[(split * p.M + cm) * p.N + cn] = make_double2(re, im);
           continue;
         }
         double ore = p.are * re - p.aim * im;
@@ -479,6 +487,48 @@ __global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
       }
     }
   }
+}
+
+template <int LARGE>
+__global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
+    KArgs p, const int64_t *__restrict__ aok, const int64_t *__restrict__ bok) {
+  direct_tile(p, aok, bok, remapped_tile(p), blockIdx.y, gridDim.y > 1);
+}
+
+// ---- grouped launches -----------------------------------------------------------
+// One launch runs the tiles of `count` independent contractions (one per batch
+// instance): workgroup b finds its descriptor d with prefix[d] <= b <
+// prefix[d+1] (binary search over scalar loads) and runs tile b - prefix[d] of
+// it.  No split-K and no XCD remap inside a grouped launch.
+__device__ __forceinline__ int find_desc(const int64_t *__restrict__ prefix, int count,
+                                         int64_t b) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(NT, 2) void gemm_direct_f64_grouped(const KArgs *__restrict__ descs,
+                                                                 const int64_t *__restrict__ prefix,
+                                                                 int count) {
+  const int64_t b = blockIdx.x;
+  const int d = find_desc(prefix, count, b);
+  const KArgs p = descs[d];
+  direct_tile(p, p.aok, p.bok, b - prefix[d], 0, false);
+}
+
+template <int MODE, typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_grouped(
+    const KArgs *__restrict__ descs, const int64_t *__restrict__ prefix, int count) {
+  const int64_t b = blockIdx.x;
+  const int d = find_desc(prefix, count, b);
+  const KArgs p = descs[d];
+  gather_tile<MODE, TA, TB, TC>(p, b - prefix[d], 0, false);
 }
 
 // sum the split-K partials in split order, then alpha / beta / scatter store
@@ -550,7 +600,10 @@ int dispatch_b(hipStream_t s, const KArgs &a, int64_t tiles, int bdt, int cdt, i
 
 }  // namespace
 
-extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
+namespace {
+// validate a descriptor and fill the kernel arguments (no split-K, no remap);
+// returns PTN_OK, or 1 for an empty product (M or N zero), or an error code
+int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
   if (!d) {
     ptn::set_error("ptn_gemm_gather: null descriptor");
     return PTN_EINVAL;
@@ -559,7 +612,7 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
     ptn::set_error("ptn_gemm_gather: negative extent");
     return PTN_EINVAL;
   }
-  if (d->M == 0 || d->N == 0) return PTN_OK;
+  if (d->M == 0 || d->N == 0) return 1;
   const bool dt_ok = (unsigned)d->a_dtype <= 1 && (unsigned)d->b_dtype <= 1 &&
                      (unsigned)d->c_dtype <= 1;
   if (!dt_ok || (d->compute != PTN_COMPUTE_F64_4M && d->compute != PTN_COMPUTE_F32_3M)) {
@@ -576,7 +629,6 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
     ptn::set_error("ptn_gemm_gather: null pointer");
     return PTN_EINVAL;
   }
-  KArgs a;
   a.M = d->M;
   a.N = d->N;
   a.K = d->K;
@@ -598,11 +650,24 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   a.beta_one = d->beta_one;
   const int64_t tiles_m = (d->M + BM - 1) / BM;
   a.tiles_n = (d->N + BN - 1) / BN;
-  const int64_t tiles = tiles_m * a.tiles_n;
+  tiles = tiles_m * a.tiles_n;
   if (tiles > 0x7fffffffLL) {
     ptn::set_error("ptn_gemm_gather: too many tiles");
     return PTN_ERANGE;
   }
+  a.remap = 0;
+  a.ws = nullptr;
+  a.kchunk = d->K > 0 ? ((d->K + BK - 1) / BK) * BK : BK;
+  return PTN_OK;
+}
+}  // namespace
+
+extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
+  KArgs a;
+  int64_t tiles = 0;
+  const int rc0 = to_kargs(d, a, tiles);
+  if (rc0 == 1) return PTN_OK;
+  if (rc0) return rc0;
   a.remap = (tiles % 8 == 0) ? 1 : 0;
   // split-K when the tile grid cannot fill the chip and K is long: each split
   // keeps >= SPLITK_MIN_K of K, partials go to the caller's workspace
@@ -628,4 +693,72 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   if (d->a_dtype == PTN_C128)
     return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
+}
+
+extern "C" int64_t ptn_gemm_grouped_work_bytes(int32_t count) {
+  if (count < 1) return 0;
+  return ((((int64_t)count * sizeof(KArgs) + 255) / 256) * 256) + ((int64_t)count + 1) * 8;
+}
+
+extern "C" int ptn_gemm_gather_grouped(void *stream, const ptn_gemm_desc *descs, int32_t count,
+                                       void *work) {
+  if (count < 1 || count > (1 << 20) || !descs || !work) {
+    ptn::set_error("ptn_gemm_gather_grouped: bad arguments (count=%d)", count);
+    return PTN_EINVAL;
+  }
+  std::vector<KArgs> ka;
+  std::vector<int64_t> prefix(1, 0);
+  ka.reserve(count);
+  const ptn_gemm_desc &d0 = descs[0];
+  for (int i = 0; i < count; ++i) {
+    const ptn_gemm_desc &d = descs[i];
+    if (d.compute != d0.compute || d.a_dtype != d0.a_dtype || d.b_dtype != d0.b_dtype ||
+        d.c_dtype != d0.c_dtype) {
+      ptn::set_error("ptn_gemm_gather_grouped: descriptor %d differs in compute mode or dtype", i);
+      return PTN_EINVAL;
+    }
+    KArgs a;
+    int64_t tiles = 0;
+    const int rc = to_kargs(&d, a, tiles);
+    if (rc == 1) continue;
+    if (rc) return rc;
+    ka.push_back(a);
+    prefix.push_back(prefix.back() + tiles);
+  }
+  const int n = (int)ka.size();
+  if (n == 0) return PTN_OK;
+  const int64_t total = prefix.back();
+  if (total > 0x7fffffffLL) {
+    ptn::set_error("ptn_gemm_gather_grouped: too many tiles");
+    return PTN_ERANGE;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char *w = static_cast<char *>(work);
+  const int64_t off = (((int64_t)count * sizeof(KArgs) + 255) / 256) * 256;
+  int rc = ptn::stage_to_device(s, ka.data(), ka.size() * sizeof(KArgs), w);
+  if (rc) return rc;
+  if ((rc = ptn::stage_to_device(s, prefix.data(), prefix.size() * sizeof(int64_t), w + off)))
+    return rc;
+  const KArgs *dk = reinterpret_cast<const KArgs *>(w);
+  const int64_t *dp = reinterpret_cast<const int64_t *>(w + off);
+  const dim3 grid((unsigned)total);
+  if (d0.compute == PTN_COMPUTE_F64_4M) {
+    hipLaunchKernelGGL(gemm_direct_f64_grouped, grid, dim3(NT), 0, s, dk, dp, n);
+  } else {
+    const bool a64 = d0.a_dtype == PTN_C128, b64 = d0.b_dtype == PTN_C128,
+               c64 = d0.c_dtype == PTN_C128;
+#define PTN_GROUPED(TA, TB, TC) \
+  hipLaunchKernelGGL((gemm_gather_grouped<PTN_COMPUTE_F32_3M, TA, TB, TC>), grid, dim3(NT), 0, s, dk, dp, n)
+    if (a64 && b64 && c64) PTN_GROUPED(double2, double2, double2);
+    else if (a64 && b64) PTN_GROUPED(double2, double2, float2);
+    else if (a64 && c64) PTN_GROUPED(double2, float2, double2);
+    else if (a64) PTN_GROUPED(double2, float2, float2);
+    else if (b64 && c64) PTN_GROUPED(float2, double2, double2);
+    else if (b64) PTN_GROUPED(float2, double2, float2);
+    else if (c64) PTN_GROUPED(float2, float2, double2);
+    else PTN_GROUPED(float2, float2, float2);
+#undef PTN_GROUPED
+  }
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
 }

@@ -307,8 +307,11 @@ BLAS_MAX_K_PER_TILE = 1 << 14
 USE_BLAS = True
 
 
+_BLAS_OFF = 0   # > 0 while recording networks that are replayed as grouped launches
+
+
 def _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine):
-    if not USE_BLAS or K < 1 or min(M, N) < BLAS_MIN_DIM:
+    if not USE_BLAS or _BLAS_OFF or K < 1 or min(M, N) < BLAS_MIN_DIM:
         return None
     tiles = -(-M // 64) * -(-N // 64)
     if tiles < BLAS_MIN_TILES or K > BLAS_MAX_K_PER_TILE:
@@ -981,6 +984,7 @@ class CompiledNetwork:
         self.out_shape = tuple(out.shape)
         self.sig = sig
         self.ops = rec.ops
+        self.rec_inputs, self.rec_out = ts, out
 
         def role(t, off=0):
             for k, v in enumerate(var):
@@ -1006,3 +1010,172 @@ class CompiledNetwork:
             else:
                 self.op_roles.append((role(op[1]),))
         return out
+
+
+# --------------------------------------------------------------------------------------
+# batched replay: one grouped launch per op over many same-shape instances
+# --------------------------------------------------------------------------------------
+_GEMM_PTR_FIELDS = {"A": GemmDesc.A.offset, "B": GemmDesc.B.offset, "C": GemmDesc.C.offset}
+
+
+class _Program:
+    """The ops of one recorded network with every device pointer expressed as
+    (role, byte offset): role ("in", i) is input i of the network, ("out", 0)
+    its output, ("tmp", k) the k-th intermediate buffer.  Instances whose
+    inputs have the template's shapes, strides and dtypes run the same ops with
+    their own pointers; the offset tables are shared."""
+
+    def __init__(self, net: "CompiledNetwork"):
+        inputs, out = net.rec_inputs, net.rec_out
+        ranges = []
+        for i, t in enumerate(inputs):
+            if isinstance(t, BlockSparse):
+                raise ValueError("batched replay needs dense operator tensors")
+            nb = t.numel() * t.element_size()
+            if nb:
+                ranges.append((t.data_ptr(), t.data_ptr() + nb, "in", i))
+        ranges.append((out.data_ptr(), out.data_ptr() + out.numel() * out.element_size(),
+                       "out", 0))
+        temps: Dict[int, int] = {}
+        self.temp_bytes: List[int] = []
+
+        def classify(t, ptr):
+            for a, b, kind, idx in ranges:
+                if a <= ptr < b:
+                    return kind, idx, ptr - a
+            st = t.untyped_storage()
+            base = st.data_ptr()
+            k = temps.get(base)
+            if k is None:
+                k = temps[base] = len(self.temp_bytes)
+                self.temp_bytes.append(-(-st.nbytes() // 256) * 256)
+            return "tmp", k, ptr - base
+
+        self.ops = []
+        for op in net.ops:
+            if op[0] == "gemm":
+                d = op[1]
+                fields = []
+                for name, t in (("A", op[2]), ("B", op[3]), ("C", op[4])):
+                    fields.append((_GEMM_PTR_FIELDS[name] // 8,
+                                   classify(t, getattr(d, name))))
+                tmpl = np.frombuffer(bytes(d), dtype=np.uint8).copy()
+                self.ops.append(("gemm", tmpl, fields))
+            elif op[0] == "fill":
+                t = op[1]
+                self.ops.append(("fill", classify(t, t.data_ptr()), t.numel(),
+                                 _DTYPE_CODE[t.dtype], op[2]))
+            else:
+                raise ValueError(f"batched replay does not support {op[0]!r} ops")
+        self.out_shape = tuple(out.shape)
+        self.out_dtype = out.dtype
+        self.n_in = len(inputs)
+        self._work: Dict[tuple, torch.Tensor] = {}
+
+    def _workspace(self, kind: str, count: int, dev) -> torch.Tensor:
+        key = (kind, count, dev)
+        w = self._work.get(key)
+        if w is None:
+            lib = _lib.load()
+            nb = (lib.ptn_gemm_grouped_work_bytes(count) if kind == "gemm"
+                  else lib.ptn_ptr_batch_work_bytes(count))
+            w = self._work[key] = torch.empty(int(nb), dtype=torch.uint8, device=dev)
+        return w
+
+    def run(self, in_ptrs: np.ndarray, dev) -> torch.Tensor:
+        """in_ptrs: (count, n_in) int64 data pointers of each instance's inputs.
+        Returns the outputs stacked as (count, *out_shape)."""
+        count = in_ptrs.shape[0]
+        outs = torch.empty((count,) + self.out_shape, dtype=self.out_dtype, device=dev)
+        ostride = outs.stride(0) * outs.element_size()
+        out_ptr = outs.data_ptr() + ostride * np.arange(count, dtype=np.int64)
+        tmp_ptr = []
+        keep = [outs]
+        for nb in self.temp_bytes:
+            buf = torch.empty(nb * count, dtype=torch.uint8, device=dev)
+            keep.append(buf)
+            tmp_ptr.append(buf.data_ptr() + nb * np.arange(count, dtype=np.int64))
+
+        def base(role):
+            kind, idx, off = role
+            if kind == "in":
+                return in_ptrs[:, idx] + off
+            if kind == "out":
+                return out_ptr + off
+            return tmp_ptr[idx] + off
+
+        lib = _lib.load()
+        stream = ctypes.c_void_p(stream_handle())
+        for op in self.ops:
+            if op[0] == "gemm":
+                tmpl, fields = op[1], op[2]
+                buf = np.tile(tmpl, (count, 1))
+                b64 = buf.view(np.int64)
+                for col, role in fields:
+                    b64[:, col] = base(role)
+                w = self._workspace("gemm", count, dev)
+                _lib.check(lib.ptn_gemm_gather_grouped(
+                    stream, buf.ctypes.data_as(ctypes.POINTER(GemmDesc)), count,
+                    ctypes.c_void_p(w.data_ptr())), "ptn_gemm_gather_grouped")
+            else:
+                _, role, numel, code, v = op
+                ptrs = np.ascontiguousarray(base(role), dtype=np.int64)
+                w = self._workspace("ptr", count, dev)
+                _lib.check(lib.ptn_fill_batched(stream, count, ptrs.ctypes.data_as(ctypes.c_void_p),
+                                                numel, code, v.real, v.imag,
+                                                ctypes.c_void_p(w.data_ptr())),
+                           "ptn_fill_batched")
+        return outs
+
+
+class BatchedNetwork:
+    """Many instances of one network contraction (one CompiledNetwork each, same
+    labels) applied together.  Instances are grouped by the shapes, strides and
+    dtypes of their inputs; the first call of a group records its first member
+    (run on its own) as the template, every other member -- and every member on
+    later calls -- runs as one grouped launch per op."""
+
+    def __init__(self, nets: Sequence["CompiledNetwork"]):
+        self.nets = list(nets)
+        self.groups: Dict[tuple, List[int]] = {}
+        for i, net in enumerate(self.nets):
+            sig = (tuple((tuple(t.shape), tuple(t.stride()) if not isinstance(t, BlockSparse)
+                          else None, t.dtype) for t in net.tensors[1:]),
+                   net.precision, net.out_dtype)
+            self.groups.setdefault(sig, []).append(i)
+        self.programs: Dict[tuple, _Program] = {}
+        self._fixed: Dict[tuple, np.ndarray] = {}
+
+    def __call__(self, idxs: Sequence[int], xs: Sequence[torch.Tensor]) -> List[torch.Tensor]:
+        global _BLAS_OFF
+        pos = {i: k for k, i in enumerate(idxs)}
+        res: List[Optional[torch.Tensor]] = [None] * len(idxs)
+        for sig, members in self.groups.items():
+            act = [i for i in members if i in pos]
+            if not act:
+                continue
+            prog = self.programs.get(sig)
+            if prog is None:
+                first = act[0]
+                _BLAS_OFF += 1
+                try:
+                    res[pos[first]] = self.nets[first](xs[pos[first]])
+                finally:
+                    _BLAS_OFF -= 1
+                prog = self.programs[sig] = _Program(self.nets[first])
+                act = act[1:]
+                if not act:
+                    continue
+            dev = xs[pos[act[0]]].device
+            key = (sig, tuple(act))
+            fixed = self._fixed.get(key)
+            if fixed is None:
+                fixed = np.array([[0] + [t.data_ptr() for t in self.nets[i].tensors[1:]]
+                                  for i in act], dtype=np.int64)
+                self._fixed[key] = fixed
+            ptrs = fixed.copy()
+            ptrs[:, 0] = [xs[pos[i]].data_ptr() for i in act]
+            outs = prog.run(ptrs, dev)
+            for k, i in enumerate(act):
+                res[pos[i]] = outs[k]
+        return res

@@ -274,3 +274,148 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     combine(Q, coeffs, out, nvec=m)
     STATS.record(m)
     return out.view(shape)
+
+
+# ---------------------------------------------------------------------------------------
+# batched form
+# ---------------------------------------------------------------------------------------
+_BWORK: Dict[tuple, torch.Tensor] = {}
+
+
+def _bwork(dev, count: int) -> torch.Tensor:
+    key = (dev, count)
+    w = _BWORK.get(key)
+    if w is None:
+        nb = int(_lib.load().ptn_krylov_batched_work_bytes(count))
+        w = _BWORK[key] = torch.empty(nb, dtype=torch.uint8, device=dev)
+    return w
+
+
+def _ptrs(ts) -> np.ndarray:
+    return np.array([t.data_ptr() for t in ts], dtype=np.int64)
+
+
+def _cp(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def krylov_exp_action_batched(psis, apply: Callable, time_difference: float,
+                              exp_factor: complex, krylov_tol: float = 1e-10,
+                              max_krylov_dim: Optional[int] = None, hermitian: bool = True,
+                              stats: Optional[list] = None):
+    """``krylov_exp_action`` for many independent instances at once.
+
+    ``apply(idxs, xs)`` returns H_b x for the active instances ``idxs``.  Each
+    instance follows the single-instance recurrence and stop rules exactly
+    (its own beta0, Hessenberg matrix, breakdown and Saad test); the vector
+    work of all instances with the same length n runs as one batched launch
+    per step, and one device->host copy per Krylov step carries the scalars of
+    every active instance.
+    """
+    count = len(psis)
+    dev = psis[0].device
+    shapes = [p.shape for p in psis]
+    v0 = [_flat(p.to(torch.complex128)) for p in psis]
+    ns = [v.numel() for v in v0]
+    steps = [n if max_krylov_dim is None else min(n, int(max_krylov_dim)) for n in ns]
+    mmax = max(steps)
+    lib = _lib.load()
+    stream = ctypes.c_void_p(E.stream_handle())
+    by_n: Dict[int, List[int]] = {}
+    for b, n in enumerate(ns):
+        by_n.setdefault(n, []).append(b)
+    Q = [torch.empty((steps[b] + 1, ns[b]), dtype=torch.complex128, device=dev)
+         for b in range(count)]
+    packed = torch.empty((count, mmax + 3), dtype=torch.complex128, device=dev)
+    pk = packed.data_ptr()
+    prow = packed.stride(0) * 16
+    for n, mem in by_n.items():
+        work = _bwork(dev, len(mem))
+        xs = _ptrs([v0[b] for b in mem])
+        ys = _ptrs([Q[b][0] for b in mem])
+        outs = np.array([pk + prow * b for b in mem], dtype=np.int64)
+        _lib.check(lib.ptn_normalize_batched(stream, n, len(mem), _cp(xs), _cp(ys), _cp(outs),
+                                             ctypes.c_void_p(work.data_ptr())),
+                   "ptn_normalize_batched")
+    host_buf = torch.empty((count, mmax + 3), dtype=torch.complex128, pin_memory=True)
+    done_ev = torch.cuda.Event()
+    hess = [np.zeros((steps[b] + 1, steps[b] + 1), dtype=complex) for b in range(count)]
+    beta0 = [0.0] * count
+    m = [0] * count
+    active = list(range(count))
+    zero = [False] * count
+    for j in range(mmax):
+        active = [b for b in active if j < steps[b]]
+        if not active:
+            break
+        ws = apply(active, [Q[b][j].view(shapes[b]) for b in active])
+        ws = [_flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
+              for w in ws]
+        wmap = dict(zip(active, ws))
+        groups: Dict[int, List[int]] = {}
+        for b in active:
+            groups.setdefault(ns[b], []).append(b)
+        for n, mem in groups.items():
+            work = _bwork(dev, len(mem))
+            qs = _ptrs([Q[b] for b in mem])
+            ld = np.array([Q[b].stride(0) for b in mem], dtype=np.int64)
+            wp = _ptrs([wmap[b] for b in mem])
+            qn = _ptrs([Q[b][j + 1] for b in mem])
+            outs = np.array([pk + prow * b + 16 for b in mem], dtype=np.int64)
+            _lib.check(lib.ptn_krylov_orthonormalize_batched(
+                stream, n, j + 1, len(mem), _cp(qs), _cp(ld), _cp(wp), _cp(qn), _cp(outs),
+                ctypes.c_void_p(work.data_ptr())), "ptn_krylov_orthonormalize_batched")
+        host_buf[:, : j + 3].copy_(packed[:, : j + 3], non_blocking=True)
+        done_ev.record()
+        done_ev.synchronize()
+        host = host_buf[:, : j + 3].numpy()
+        still = []
+        for b in active:
+            h = host[b]
+            if j == 0:
+                beta0[b] = math.sqrt(max(h[0].real, 0.0))
+                if beta0[b] == 0.0:
+                    zero[b] = True
+                    continue
+            hess[b][: j + 1, j] = h[1: j + 2]
+            w0_norm = math.sqrt(max(h[j + 2].real, 0.0))
+            beta = math.sqrt(max(h[j + 2].imag, 0.0))
+            hess[b][j + 1, j] = beta
+            m[b] = j + 1
+            if beta <= max(np.finfo(float).eps * max(ns[b], 1), 1e-12 * w0_norm):
+                continue
+            if krylov_tol > 0.0:
+                col = small_propagator(hess[b][: m[b], : m[b]], time_difference, exp_factor,
+                                       hermitian)
+                if beta0[b] * beta * abs(col[m[b] - 1]) < krylov_tol:
+                    continue
+            still.append(b)
+        active = still
+    results: List[Optional[torch.Tensor]] = [None] * count
+    for b in range(count):
+        if zero[b]:
+            results[b] = psis[b].clone()
+    live = [b for b in range(count) if not zero[b]]
+    by_nm: Dict[tuple, List[int]] = {}
+    for b in live:
+        by_nm.setdefault((ns[b], m[b]), []).append(b)
+    for (n, mb), mem in by_nm.items():
+        coeffs = np.stack([beta0[b] * small_propagator(hess[b][:mb, :mb], time_difference,
+                                                       exp_factor, hermitian)
+                           for b in mem]).astype(np.complex128)
+        outs = [torch.empty(n, dtype=torch.complex128, device=dev) for _ in mem]
+        work = _bwork(dev, len(mem))
+        qs = _ptrs([Q[b] for b in mem])
+        ld = np.array([Q[b].stride(0) for b in mem], dtype=np.int64)
+        op = _ptrs(outs)
+        _lib.check(lib.ptn_krylov_combine_batched(stream, n, mb, len(mem), _cp(qs), _cp(ld),
+                                                  ctypes.c_void_p(coeffs.ctypes.data), _cp(op),
+                                                  ctypes.c_void_p(work.data_ptr())),
+                   "ptn_krylov_combine_batched")
+        for b, o in zip(mem, outs):
+            results[b] = o.view(shapes[b])
+    for b in range(count):
+        STATS.record(m[b])
+        if stats is not None:
+            stats.append(m[b])
+    return results

@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
-from typing import Any, Dict, List, Optional, Union
+from typing import Any, Dict, List, Optional, Sequence, Union
 
 import numpy as np
 import torch
@@ -258,6 +258,16 @@ def find_orthogonalisation_path(update_path: List[str], ttn,
 # ---------------------------------------------------------------------------------------
 # TDVP
 # ---------------------------------------------------------------------------------------
+class KrylovRequest:
+    """One pending exp(f dt H) x of a sweep: x, the H action, dt, direction."""
+    __slots__ = ("x", "action", "dt", "direction", "result", "applies")
+
+    def __init__(self, x, action, dt, direction):
+        self.x, self.action, self.dt, self.direction = x, action, dt, direction
+        self.result = None
+        self.applies = 0
+
+
 class TDVPAlgorithm(TTNOBasedTimeEvolution):
     config_class = TDVPConfig
 
@@ -278,6 +288,9 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         if self.precision not in (E.FP64, E.FP32):
             raise ValueError(f"unknown precision {self.precision!r}")
         self.apply_log: List[tuple] = []   # (kind, node or edge, Krylov applies)
+        self._last_applies = 0
+        # dense TTNO tensors in the site networks (the batched replay needs them)
+        self.dense_site_operator = False
         self.update_path = self._finds_update_path()
         self.orthogonalization_path = find_orthogonalisation_path(self.update_path, self.state)
         self._orthogonalize_init()
@@ -305,10 +318,50 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         return self._mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
                                              **opts)
 
+    def krylov_options(self) -> Dict[str, Any]:
+        opts = dict(TimeEvoMode.default_solver_options(self._mode))
+        opts.update(self.solver_options)
+        return opts
+
+    # The sweep steps are generators: every Krylov exponential is handed out as a
+    # KrylovRequest (``y = yield from self._evolve(...)``).  ``_drive`` answers
+    # the requests one by one; a BatchedTDVP answers those of many instances at
+    # once.
+    def _evolve(self, x: torch.Tensor, action, dt: float, direction: EvoDirection):
+        req = KrylovRequest(x, action, dt, direction)
+        yield req
+        self._last_applies = req.applies
+        return req.result
+
+    def _drive(self, gen):
+        from .krylov import STATS
+        try:
+            req = next(gen)
+            while True:
+                a0 = STATS.applies
+                req.result = self._krylov(req.x, req.action, req.dt, req.direction)
+                req.applies = STATS.applies - a0
+                req = gen.send(None)
+        except StopIteration as stop:
+            return stop.value
+
+    def _time_step_gen(self):
+        """One time step as a generator of KrylovRequests."""
+        raise NotImplementedError
+        yield
+
+    def run_one_time_step(self, **kwargs):
+        E.refresh_stream()
+        self._drive(self._time_step_gen())
+
     def _update_site(self, node_id: str, time_step_factor: float = 1):
+        return self._drive(self._update_site_gen(node_id, time_step_factor))
+
+    def _update_site_gen(self, node_id: str, time_step_factor: float = 1):
         node, tensor = self.state[node_id]
         op_node = self.hamiltonian.nodes[node_id]
-        op = operator_tensor(self.hamiltonian, node_id, self.precision)
+        op = operator_tensor(self.hamiltonian, node_id, self.precision,
+                             block=False if self.dense_site_operator else None)
         cache = self.partial_tree_cache
         envs = [cache.get_entry(nb, node_id) for nb in node.neighbouring_nodes()]
         tensors, labels, out_l = heff_network(node, op_node, op, envs)
@@ -316,12 +369,9 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         # recorded on the first Krylov apply, replayed for the others
         action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision)
 
-        from .krylov import STATS
-        n0 = STATS.applies
-        self.state.tensors[node_id] = self._krylov(tensor, action,
-                                                   self.time_step_size * time_step_factor,
-                                                   EvoDirection.FORWARD)
-        self.apply_log.append(("site", node_id, STATS.applies - n0))
+        self.state.tensors[node_id] = yield from self._evolve(
+            tensor, action, self.time_step_size * time_step_factor, EvoDirection.FORWARD)
+        self.apply_log.append(("site", node_id, self._last_applies))
 
     def _move_orth_and_update_cache_for_path(self, path: List[str]):
         if len(path) == 0:
@@ -336,6 +386,9 @@ class OneSiteTDVP(TDVPAlgorithm):
     """Site + link updates; the link is the R factor of a QR(KEEP) split."""
 
     def _update_link(self, node_id: str, next_node_id: str, time_step_factor: float = 1):
+        return self._drive(self._update_link_gen(node_id, next_node_id, time_step_factor))
+
+    def _update_link_gen(self, node_id: str, next_node_id: str, time_step_factor: float = 1):
         assert self.state.orthogonality_center_id == node_id
         from .splitting import qr_split_to_neighbour, absorb_into_neighbour
         q, r = qr_split_to_neighbour(self.state, node_id, next_node_id)
@@ -343,14 +396,12 @@ class OneSiteTDVP(TDVPAlgorithm):
         self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
         env_a = self.partial_tree_cache.get_entry(node_id, next_node_id)
         env_b = self.partial_tree_cache.get_entry(next_node_id, node_id)
-        from .krylov import STATS
-        n0 = STATS.applies
         bond = E.CompiledNetwork([r, env_a, env_b],
                                  [["a0", "a1"], ["a0", "w", "b0"], ["a1", "w", "b1"]],
                                  ["b0", "b1"], precision=self.precision)
-        r = self._krylov(r, bond, self.time_step_size * time_step_factor,
-                         EvoDirection.BACKWARD)
-        self.apply_log.append(("bond", (node_id, next_node_id), STATS.applies - n0))
+        r = yield from self._evolve(r, bond, self.time_step_size * time_step_factor,
+                                    EvoDirection.BACKWARD)
+        self.apply_log.append(("bond", (node_id, next_node_id), self._last_applies))
         absorb_into_neighbour(self.state, next_node_id, node_id, r)
         self.state.orthogonality_center_id = next_node_id
 
@@ -361,27 +412,29 @@ class OneSiteTDVP(TDVPAlgorithm):
 
 class FirstOrderOneSiteTDVP(OneSiteTDVP):
     def _update_site_and_link(self, node_id: str, update_index: int):
-        self._update_site(node_id)
-        self._update_link(node_id, self.orthogonalization_path[update_index][0])
+        return self._drive(self._update_site_and_link_gen(node_id, update_index))
+
+    def _update_site_and_link_gen(self, node_id: str, update_index: int):
+        yield from self._update_site_gen(node_id)
+        yield from self._update_link_gen(node_id, self.orthogonalization_path[update_index][0])
 
     def _reset_for_next_time_step(self):
         self.state.move_orthogonalization_center(self.update_path[0], mode=SplitMode.KEEP)
         self.partial_tree_cache = self._init_partial_tree_cache()
 
-    def run_one_time_step(self, **kwargs):
-        E.refresh_stream()
+    def _time_step_gen(self):
         up, op = self.update_path, self.orthogonalization_path
         for i, node_id in enumerate(up):
             if i == len(up) - 1:
                 if len(op) > 0:
                     self._move_orth_and_update_cache_for_path(op[-1])
-                self._update_site(node_id)
+                yield from self._update_site_gen(node_id)
                 self._reset_for_next_time_step()
             elif i == 0:
-                self._update_site_and_link(node_id, 0)
+                yield from self._update_site_and_link_gen(node_id, 0)
             else:
                 self._move_orth_and_update_cache_for_path(op[i - 1])
-                self._update_site_and_link(node_id, i)
+                yield from self._update_site_and_link_gen(node_id, i)
 
 
 class SecondOrderOneSiteTDVP(OneSiteTDVP):
@@ -399,35 +452,44 @@ class SecondOrderOneSiteTDVP(OneSiteTDVP):
         return out
 
     def forward_sweep(self):
+        return self._drive(self._forward_sweep_gen())
+
+    def _forward_sweep_gen(self):
         for i, node_id in enumerate(self.update_path[:-1]):
             if i > 0:
                 self._move_orth_and_update_cache_for_path(self.orthogonalization_path[i - 1])
             nxt = self.orthogonalization_path[i][0]
-            self._update_site(node_id, time_step_factor=0.5)
-            self._update_link(node_id, nxt, time_step_factor=0.5)
+            yield from self._update_site_gen(node_id, time_step_factor=0.5)
+            yield from self._update_link_gen(node_id, nxt, time_step_factor=0.5)
 
     def _final_forward_update(self):
+        return self._drive(self._final_forward_update_gen())
+
+    def _final_forward_update_gen(self):
         node_id = self.update_path[-1]
         assert self.state.orthogonality_center_id == node_id
-        self._update_site(node_id)
+        yield from self._update_site_gen(node_id)
 
     def backward_sweep(self):
+        return self._drive(self._backward_sweep_gen())
+
+    def _backward_sweep_gen(self):
         bp = self.backwards_update_path
-        self._update_link(self.state.orthogonality_center_id, bp[1], time_step_factor=0.5)
+        yield from self._update_link_gen(self.state.orthogonality_center_id, bp[1],
+                                         time_step_factor=0.5)
         for i, node_id in enumerate(bp[1:-1]):
             idx = i + 1
             assert self.state.orthogonality_center_id == node_id
-            self._update_site(node_id, time_step_factor=0.5)
+            yield from self._update_site_gen(node_id, time_step_factor=0.5)
             self._move_orth_and_update_cache_for_path(self.backwards_orth_path[idx - 1])
-            self._update_link(self.state.orthogonality_center_id, bp[idx + 1],
-                              time_step_factor=0.5)
-        self._update_site(bp[-1], time_step_factor=0.5)
+            yield from self._update_link_gen(self.state.orthogonality_center_id, bp[idx + 1],
+                                             time_step_factor=0.5)
+        yield from self._update_site_gen(bp[-1], time_step_factor=0.5)
 
-    def run_one_time_step(self, **kwargs):
-        E.refresh_stream()
-        self.forward_sweep()
-        self._final_forward_update()
-        self.backward_sweep()
+    def _time_step_gen(self):
+        yield from self._forward_sweep_gen()
+        yield from self._final_forward_update_gen()
+        yield from self._backward_sweep_gen()
 
 
 class TwoSiteTDVP(TDVPAlgorithm):
@@ -472,6 +534,11 @@ class TwoSiteTDVP(TDVPAlgorithm):
 
     def _update_two_site_nodes(self, target_node_id: str, next_node_id: str,
                                time_step_factor: float = 1):
+        return self._drive(self._update_two_site_nodes_gen(target_node_id, next_node_id,
+                                                           time_step_factor))
+
+    def _update_two_site_nodes_gen(self, target_node_id: str, next_node_id: str,
+                                   time_step_factor: float = 1):
         s = self.state
         n1, n2 = target_node_id, next_node_id
         assert s.orthogonality_center_id == n1
@@ -488,11 +555,9 @@ class TwoSiteTDVP(TDVPAlgorithm):
         outer2 = [x for x in l2[:-1] if x != "bond"]
         x = E.contract(A1, l1, A2, l2, outer1 + outer2 + ["p1", "p2"])
         make, _, _ = self._two_site_action(n1, n2)
-        from .krylov import STATS
-        a0 = STATS.applies
-        x = self._krylov(x, make(x), self.time_step_size * time_step_factor,
-                         EvoDirection.FORWARD)
-        self.apply_log.append(("two_site", (n1, n2), STATS.applies - a0))
+        x = yield from self._evolve(x, make(x), self.time_step_size * time_step_factor,
+                                    EvoDirection.FORWARD)
+        self.apply_log.append(("two_site", (n1, n2), self._last_applies))
         # truncated SVD, S absorbed into V (VCONTR)
         dims = list(x.shape)
         udims = dims[:k1] + [dims[k1 + k2]]
@@ -527,7 +592,10 @@ class TwoSiteTDVP(TDVPAlgorithm):
         self.update_tree_cache(n1, n2)
 
     def _single_site_backwards_update(self, node_id: str, time_step_factor: float = 1):
-        self._update_site(node_id, -1 * time_step_factor)
+        return self._drive(self._single_site_backwards_update_gen(node_id, time_step_factor))
+
+    def _single_site_backwards_update_gen(self, node_id: str, time_step_factor: float = 1):
+        yield from self._update_site_gen(node_id, -1 * time_step_factor)
 
     @staticmethod
     def create_two_site_id(node_id: str, next_node_id: str) -> str:
@@ -542,28 +610,114 @@ class SecondOrderTwoSiteTDVP(TwoSiteTDVP):
                                     for p in reversed(self.orthogonalization_path)]
 
     def forward_sweep(self):
+        return self._drive(self._forward_sweep_gen())
+
+    def _forward_sweep_gen(self):
         up, op = self.update_path, self.orthogonalization_path
         for i, node_id in enumerate(up[:-2]):
             if i > 0:
                 self._move_orth_and_update_cache_for_path(op[i - 1])
             nxt = op[i][0]
-            self._update_two_site_nodes(node_id, nxt, time_step_factor=0.5)
-            self._single_site_backwards_update(nxt, time_step_factor=0.5)
-        self._update_two_site_nodes(up[-2], up[-1], time_step_factor=0.5)
+            yield from self._update_two_site_nodes_gen(node_id, nxt, time_step_factor=0.5)
+            yield from self._single_site_backwards_update_gen(nxt, time_step_factor=0.5)
+        yield from self._update_two_site_nodes_gen(up[-2], up[-1], time_step_factor=0.5)
 
     def backwards_sweep(self):
+        return self._drive(self._backwards_sweep_gen())
+
+    def _backwards_sweep_gen(self):
         bp, bo = self.backwards_update_path, self.backwards_orth_path
         for i in range(len(bp) - 1):
             if i == 0:
-                self._update_two_site_nodes(bp[0], bp[1], time_step_factor=0.5)
+                yield from self._update_two_site_nodes_gen(bp[0], bp[1], time_step_factor=0.5)
             else:
                 path = bo[i]
                 self._move_orth_and_update_cache_for_path(path)
                 target = path[-1]
-                self._single_site_backwards_update(target, time_step_factor=0.5)
-                self._update_two_site_nodes(target, bp[i + 1], time_step_factor=0.5)
+                yield from self._single_site_backwards_update_gen(target, time_step_factor=0.5)
+                yield from self._update_two_site_nodes_gen(target, bp[i + 1],
+                                                           time_step_factor=0.5)
 
-    def run_one_time_step(self, **kwargs):
+    def _time_step_gen(self):
+        yield from self._forward_sweep_gen()
+        yield from self._backwards_sweep_gen()
+
+
+# ---------------------------------------------------------------------------------------
+# batched TDVP: independent instances stepped together
+# ---------------------------------------------------------------------------------------
+class BatchedTDVP:
+    """Many TDVP runs on the same tree (disorder realisations, parameter sweeps)
+    advanced in lockstep.
+
+    Every instance is an ordinary TDVP algorithm object (its own state, TTNO,
+    environment cache and truncation decisions, so bond dimensions may differ
+    between instances).  All instances share the tree, hence the update and
+    orthogonalisation paths, the time step and the solver options: their sweeps
+    reach the same Krylov exponentials in the same order.  Each such exponential
+    runs for all instances at once (``krylov_exp_action_batched``): the H_eff
+    applies as grouped launches over instances with equal shapes
+    (``engine.BatchedNetwork``), the Arnoldi vector work as batched launches, one
+    host round trip per Krylov step.  Splits, truncation and environment updates
+    run per instance between those points.  Each instance follows exactly the
+    single-instance algorithm.
+    """
+
+    def __init__(self, algorithms: Sequence[TDVPAlgorithm]):
+        algs = list(algorithms)
+        if not algs:
+            raise ValueError("BatchedTDVP needs at least one instance")
+        a0 = algs[0]
+        for a in algs[1:]:
+            if type(a) is not type(a0) or a.update_path != a0.update_path or \
+                    a.time_step_size != a0.time_step_size or a.precision != a0.precision or \
+                    a.krylov_options() != a0.krylov_options():
+                raise ValueError("batched instances must share the algorithm, tree, time "
+                                 "step, precision and solver options")
+        for a in algs:
+            a.dense_site_operator = True
+        self.algorithms = algs
+        self.krylov_dims: List[int] = []
+
+    def __len__(self):
+        return len(self.algorithms)
+
+    def _solve(self, reqs: List[KrylovRequest]):
+        from .krylov import krylov_exp_action_batched
+        opts = dict(self.algorithms[0].krylov_options())
+        direction = reqs[0].direction
+        override = opts.pop("exp_factor", None)
+        exp_factor = (direction.exp_factor() if override is None
+                      else (override if direction is EvoDirection.FORWARD else -override))
+        net = E.BatchedNetwork([r.action for r in reqs])
+        dims: List[int] = []
+        res = krylov_exp_action_batched([r.x for r in reqs], net, reqs[0].dt, exp_factor,
+                                        krylov_tol=opts.get("krylov_tol", 1e-10),
+                                        max_krylov_dim=opts.get("max_krylov_dim", None),
+                                        hermitian=opts.get("hermitian", True), stats=dims)
+        for r, y, m in zip(reqs, res, dims):
+            r.result, r.applies = y, m
+        self.krylov_dims.extend(dims)
+
+    def run_one_time_step(self):
         E.refresh_stream()
-        self.forward_sweep()
-        self.backwards_sweep()
+        gens = [a._time_step_gen() for a in self.algorithms]
+        reqs = []
+        for g in gens:
+            reqs.append(next(g, None))
+        while True:
+            if all(r is None for r in reqs):
+                return
+            if any(r is None for r in reqs):
+                raise RuntimeError("batched instances left the common sweep schedule")
+            self._solve(reqs)
+            reqs = []
+            for g in gens:
+                try:
+                    reqs.append(g.send(None))
+                except StopIteration:
+                    reqs.append(None)
+
+    def run(self, steps: int):
+        for _ in range(steps):
+            self.run_one_time_step()
