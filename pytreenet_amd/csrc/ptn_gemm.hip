@@ -651,8 +651,8 @@ int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
   const int64_t tiles_m = (d->M + BM - 1) / BM;
   a.tiles_n = (d->N + BN - 1) / BN;
   tiles = tiles_m * a.tiles_n;
-  if (tiles > 0x7fffffffLL) {
-    ptn::set_error("ptn_gemm_gather: too many tiles");
+  if (tiles > (int64_t)(0xffffffffu / NT)) {
+    ptn::set_error("ptn_gemm_gather: %lld tiles exceed one launch", (long long)tiles);
     return PTN_ERANGE;
   }
   a.remap = 0;
@@ -728,8 +728,10 @@ extern "C" int ptn_gemm_gather_grouped(void *stream, const ptn_gemm_desc *descs,
   const int n = (int)ka.size();
   if (n == 0) return PTN_OK;
   const int64_t total = prefix.back();
-  if (total > 0x7fffffffLL) {
-    ptn::set_error("ptn_gemm_gather_grouped: too many tiles");
+  // the dispatch counts work-items in 32 bits
+  if (total > (int64_t)(0xffffffffu / NT)) {
+    ptn::set_error("ptn_gemm_gather_grouped: %lld tiles exceed one launch; split the batch",
+                   (long long)total);
     return PTN_ERANGE;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -268,7 +268,7 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
         return [_offsets(mdims, key[3]), _offsets(kdims, key[4]), _offsets(kdims, key[5]),
                 _offsets(ndims, key[6]), _offsets(mdims, key[7]), _offsets(ndims, key[8])]
 
-    _, ptrs = _CACHE.get(key, build, A.device)
+    tabs, ptrs = _CACHE.get(key, build, A.device)
     M = math.prod(mdims)
     N = math.prod(ndims)
     K = math.prod(kdims) if kdims else 1
@@ -276,13 +276,15 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     kb = _inner_is_unit(kdims, key[5]) and not _inner_is_unit(ndims, key[6])
     aff = _AFFINE.get(key)
     if aff is None:
+        if len(_AFFINE) > 65536:
+            _AFFINE.clear()
         st = [_affine_modes(d, sx) for d, sx in ((mdims, key[3]), (kdims, key[4]),
                                                 (kdims, key[5]), (ndims, key[6]),
                                                 (mdims, key[7]), (ndims, key[8]))]
         aff = _AFFINE[key] = (False if any(x is None for x in st)
                               else tuple((0, x) for x in st))
     _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate, alpha,
-            aff or None)
+            aff or None, tabs)
     return out
 
 
@@ -360,9 +362,13 @@ def _run_blas(sd: StridedDesc, M: int, N: int, K: int, precision: str):
 
 
 def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, accumulate,
-            alpha, affine=None):
+            alpha, affine=None, tabs=None):
     """One contraction launch from device offset tables
     ptrs = (a_off_m, a_off_k, b_off_k, b_off_n, c_off_m, c_off_n).
+
+    ``tabs`` is the device tensor holding those tables: a recorded launch keeps
+    a reference to it, so the tables outlive the offset-table cache's LRU
+    eviction for as long as the recording can be replayed.
 
     ``affine`` holds (base, stride) of each table when all six are arithmetic
     progressions; such a launch with a full tile grid runs as one strided
@@ -405,7 +411,7 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     ws = _splitk_workspace(A.device)
     d.ws, d.ws_bytes = ws.data_ptr(), ws.numel() * ws.element_size()
     if _RECORDER is not None:
-        _RECORDER.ops.append(("gemm", d, A, B, C, M, N, K, precision))
+        _RECORDER.ops.append(("gemm", d, A, B, C, M, N, K, precision, tabs))
     _run_gemm(d, M, N, K, precision)
 
 
@@ -626,10 +632,11 @@ def _contract_block(X: _Blk, Y: torch.Tensor, ly: Sequence[Label], conj_y: bool,
                 host_groups.append((a_m, a_k, b_k, b_n, c_m0 + shift, c_n))
             groups = []
             for tabs in host_groups:
-                _, ptrs = _CACHE.get(("blk", key, len(groups)), lambda t=tabs: list(t), dev)
+                dtabs, ptrs = _CACHE.get(("blk", key, len(groups)), lambda t=tabs: list(t), dev)
                 aff = [_affine_table(t) for t in tabs]
+                # the group keeps its device tables alive (dtabs) beyond cache eviction
                 groups.append((ptrs, len(tabs[0]), len(tabs[1]), len(tabs[3]),
-                               None if any(a is None for a in aff) else tuple(aff)))
+                               None if any(a is None for a in aff) else tuple(aff), dtabs))
             _BLK_CACHE[key] = groups
             if len(_BLK_CACHE) > 4096:
                 _BLK_CACHE.popitem(last=False)
@@ -644,9 +651,9 @@ def _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, t
         not _inner_is_unit([dims[l] for l in m_d], [sx[l] for l in m_d])
     kb = _inner_is_unit([dims[l] for l in k_d], [sy[l] for l in k_d]) and \
         not _inner_is_unit([dims[l] for l in n_l], [sy[l] for l in n_l])
-    for ptrs, M, K, N, aff in groups:
+    for ptrs, M, K, N, aff, dtabs in groups:
         _launch(X.data, Y, target, ptrs, M, N, K, ka, kb, X.conj, conj_y, precision, False,
-                1.0, aff)
+                1.0, aff, dtabs)
 
 
 _BAPPLY_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()
@@ -1038,6 +1045,7 @@ class _Program:
                        "out", 0))
         temps: Dict[int, int] = {}
         self.temp_bytes: List[int] = []
+        self.tables: List[Optional[torch.Tensor]] = []
 
         def classify(t, ptr):
             for a, b, kind, idx in ranges:
@@ -1053,6 +1061,7 @@ class _Program:
 
         self.ops = []
         for op in net.ops:
+            n_before = len(self.temp_bytes)
             if op[0] == "gemm":
                 d = op[1]
                 fields = []
@@ -1061,12 +1070,25 @@ class _Program:
                                    classify(t, getattr(d, name))))
                 tmpl = np.frombuffer(bytes(d), dtype=np.uint8).copy()
                 self.ops.append(("gemm", tmpl, fields))
+                self.tables.append(op[9])   # offset tables the template points into
             elif op[0] == "fill":
                 t = op[1]
                 self.ops.append(("fill", classify(t, t.data_ptr()), t.numel(),
                                  _DTYPE_CODE[t.dtype], op[2]))
             else:
                 raise ValueError(f"batched replay does not support {op[0]!r} ops")
+        # live range of every intermediate: allocated before its first op, released
+        # after its last one (the caching allocator reuses it in stream order)
+        first: Dict[int, int] = {}
+        last: Dict[int, int] = {}
+        for i, op in enumerate(self.ops):
+            roles = [r for _, r in op[2]] if op[0] == "gemm" else [op[1]]
+            for kind, idx, _ in roles:
+                if kind == "tmp":
+                    first.setdefault(idx, i)
+                    last[idx] = i
+        self.alloc_at = [[k for k, f in first.items() if f == i] for i in range(len(self.ops))]
+        self.free_at = [[k for k, l in last.items() if l == i] for i in range(len(self.ops))]
         self.out_shape = tuple(out.shape)
         self.out_dtype = out.dtype
         self.n_in = len(inputs)
@@ -1089,12 +1111,8 @@ class _Program:
         outs = torch.empty((count,) + self.out_shape, dtype=self.out_dtype, device=dev)
         ostride = outs.stride(0) * outs.element_size()
         out_ptr = outs.data_ptr() + ostride * np.arange(count, dtype=np.int64)
-        tmp_ptr = []
-        keep = [outs]
-        for nb in self.temp_bytes:
-            buf = torch.empty(nb * count, dtype=torch.uint8, device=dev)
-            keep.append(buf)
-            tmp_ptr.append(buf.data_ptr() + nb * np.arange(count, dtype=np.int64))
+        tmp_ptr: Dict[int, np.ndarray] = {}
+        bufs: Dict[int, torch.Tensor] = {}
 
         def base(role):
             kind, idx, off = role
@@ -1106,7 +1124,11 @@ class _Program:
 
         lib = _lib.load()
         stream = ctypes.c_void_p(stream_handle())
-        for op in self.ops:
+        for i, op in enumerate(self.ops):
+            for k in self.alloc_at[i]:
+                nb = self.temp_bytes[k]
+                bufs[k] = torch.empty(nb * count, dtype=torch.uint8, device=dev)
+                tmp_ptr[k] = bufs[k].data_ptr() + nb * np.arange(count, dtype=np.int64)
             if op[0] == "gemm":
                 tmpl, fields = op[1], op[2]
                 buf = np.tile(tmpl, (count, 1))
@@ -1125,6 +1147,8 @@ class _Program:
                                                 numel, code, v.real, v.imag,
                                                 ctypes.c_void_p(w.data_ptr())),
                            "ptn_fill_batched")
+            for k in self.free_at[i]:
+                del bufs[k]
         return outs
 
 
@@ -1179,3 +1203,31 @@ class BatchedNetwork:
             for k, i in enumerate(act):
                 res[pos[i]] = outs[k]
         return res
+
+
+def network_live_peak(labels, dims: Dict[Label, int], out_labels) -> int:
+    """Largest number of intermediate elements alive at once while the network
+    is contracted in its planned order (the output counted, inputs not)."""
+    plan = plan_network(labels, dims, out_labels)
+    n = len(labels)
+    members: List[frozenset] = [frozenset([i]) for i in range(n)]
+    out_set = frozenset(out_labels)
+    alive: Dict[int, int] = {}
+    peak = 0
+    for a, b, _ in plan.steps:
+        cover = members[a] | members[b]
+        inside = set()
+        for i in cover:
+            inside |= set(labels[i])
+        outside = set()
+        for i in range(n):
+            if i not in cover:
+                outside |= set(labels[i])
+        res = [x for x in inside if x in out_set or x in outside]
+        members.append(cover)
+        size = math.prod(dims[x] for x in res)
+        peak = max(peak, sum(alive.values()) + size)   # operands and result live together
+        alive.pop(a, None)
+        alive.pop(b, None)
+        alive[len(members) - 1] = size
+    return peak

@@ -678,9 +678,27 @@ class BatchedTDVP:
             a.dense_site_operator = True
         self.algorithms = algs
         self.krylov_dims: List[int] = []
+        self.memory_fraction = 0.5   # share of free device memory a batched solve may use
+        self.max_batch: Optional[int] = None
 
     def __len__(self):
         return len(self.algorithms)
+
+    def _instance_bytes(self, req: KrylovRequest, max_dim: Optional[int]) -> int:
+        """Device bytes one instance needs inside a batched Krylov solve: the
+        basis, the vectors of one step and the live intermediates of its H_eff
+        network (planned order)."""
+        net = req.action
+        n = req.x.numel()
+        m = n if max_dim is None else min(n, int(max_dim))
+        dims: Dict[Any, int] = {}
+        for t, lab in zip(net.tensors, net.labels):
+            t = req.x if t is None else t
+            for l, d in zip(lab, t.shape):
+                dims[l] = d
+        live = E.network_live_peak(net.labels, dims, net.out_labels)
+        esize = torch.empty((), dtype=E.storage_dtype(net.precision)).element_size()
+        return 16 * n * (m + 5) + esize * live
 
     def _solve(self, reqs: List[KrylovRequest]):
         from .krylov import krylov_exp_action_batched
@@ -689,14 +707,31 @@ class BatchedTDVP:
         override = opts.pop("exp_factor", None)
         exp_factor = (direction.exp_factor() if override is None
                       else (override if direction is EvoDirection.FORWARD else -override))
-        net = E.BatchedNetwork([r.action for r in reqs])
+        max_dim = opts.get("max_krylov_dim", None)
+        # instances per launch: bounded by a share of the free device memory, so the
+        # batched buffers never crowd out the factorisation libraries' workspaces
+        free, total = torch.cuda.mem_get_info()
+        cached = torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+        budget = self.memory_fraction * (free + cached)
+        per = max(self._instance_bytes(r, max_dim) for r in reqs)
+        chunk = max(1, min(len(reqs), int(budget // per)))
+        if self.max_batch is not None:
+            chunk = min(chunk, self.max_batch)
         dims: List[int] = []
-        res = krylov_exp_action_batched([r.x for r in reqs], net, reqs[0].dt, exp_factor,
-                                        krylov_tol=opts.get("krylov_tol", 1e-10),
-                                        max_krylov_dim=opts.get("max_krylov_dim", None),
-                                        hermitian=opts.get("hermitian", True), stats=dims)
-        for r, y, m in zip(reqs, res, dims):
-            r.result, r.applies = y, m
+        for c0 in range(0, len(reqs), chunk):
+            part = reqs[c0:c0 + chunk]
+            net = E.BatchedNetwork([r.action for r in part])
+            res = krylov_exp_action_batched([r.x for r in part], net, part[0].dt, exp_factor,
+                                            krylov_tol=opts.get("krylov_tol", 1e-10),
+                                            max_krylov_dim=max_dim,
+                                            hermitian=opts.get("hermitian", True), stats=dims)
+            for r, y in zip(part, res):
+                r.result = y
+            del net, res
+            if chunk < len(reqs) or per * len(part) > 0.25 * total:
+                torch.cuda.empty_cache()
+        for r, m in zip(reqs, dims):
+            r.applies = m
         self.krylov_dims.extend(dims)
 
     def run_one_time_step(self):

@@ -1,0 +1,57 @@
+"""Engine invariants on the device: recorded launches own what they point at."""
+import numpy as np
+import pytest
+import torch
+
+from pytreenet_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def _crand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a)).to("cuda")
+
+
+def _flood_cache(n):
+    """Push n fresh offset-table entries through the LRU cache."""
+    for i in range(n):
+        E._CACHE.get(("flood", i), lambda i=i: [np.arange(3 + i % 5, dtype=np.int64)], "cuda")
+
+
+def test_recorded_tables_survive_cache_eviction(gpu):
+    """A recorded network and a cached block-sparse group keep their offset tables:
+    evicting every cache entry and releasing torch's cached blocks must not change
+    a replay (before the fix such replays read freed tables)."""
+    print("[engine] table ownership", flush=True)
+    rng = np.random.default_rng(4)
+    cap = E._CACHE.cap
+    E._CACHE.cap = 4
+    try:
+        x = _dev(_crand(rng, (16, 12, 2)))
+        env = _dev(_crand(rng, (16, 5, 16)))
+        W = np.zeros((5, 5, 2, 2), dtype=complex)
+        W[0, 0] = np.eye(2)
+        W[1, 2] = _crand(rng, (2, 2))
+        W[3, 4] = _crand(rng, (2, 2))
+        Wb = E.BlockSparse.from_dense(_dev(W), [0, 1])
+        labels = [["a", "c", "p"], ["a", "w", "b"], ["w", "v", "q", "p"]]
+        out = ["b", "c", "v", "q"]
+        net = E.CompiledNetwork([x, env, Wb], labels, out)
+        first = net(x).cpu().numpy()
+        ref_block = E.contract_network([x, env, Wb], labels, out).cpu().numpy()
+        for rep in range(3):
+            _flood_cache(16)
+            torch.cuda.empty_cache()
+            y = _dev(_crand(rng, (16, 12, 2)))
+            got = net(y).cpu().numpy()
+            ref = E.contract_network([y, env, _dev(W)], labels, out).cpu().numpy()
+            np.testing.assert_allclose(got, ref, atol=1e-12 * np.abs(ref).max())
+            again = E.contract_network([x, env, Wb], labels, out).cpu().numpy()
+            np.testing.assert_allclose(again, ref_block, atol=1e-12 * np.abs(ref_block).max())
+        assert np.abs(first).max() > 0
+    finally:
+        E._CACHE.cap = cap
