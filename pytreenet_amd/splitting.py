@@ -266,8 +266,61 @@ def absorb_into_neighbour(ttn, node_id: str, from_id: str, r: torch.Tensor):
     ttn.tensors[node_id] = E.contract(r, ["new", "old"], t, lt, lo)
 
 
+SVD_STATS = {"direct": 0, "qr_tall": 0, "qr_wide": 0, "qr_fallback": 0}
+# aspect ratio from which a panel is reduced by QR before the SVD
+SVD_QR_RATIO = 4
+
+
 def svd_colmajor(panel: torch.Tensor, m: int, n: int):
-    """Thin SVD of a column-major m x n panel (destroyed): (s host, U_cm, Vh_cm, k)."""
+    """Thin SVD of a column-major m x n panel (destroyed): (s host, U_cm, Vh_cm, k).
+
+    Strongly rectangular panels (long side >= SVD_QR_RATIO x short side, short
+    side 8..1024) are first reduced by a CholeskyQR3 of the long side, A = Q R
+    (tall) or A^T = Q R (wide), and only the small triangle goes through the
+    SVD: A = (Q U_R) S V_R^H, or A = U_B S (W^H Q^T) with R^T = U_B S W^H.
+    Both steps are backward stable, so the singular values equal those of the
+    direct SVD to O(u ||A||).  A collapsed CholeskyQR pivot (numerically rank
+    deficient panel) falls back to the direct SVD.
+    """
+    short, long_ = min(m, n), max(m, n)
+    if 8 <= short <= 1024 and long_ >= SVD_QR_RATIO * short:
+        hit = _svd_via_qr(panel, m, n)
+        if hit is not None:
+            return hit
+        SVD_STATS["qr_fallback"] += 1
+    SVD_STATS["direct"] += 1
+    return _svd_direct(panel, m, n)
+
+
+def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
+    dev = panel.device
+    T = panel.view(n, m)                 # T[j][i] = A(i, j)
+    if m >= n:
+        # A = Q R: matricise T as (i) x (j), the QR leg is j (leg 0)
+        q, R, ok = cholesky_qr3_leg(T, 0)  # q[r][i] = Q(i, r), R[r][j]
+        if not ok:
+            return None
+        B = E.permute_copy(R, [1, 0])      # B col-major n x n = R
+        s, Ub, Vhb, k = _svd_direct(B.reshape(-1), n, n)
+        U = torch.empty(m * k, dtype=torch.complex128, device=dev)
+        E.contract(q, ["r", "i"], torch.as_strided(Ub, (n, k), (1, n)), ["r", "c"], ["i", "c"],
+                   out=torch.as_strided(U, (m, k), (1, m)))
+        SVD_STATS["qr_tall"] += 1
+        return s, U, Vhb, k
+    # A^T = Q R: matricise T as (j) x (i), the QR leg is i (leg 1)
+    q, R, ok = cholesky_qr3_leg(T, 1)      # q[j][r] = Q(j, r), R[r][i]
+    if not ok:
+        return None
+    B = R.contiguous().clone()             # col-major m x m of R^T: B(i, r) = R[r][i]
+    s, Ub, Whb, k = _svd_direct(B.reshape(-1), m, m)
+    Vh = torch.empty(k * n, dtype=torch.complex128, device=dev)
+    E.contract(torch.as_strided(Whb, (k, m), (1, k)), ["c", "r"], q, ["j", "r"], ["c", "j"],
+               out=torch.as_strided(Vh, (k, n), (1, k)))
+    SVD_STATS["qr_wide"] += 1
+    return s, Ub, Vh, k
+
+
+def _svd_direct(panel: torch.Tensor, m: int, n: int):
     k = min(m, n)
     dev = panel.device
     S = torch.empty(k, dtype=torch.float64, device=dev)

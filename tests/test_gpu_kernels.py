@@ -368,3 +368,31 @@ def test_contract_blas_matches_gather(gpu):
         E.USE_BLAS = True
     np.testing.assert_allclose(again, ref, atol=1e-11 * np.abs(ref).max())
     assert not np.allclose(first, again)
+
+
+@pytest.mark.parametrize("m,n,rank", [(2000, 64, None), (48, 3000, None), (40, 640, 12),
+                                      (256, 4096, None)])
+def test_svd_qr_preconditioned(gpu, m, n, rank):
+    """Rectangular panels go through CholeskyQR3 + small SVD (or, rank deficient, the
+    direct SVD): singular values equal numpy's to O(u ||A||), U S V^H = A, U and V
+    orthonormal."""
+    _say(f"svd via qr {m}x{n} rank={rank}")
+    rng = np.random.default_rng(m + n)
+    A = _crand(rng, (m, n))
+    if rank is not None:
+        A = _crand(rng, (m, rank)) @ _crand(rng, (rank, n))
+    panel = _dev(np.asfortranarray(A).reshape(-1, order="F").copy())
+    before = dict(S.SVD_STATS)
+    s, U, Vh, k = S.svd_colmajor(panel, m, n)
+    ref = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(s, ref, atol=1e-13 * ref[0])
+    Um = U.cpu().numpy().reshape(k, m).T
+    Vm = Vh.cpu().numpy().reshape(n, k).T
+    np.testing.assert_allclose((Um * s) @ Vm, A, atol=1e-12 * ref[0])
+    if rank is None:
+        np.testing.assert_allclose(Um.conj().T @ Um, np.eye(k), atol=1e-12)
+        np.testing.assert_allclose(Vm @ Vm.conj().T, np.eye(k), atol=1e-12)
+        assert S.SVD_STATS["qr_tall" if m > n else "qr_wide"] == before["qr_tall" if m > n
+                                                                          else "qr_wide"] + 1
+    else:
+        assert S.SVD_STATS["qr_fallback"] == before["qr_fallback"] + 1
