@@ -34,7 +34,7 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
 # measured HBM bytes per LARGE FP64 launch of each backend (rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE on this command, scaled by the copy calibration; the
 # summaries are under profiles/); None where not yet measured
-TRAFFIC_BYTES_PER_LAUNCH = {"gather": None, "blas": None}
+TRAFFIC_BYTES_PER_LAUNCH = {"gather": 3.80e8, "blas": None}
 KERNEL_NAME = {"gather": "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, F64_4M)",
                "blas": "rocblas_zgemm (ptn_gemm_strided)"}
 METRIC = "TDVP sweeps/s & H_eff TFLOP/s at χ=256, mixed vs FP64 GPU at ΔE≤1e-8"
