@@ -125,7 +125,7 @@ int ptn_gemm_gather_grouped(void *stream, const ptn_gemm_desc *descs, int32_t co
                             void *work);
 
 /* ptn_krylov_orthonormalize / ptn_normalize / ptn_krylov_combine over
- * `count` instances of one vector length n: instance b uses Q[b] (row stride
+ * `count` instances of one vector length n (nvec <= 128): instance b uses Q[b] (row stride
  * ldq[b]), w[b], qnext[b] (qnext itself may be NULL), out[b]; combine takes
  * coeffs as a host array of count x nvec complex128 (instance-major).      */
 int64_t ptn_krylov_batched_work_bytes(int32_t count);
@@ -172,6 +172,8 @@ int ptn_fill(void *stream, void *dst, int64_t n, int32_t dtype, double re, doubl
  *   h1 = Q^H w;  w -= Q h1;  h2 = Q^H w;  w -= Q h2   (two classical passes),
  *   beta_sq     = ||w||^2 after projection.
  * Writes out[0..nvec) = h1 + h2 (complex128), out[nvec] = (w0sq, beta_sq).
+ * nvec <= 128 per call (PTN_ERANGE beyond); longer bases are projected out in
+ * chunks by the caller.
  * `work` must hold at least ptn_krylov_work_bytes(n, nvec) bytes.           */
 int64_t ptn_krylov_work_bytes(int64_t n, int32_t nvec);
 int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, int64_t ldq,

@@ -70,7 +70,7 @@ namespace {
 
 constexpr int RT = 256;       // threads per reduction block
 constexpr int KV = 8;         // vectors per sweep over w
-constexpr int MAXV = 64;      // max basis vectors per call
+constexpr int MAXV = 128;     // max basis vectors per call
 constexpr int MAXBLK = 1024;  // max partial blocks
 
 inline int nblocks_for(int64_t n) {
@@ -426,7 +426,7 @@ struct BVec {
   double2 *out;      // packed scalars
   const double2 *c;  // combine coefficients
 };
-constexpr int BNB = 64;                      // slices per instance (long vectors)
+constexpr int BNB = 32;                      // slices per instance (long vectors)
 constexpr int BPSTRIDE = MAXV + 2;           // partial row length
 constexpr int BSCR = 2 * (MAXV + 1) + 2;     // h1, h2, inv per instance
 

@@ -193,6 +193,167 @@ __global__ __launch_bounds__(CT) void potrf_lower_kernel(int n, const double2 *G
   if (tid == 0) *status = sh_fail;
 }
 
+// Blocked right-looking Cholesky in one workgroup (the working matrix L lives in
+// global memory, L2 resident for n <= 1024): for each block column of PB
+// columns, (a) the PB x PB diagonal block is factored in LDS, (b) the panel below
+// it is solved against that block, one thread per row (x L_JJ^H = a), and (c) the
+// trailing lower triangle is updated, L22 -= L21 L21^H, in 4x4 register tiles
+// spread over all threads.  Same shift, pivot test and status as the unblocked
+// kernel above.
+constexpr int PB = 16;
+constexpr int PT = 512;   // threads of the blocked kernel (256 VGPRs per lane)
+
+__global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 *G, int64_t ldg,
+                                                           double2 *L, double shift_factor,
+                                                           double min_pivot, int *status) {
+  __shared__ double2 D[PB][PB + 1];
+  __shared__ double red[PT / 64];
+  __shared__ double sh_shift, sh_inv;
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x;
+  double tr = 0;
+  for (int i = tid; i < n; i += PT) tr += G[(int64_t)i * ldg + i].x;
+  for (int off = 32; off > 0; off >>= 1) tr += __shfl_down(tr, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = tr;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0;
+    for (int w = 0; w < PT / 64; ++w) t += red[w];
+    sh_shift = shift_factor * t;
+    sh_fail = 0;
+  }
+  __syncthreads();
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t t = tid; t < nn; t += PT) {
+    const int i = (int)(t / n), j = (int)(t - (int64_t)i * n);
+    double2 v = make_double2(0, 0);
+    if (j <= i) {
+      v = G[(int64_t)i * ldg + j];
+      if (i == j) v = make_double2(v.x + sh_shift, 0.0);
+    }
+    L[t] = v;
+  }
+  __syncthreads();
+  for (int J = 0; J < n; J += PB) {
+    const int jb = (n - J) < PB ? (n - J) : PB;
+    // (a) diagonal block in LDS
+    for (int t = tid; t < PB * PB; t += PT) {
+      const int i = t / PB, k = t % PB;
+      D[i][k] = (i < jb && k <= i) ? L[(int64_t)(J + i) * n + J + k] : make_double2(0, 0);
+    }
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+      if (tid == 0) {
+        double d = D[j][j].x;
+        if (!(d > min_pivot)) {  // also catches NaN
+          sh_fail = 1;
+          d = 1.0;
+        }
+        const double r = sqrt(d);
+        D[j][j] = make_double2(r, 0.0);
+        sh_inv = 1.0 / r;
+      }
+      __syncthreads();
+      if (tid > j && tid < jb) {
+        D[tid][j].x *= sh_inv;
+        D[tid][j].y *= sh_inv;
+      }
+      __syncthreads();
+      if (tid < PB * PB) {
+        const int i = tid / PB, k = tid % PB;
+        if (i < jb && k > j && k <= i) {
+          const double2 a = D[i][j], b = D[k][j];
+          D[i][k].x -= a.x * b.x + a.y * b.y;  // a * conj(b)
+          D[i][k].y -= a.y * b.x - a.x * b.y;
+        }
+      }
+      __syncthreads();
+    }
+    for (int t = tid; t < jb * jb; t += PT) {
+      const int i = t / jb, k = t % jb;
+      if (k <= i) L[(int64_t)(J + i) * n + J + k] = D[i][k];
+    }
+    // (b) panel rows below the block: x L_JJ^H = a, i.e. x_k = (a_k - sum_{l<k} x_l
+    //     conj(L_kl)) / L_kk.  Sixteen lanes per row, lane k owns x_k; x_l is
+    //     broadcast within the 16-lane group once it is final.
+    {
+      const int lk = tid & (PB - 1), grp = tid / PB;
+      for (int i0 = J + jb; i0 < n; i0 += PT / PB) {
+        const int i = i0 + grp;
+        const bool ok = i < n && lk < jb;
+        double2 v = ok ? L[(int64_t)i * n + J + lk] : make_double2(0, 0);
+        for (int l = 0; l < jb; ++l) {
+          if (lk == l) {
+            const double inv = 1.0 / D[l][l].x;
+            v.x *= inv;
+            v.y *= inv;
+          }
+          const double xr = __shfl(v.x, l, PB), xi = __shfl(v.y, l, PB);
+          if (lk > l) {
+            const double2 b = D[lk][l];   // zero outside the block
+            v.x -= xr * b.x + xi * b.y;
+            v.y -= xi * b.x - xr * b.y;
+          }
+        }
+        if (ok) L[(int64_t)i * n + J + lk] = v;
+      }
+    }
+    __syncthreads();
+    // (c) trailing update of the lower triangle, 4x4 tiles
+    const int r0 = J + jb, rem = n - r0;
+    if (rem > 0) {
+      const int nt = (rem + 3) / 4;
+      const int ntiles = nt * (nt + 1) / 2;
+      for (int t = tid; t < ntiles; t += PT) {
+        int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        while (ti * (ti + 1) / 2 > t) --ti;
+        const int tk = t - ti * (ti + 1) / 2;
+        double2 acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[r][c] = make_double2(0, 0);
+        int rows[4], cols[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          rows[r] = r0 + ti * 4 + r;
+          cols[r] = r0 + tk * 4 + r;
+        }
+        for (int l = 0; l < jb; ++l) {
+          double2 a[4], b[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            a[r] = rows[r] < n ? L[(int64_t)rows[r] * n + J + l] : make_double2(0, 0);
+            b[r] = cols[r] < n ? L[(int64_t)cols[r] * n + J + l] : make_double2(0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              acc[r][c].x += a[r].x * b[c].x + a[r].y * b[c].y;
+              acc[r][c].y += a[r].y * b[c].x - a[r].x * b[c].y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if (rows[r] < n && cols[c] <= rows[r]) {
+              double2 *p = L + (int64_t)rows[r] * n + cols[c];
+              double2 v = *p;
+              v.x -= acc[r][c].x;
+              v.y -= acc[r][c].y;
+              *p = v;
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *status = sh_fail;
+}
+
 // X = L^{-1} (lower); block j solves L x = e_j by forward substitution
 constexpr int TT = 256;
 __global__ __launch_bounds__(TT) void trtri_lower_kernel(int n, const double2 *L, double2 *X) {
@@ -245,7 +406,7 @@ extern "C" int ptn_potrf_lower(void *stream, int32_t n, const void *G, int64_t l
     return n > 1024 ? PTN_ERANGE : PTN_EINVAL;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(potrf_lower_kernel, dim3(1), dim3(CT), 0, s, n,
+  hipLaunchKernelGGL(potrf_blocked_kernel, dim3(1), dim3(PT), 0, s, n,
                      static_cast<const double2 *>(G), ldg, static_cast<double2 *>(L),
                      shift_factor, min_pivot, static_cast<int *>(status));
   PTN_LAUNCH_CHECK();

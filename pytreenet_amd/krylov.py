@@ -81,6 +81,34 @@ class TimeEvoMode(Enum):
     def action_evolvable(self) -> bool:
         return self is TimeEvoMode.KRYLOV
 
+    def is_dense(self) -> bool:
+        """Modes that exponentiate a dense matrix (``util/std_utils.py:134-175``)."""
+        return self in (TimeEvoMode.FASTEST, TimeEvoMode.EXPM, TimeEvoMode.CHEBYSHEV,
+                        TimeEvoMode.SPARSE)
+
+    def time_evolve(self, psi: torch.Tensor, hamiltonian: torch.Tensor,
+                    time_difference: float,
+                    forward: Union[EvoDirection, bool] = EvoDirection.FORWARD,
+                    **options) -> torch.Tensor:
+        """exp(f dt H) psi for a dense device matrix H (``time_evolution.py:617-655``).
+
+        KRYLOV runs the Krylov recurrence on H @ y.  The dense modes (FASTEST, EXPM,
+        CHEBYSHEV, SPARSE) all compute the exact exponential action, here as
+        ``torch.linalg.matrix_exp`` of the (n x n) exponent on the device.
+        """
+        if isinstance(forward, bool):
+            forward = EvoDirection.from_bool(forward)
+        if self.action_evolvable():
+            return self.time_evolve_action(psi, lambda t, y: (hamiltonian @ y.reshape(-1)),
+                                           time_difference, forward=forward, **options)
+        if not self.is_dense():
+            raise NotImplementedError(f"mode {self.value}: no device implementation")
+        E.require_device(psi, hamiltonian)
+        override = options.pop("exp_factor", None)
+        exp_factor = (forward.exp_factor() if override is None
+                      else (override if forward is EvoDirection.FORWARD else -override))
+        return dense_exp_action(hamiltonian, psi, exp_factor * time_difference)
+
     def time_evolve_action(self, psi: torch.Tensor, time_evo_action: Callable,
                            time_difference: float,
                            forward: Union[EvoDirection, bool] = EvoDirection.FORWARD,
@@ -99,6 +127,18 @@ class TimeEvoMode(Enum):
                                  exp_factor, krylov_tol=options.pop("krylov_tol", 1e-10),
                                  max_krylov_dim=options.pop("max_krylov_dim", None),
                                  hermitian=hermitian)
+
+
+def dense_exp_action(hamiltonian: torch.Tensor, psi: torch.Tensor, factor: complex
+                     ) -> torch.Tensor:
+    """exp(factor * H) psi with H dense (n x n) on the device, complex128."""
+    H = hamiltonian.to(torch.complex128)
+    n = psi.numel()
+    if H.shape != (n, n):
+        raise ValueError(f"dense operator {tuple(H.shape)} does not act on a vector of {n}")
+    U = torch.linalg.matrix_exp(H * complex(factor))
+    y = U @ psi.to(torch.complex128).reshape(-1)
+    return y.reshape(psi.shape)
 
 
 # ---------------------------------------------------------------------------------------
@@ -151,9 +191,20 @@ def combine(Q: torch.Tensor, coeffs, out: torch.Tensor, nvec: Optional[int] = No
     nvec = len(coeffs) if nvec is None else nvec
     c = torch.as_tensor(np.asarray(coeffs, dtype=np.complex128)).pin_memory().to(
         Q.device, non_blocking=True)
-    _lib.check(_lib.load().ptn_krylov_combine(ctypes.c_void_p(E.stream_handle()),
-                                              Q.shape[1], _vp(Q), Q.stride(0), nvec, _vp(c),
-                                              _vp(out)), "ptn_krylov_combine")
+    lib = _lib.load()
+    stream = ctypes.c_void_p(E.stream_handle())
+    if nvec <= 128:
+        _lib.check(lib.ptn_krylov_combine(stream, Q.shape[1], _vp(Q), Q.stride(0), nvec,
+                                          _vp(c), _vp(out)), "ptn_krylov_combine")
+        return out
+    part = torch.empty_like(out)
+    for c0 in range(0, nvec, 128):
+        cnt = min(128, nvec - c0)
+        dst = out if c0 == 0 else part
+        _lib.check(lib.ptn_krylov_combine(stream, Q.shape[1], _vp(Q[c0]), Q.stride(0), cnt,
+                                          _vp(c[c0:]), _vp(dst)), "ptn_krylov_combine")
+        if c0:
+            out.add_(part)
     return out
 
 
@@ -196,6 +247,29 @@ STATS = KrylovStats()
 
 
 LOOKAHEAD_N = 1 << 16
+MAXV_PER_CALL = 128      # basis vectors per ptn_krylov_orthonormalize call
+
+
+def _orthonormalize_chunked(lib, stream, n: int, Q: torch.Tensor, nvec: int, w: torch.Tensor,
+                            packed: torch.Tensor, work: torch.Tensor):
+    """The orthonormalisation step for a basis longer than one call takes: the
+    basis is projected out chunk by chunk (two passes per chunk), the chunks'
+    coefficients land in packed[1 : nvec+1] and (|w_in|^2, |w_out|^2) in
+    packed[nvec+1], as the one-call form writes them."""
+    tmp = torch.empty(MAXV_PER_CALL + 1, dtype=torch.complex128, device=Q.device)
+    first = None
+    for c0 in range(0, nvec, MAXV_PER_CALL):
+        cnt = min(MAXV_PER_CALL, nvec - c0)
+        last = c0 + cnt == nvec
+        _lib.check(lib.ptn_krylov_orthonormalize(
+            stream, n, _vp(Q[c0]), Q.stride(0), cnt, _vp(w),
+            _vp(Q[nvec]) if last else ctypes.c_void_p(0), _vp(tmp), _vp(work)),
+            "ptn_krylov_orthonormalize")
+        packed[1 + c0: 1 + c0 + cnt].copy_(tmp[:cnt])
+        if first is None:
+            first = tmp[cnt].real.clone()
+        if last:
+            packed[nvec + 1] = torch.complex(first, tmp[cnt].imag)
 _PINNED: Dict[int, torch.Tensor] = {}
 
 
@@ -245,9 +319,12 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
         w = _flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
         if w.data_ptr() == Q[j].data_ptr():
             w = w.clone()
-        _lib.check(lib.ptn_krylov_orthonormalize(stream, n, _vp(Q), Q.stride(0), j + 1,
-                                                 _vp(w), _vp(Q[j + 1]), _vp(packed[1:]),
-                                                 _vp(work)), "ptn_krylov_orthonormalize")
+        if j + 1 <= MAXV_PER_CALL:
+            _lib.check(lib.ptn_krylov_orthonormalize(stream, n, _vp(Q), Q.stride(0), j + 1,
+                                                     _vp(w), _vp(Q[j + 1]), _vp(packed[1:]),
+                                                     _vp(work)), "ptn_krylov_orthonormalize")
+        else:
+            _orthonormalize_chunked(lib, stream, n, Q, j + 1, w, packed, work)
         host_buf[: j + 3].copy_(packed[: j + 3], non_blocking=True)
         done.record()
         if lookahead and j + 1 < max_steps:

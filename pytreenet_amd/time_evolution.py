@@ -279,10 +279,9 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         super().__init__(initial_state, hamiltonian, time_step_size, final_time, operators,
                          config=config, solver_options=solver_options)
         mode = self.config.time_evo_mode
-        if mode is TimeEvoMode.FASTEST:
-            mode = TimeEvoMode.KRYLOV
-        if mode is not TimeEvoMode.KRYLOV:
-            raise NotImplementedError("the device path evolves with TimeEvoMode.KRYLOV")
+        if mode is not TimeEvoMode.KRYLOV and not mode.is_dense():
+            raise NotImplementedError(f"mode {mode.value}: the device path evolves with "
+                                      "TimeEvoMode.KRYLOV or a dense exponential mode")
         self._mode = mode
         self.precision = getattr(self.config, "precision", E.FP64)
         if self.precision not in (E.FP64, E.FP32):
@@ -312,11 +311,30 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
     def update_tree_cache(self, node_id: str, next_node_id: str):
         self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
 
+    # FASTEST evolves with the dense effective Hamiltonian (like the reference) up to
+    # this local dimension and with the Krylov recurrence above it, where a dense
+    # matrix (dim^2 complex) is out of reach
+    DENSE_MAX_DIM = 4096
+
     def _krylov(self, x: torch.Tensor, action, dt: float, direction: EvoDirection):
-        opts = dict(TimeEvoMode.default_solver_options(self._mode))
+        """exp(f dt H_eff) x for the site, bond or two-site action ``action``.
+
+        KRYLOV (and FASTEST above DENSE_MAX_DIM) runs the matrix-free Krylov
+        exponential; the dense modes build H_eff as a matrix (the network without x,
+        ``effective_hamiltonians.py:20-470``) and exponentiate it exactly
+        (``time_evolution.py:617-655``, ``util/std_utils.py:134-175``).
+        """
+        mode = self._mode
+        if mode is TimeEvoMode.FASTEST and x.numel() > self.DENSE_MAX_DIM:
+            mode = TimeEvoMode.KRYLOV
+        opts = dict(TimeEvoMode.default_solver_options(mode))
         opts.update(self.solver_options)
-        return self._mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
-                                             **opts)
+        if mode is TimeEvoMode.KRYLOV:
+            return mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
+                                           **opts)
+        H = E.dense_operator(action)
+        return mode.time_evolve(x, H, dt, forward=direction,
+                                **{k: v for k, v in opts.items() if k == "exp_factor"})
 
     def krylov_options(self) -> Dict[str, Any]:
         opts = dict(TimeEvoMode.default_solver_options(self._mode))
@@ -702,6 +720,13 @@ class BatchedTDVP:
 
     def _solve(self, reqs: List[KrylovRequest]):
         from .krylov import krylov_exp_action_batched
+        a0 = self.algorithms[0]
+        if a0._mode is not TimeEvoMode.KRYLOV and not (
+                a0._mode is TimeEvoMode.FASTEST and reqs[0].x.numel() > a0.DENSE_MAX_DIM):
+            # dense exponential modes: one dense solve per instance
+            for a, r in zip(self.algorithms, reqs):
+                r.result, r.applies = a._krylov(r.x, r.action, r.dt, r.direction), 0
+            return
         opts = dict(self.algorithms[0].krylov_options())
         direction = reqs[0].direction
         override = opts.pop("exp_factor", None)
