@@ -910,6 +910,9 @@ def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
             tb, lb_, cb = B_
             if res is None:
                 res = tuple([l for l in la_ if l not in lb_] + [l for l in lb_ if l not in la_])
+                if LAYOUT_LOOKAHEAD:
+                    res = _lookahead_layout(res, len(tensors) + idx, plan.steps, idx, slots,
+                                            dims)
             r = contract(ta, la_, tb, lb_, res, conj_a=ca, conj_b=cb, precision=precision,
                          out=out if last else None,
                          out_dtype=out_dtype if last else inter)
@@ -918,6 +921,48 @@ def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
         slots[sb] = None
     final = slots[-1]
     return final[0]
+
+
+LAYOUT_LOOKAHEAD = True
+
+
+def _lookahead_layout(res: tuple, slot: int, steps, idx: int, slots, dims) -> tuple:
+    """Memory order of the intermediate made by step ``idx`` (slot ``slot``).
+
+    When its consumer step is a long-K contraction with a small output (a
+    split-K launch: < 256 output tiles, K >= 1024), the labels that step
+    contracts go innermost, in the order of the consumer's other operand (block
+    labels first, then dense labels, for a block-sparse partner): both operands
+    then have the same contiguous innermost K mode, so the tile rows of
+    consecutive k share cache lines.  Otherwise, or when the partner's layout is
+    not known yet, the default order is kept (it keeps strided launches strided
+    and the block-apply rows contiguous)."""
+    for j in range(idx + 1, len(steps)):
+        a, b, _ = steps[j]
+        if slot in (a, b):
+            pi = b if a == slot else a
+            if pi >= len(slots):      # produced after this step: layout not known yet
+                return res
+            partner = slots[pi]
+            break
+    else:
+        return res
+    if partner is None:
+        return res
+    if isinstance(partner, _Blk):
+        order = list(partner.blabels) + list(partner.dense_labels)
+    else:
+        order = list(partner[1])
+    pset = set(order)
+    rset = set(res)
+    m = math.prod(dims[l] for l in res if l not in pset)
+    n = math.prod(dims[l] for l in order if l not in rset)
+    k = math.prod(dims[l] for l in res if l in pset)
+    if -(-m // 64) * -(-n // 64) >= 256 or k < 1024:
+        return res
+    free = [l for l in res if l not in pset]
+    shared = [l for l in order if l in set(res)]
+    return tuple(free + shared)
 
 
 class CompiledNetwork:
