@@ -197,6 +197,24 @@ int ptn_normalize(void *stream, int64_t n, const void *x, void *y, void *out, vo
 int ptn_krylov_combine(void *stream, int64_t n, const void *Q, int64_t ldq,
                        int32_t nvec, const void *coeffs, void *out);
 
+/* The whole Krylov exponential y = exp(f dt H) x of a short vector in one launch
+ * (replaces krylov_exp_action, time_evolution/time_evolution.py:721-830, for
+ * the small local spaces of a sweep): H is the dense n x n effective
+ * Hamiltonian (complex128, device, column-major: element (r, k) at H[r + k n]),
+ * f = exp_re + i exp_im,
+ * n <= 1024, max_steps <= min(n, 24).  Same recurrence and stop rules as the
+ * ptn_normalize / ptn_krylov_orthonormalize loop (CGS2, breakdown at
+ * max(eps n, 1e-12 |H q_j|), Saad estimate < krylov_tol when krylov_tol > 0);
+ * the small propagator column is exp(f dt S) e_1 with S = (T + T^H)/2 when
+ * hermitian != 0, else T.  work: ptn_krylov_small_work_bytes(n, max_steps)
+ * bytes of device memory; info (device int32[3]): steps used, 1 when |x| = 0
+ * (y = x), 1 when a stop rule ended the recurrence before max_steps.        */
+int64_t ptn_krylov_small_work_bytes(int64_t n, int32_t max_steps);
+int ptn_krylov_expm_small(void *stream, int64_t n, const void *H, const void *x, void *y,
+                          int32_t max_steps, double time_difference, double exp_re,
+                          double exp_im, double krylov_tol, int32_t hermitian, void *work,
+                          void *info);
+
 /* ||x||^2 of a complex128 vector into out[0] (double, device).              */
 int ptn_norm_sq(void *stream, int64_t n, const void *x, void *out, void *work);
 

@@ -18,7 +18,8 @@ PTN_COMPUTE_F64_4M, PTN_COMPUTE_F32_3M = 0, 1
 EXPORTED = (
     "ptn_gemm_gather", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
     "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize", "ptn_normalize",
-    "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc",
+    "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc", "ptn_krylov_small_work_bytes",
+    "ptn_krylov_expm_small",
     "ptn_gemm_strided", "ptn_gemm_strided_ok", "ptn_qr", "ptn_svd",
     "ptn_gemm_grouped_work_bytes", "ptn_gemm_gather_grouped", "ptn_krylov_batched_work_bytes",
     "ptn_krylov_orthonormalize_batched", "ptn_normalize_batched", "ptn_krylov_combine_batched",
@@ -88,6 +89,9 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_normalize": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
         "ptn_krylov_combine": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp]),
         "ptn_norm_sq": (ctypes.c_int, [vp, i64, vp, vp, vp]),
+        "ptn_krylov_small_work_bytes": (i64, [i64, i32]),
+        "ptn_krylov_expm_small": (ctypes.c_int, [vp, i64, vp, vp, vp, i32, dbl, dbl, dbl, dbl,
+                                                 i32, vp, vp]),
         "ptn_dotc": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
         "ptn_qr": (ctypes.c_int, [vp, i64, i64, vp, vp, vp]),
         "ptn_svd": (ctypes.c_int, [vp, i64, i64, vp, vp, vp, vp, vp]),

@@ -996,3 +996,288 @@ extern "C" int ptn_permute_batched(void *stream, int32_t count, int32_t rank, co
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }
+
+// ---- whole Krylov exponential of a short vector in one workgroup -----------------
+// For n <= 1024 and at most KMAX Krylov steps: q_0 = x/|x|, then per step
+// w = H q_j, two classical Gram-Schmidt passes, beta = |w|, the stop rules of
+// krylov_exp_action (breakdown at max(eps n, 1e-12 |H q_j|), Saad estimate
+// |x| beta |[exp(f dt T_m) e_1]_{m-1}| < tol), q_{j+1} = w / beta; finally
+// y = |x| Q exp(f dt T_m) e_1.
+// H is column-major.  The matvec splits k into S slices (S = min(KXT / n, n)):
+// thread (row r, slice s) sums H[r + k n] q[k] over k = s, s + S, ... with q_j
+// staged in LDS, so a wave loads consecutive rows of one column; the slices
+// are added through LDS.
+// The small exponential is a degree-KTAYLOR Taylor polynomial of (f dt S)/2^s
+// (1-norm <= 1/2, truncation below 3e-17) by Horner in two LDS buffers,
+// squared s times (S = (T + T^H)/2 when hermitian, else T): the same column
+// as the eigendecomposition form to rounding.
+// info[0] = m used, info[1] = 1 when |x| = 0 (then y = x), info[2] = 1 when a
+// stop rule ended the recurrence before max_steps.
+namespace {
+constexpr int KXT = 512;
+constexpr int KMAX = 24;
+constexpr int KNMAX = 1024;
+constexpr int KTAYLOR = 14;
+constexpr int KLD = KMAX + 1;
+
+struct SmallLds {
+  double2 T[KLD][KLD];
+  double2 A[KLD][KLD];
+  double2 E0[KLD][KLD];
+  double2 E1[KLD][KLD];
+  double2 q[KNMAX];
+  double2 part[KXT];
+  double2 hc[KMAX];
+  double red[KXT / 64];
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// block sum of one double (every thread gets the result)
+__device__ double block_sum1(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  for (int w = 0; w < KXT / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// exp(c S) of the leading m x m block of S(T); returns the LDS buffer holding
+// it (column 0 is the propagator column).  All threads call it.
+__device__ double2 (*small_expm(int m, SmallLds &L, int hermitian, double2 c))[KLD] {
+  const int tid = threadIdx.x;
+  for (int t = tid; t < m * m; t += KXT) {
+    const int r = t / m, k = t % m;
+    double2 s = L.T[r][k];
+    if (hermitian) {
+      const double2 u = L.T[k][r];
+      s = make_double2(0.5 * (s.x + u.x), 0.5 * (s.y - u.y));
+    }
+    L.A[r][k] = cmul(c, s);
+  }
+  __syncthreads();
+  double nrm = 0;
+  if (tid < m)
+    for (int r = 0; r < m; ++r) nrm += sqrt(L.A[r][tid].x * L.A[r][tid].x + L.A[r][tid].y * L.A[r][tid].y);
+  for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_down(nrm, off, 64));
+  if ((tid & 63) == 0) L.red[tid >> 6] = nrm;
+  __syncthreads();
+  double amax = 0;
+  for (int w = 0; w < KXT / 64; ++w) amax = fmax(amax, L.red[w]);
+  int sq = 0;
+  while (amax > 0.5 && sq < 60) {
+    amax *= 0.5;
+    ++sq;
+  }
+  const double scale = ldexp(1.0, -sq);
+  for (int t = tid; t < m * m; t += KXT) {
+    const int r = t / m, k = t % m;
+    L.A[r][k].x *= scale;
+    L.A[r][k].y *= scale;
+    L.E0[r][k] = make_double2(r == k ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  double2(*cur)[KLD] = L.E0;
+  double2(*nxt)[KLD] = L.E1;
+  // Horner: E <- I + A E / k for k = KTAYLOR .. 1
+  for (int k = KTAYLOR; k >= 1; --k) {
+    const double inv = 1.0 / k;
+    for (int t = tid; t < m * m; t += KXT) {
+      const int r = t / m, cc = t % m;
+      double2 acc = make_double2(0, 0);
+      for (int l = 0; l < m; ++l) {
+        const double2 p = cmul(L.A[r][l], cur[l][cc]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+      nxt[r][cc] = make_double2(acc.x * inv + (r == cc ? 1.0 : 0.0), acc.y * inv);
+    }
+    __syncthreads();
+    double2(*t)[KLD] = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  for (int q = 0; q < sq; ++q) {
+    for (int t = tid; t < m * m; t += KXT) {
+      const int r = t / m, cc = t % m;
+      double2 acc = make_double2(0, 0);
+      for (int l = 0; l < m; ++l) {
+        const double2 p = cmul(cur[r][l], cur[l][cc]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+      nxt[r][cc] = acc;
+    }
+    __syncthreads();
+    double2(*t)[KLD] = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  return cur;
+}
+
+__global__ __launch_bounds__(KXT) void krylov_small_kernel(
+    int n, const double2 *__restrict__ H, const double2 *__restrict__ x, double2 *y,
+    double2 *Q, int max_steps, double2 c, double tol, int hermitian, int *info) {
+  __shared__ SmallLds L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int t = tid; t < KLD * KLD; t += KXT) L.T[t / KLD][t % KLD] = make_double2(0, 0);
+  double loc = 0;
+  for (int i = tid; i < n; i += KXT) loc += x[i].x * x[i].x + x[i].y * x[i].y;
+  const double beta0 = sqrt(block_sum1(loc, L.red));
+  if (beta0 == 0.0) {
+    for (int i = tid; i < n; i += KXT) y[i] = x[i];
+    if (tid == 0) {
+      info[0] = 0;
+      info[1] = 1;
+      info[2] = 1;
+    }
+    return;
+  }
+  const double inv0 = 1.0 / beta0;
+  for (int i = tid; i < n; i += KXT) Q[i] = make_double2(x[i].x * inv0, x[i].y * inv0);
+  const double breakdown = 2.220446049250313e-16 * (double)(n > 1 ? n : 1);
+  const int S = n <= KXT ? min(KXT / n, n) : 1;
+  int m = 0, stopped = 0;
+  for (int j = 0; j < max_steps; ++j) {
+    __syncthreads();
+    const double2 *qj = Q + (int64_t)j * n;
+    double2 *w = Q + (int64_t)(j + 1) * n;
+    for (int i = tid; i < n; i += KXT) L.q[i] = qj[i];
+    __syncthreads();
+    // w = H q_j
+    if (S == 1) {
+      for (int r = tid; r < n; r += KXT) {
+        double2 acc = make_double2(0, 0);
+#pragma unroll 8
+        for (int k = 0; k < n; ++k) {
+          const double2 p = cmul(H[r + (int64_t)k * n], L.q[k]);
+          acc.x += p.x;
+          acc.y += p.y;
+        }
+        w[r] = acc;
+      }
+    } else {
+      if (tid < S * n) {
+        const int r = tid % n, sl = tid / n;
+        double2 acc = make_double2(0, 0);
+#pragma unroll 8
+        for (int k = sl; k < n; k += S) {
+          const double2 p = cmul(H[r + (int64_t)k * n], L.q[k]);
+          acc.x += p.x;
+          acc.y += p.y;
+        }
+        L.part[tid] = acc;
+      }
+      __syncthreads();
+      for (int r = tid; r < n; r += KXT) {
+        double2 acc = make_double2(0, 0);
+        for (int sl = 0; sl < S; ++sl) {
+          acc.x += L.part[sl * n + r].x;
+          acc.y += L.part[sl * n + r].y;
+        }
+        w[r] = acc;
+      }
+    }
+    __syncthreads();
+    loc = 0;
+    for (int i = tid; i < n; i += KXT) loc += w[i].x * w[i].x + w[i].y * w[i].y;
+    const double w0 = sqrt(block_sum1(loc, L.red));
+    for (int pass = 0; pass < 2; ++pass) {
+      // h_l = <q_l, w>, one wave per l
+      for (int l = wave; l <= j; l += KXT / 64) {
+        const double2 *ql = Q + (int64_t)l * n;
+        double2 acc = make_double2(0, 0);
+        for (int k = lane; k < n; k += 64) {
+          const double2 q = ql[k], v = w[k];
+          acc.x += q.x * v.x + q.y * v.y;
+          acc.y += q.x * v.y - q.y * v.x;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          acc.x += __shfl_down(acc.x, off, 64);
+          acc.y += __shfl_down(acc.y, off, 64);
+        }
+        if (lane == 0) L.hc[l] = acc;
+      }
+      __syncthreads();
+      for (int i = tid; i < n; i += KXT) {
+        double2 v = w[i];
+        for (int l = 0; l <= j; ++l) {
+          const double2 p = cmul(L.hc[l], Q[(int64_t)l * n + i]);
+          v.x -= p.x;
+          v.y -= p.y;
+        }
+        w[i] = v;
+      }
+      if (tid <= j) {
+        L.T[tid][j].x += L.hc[tid].x;
+        L.T[tid][j].y += L.hc[tid].y;
+      }
+      __syncthreads();
+    }
+    loc = 0;
+    for (int i = tid; i < n; i += KXT) loc += w[i].x * w[i].x + w[i].y * w[i].y;
+    const double beta = sqrt(block_sum1(loc, L.red));
+    if (tid == 0) L.T[j + 1][j] = make_double2(beta, 0.0);
+    m = j + 1;
+    int stop = beta <= fmax(breakdown, 1e-12 * w0) ? 1 : 0;
+    if (!stop && tol > 0.0) {
+      __syncthreads();
+      double2(*Ex)[KLD] = small_expm(m, L, hermitian, c);
+      const double2 e = Ex[m - 1][0];
+      if (beta0 * beta * sqrt(e.x * e.x + e.y * e.y) < tol) stop = 1;
+    }
+    if (stop) {
+      stopped = 1;
+      break;
+    }
+    const double ib = 1.0 / beta;
+    for (int i = tid; i < n; i += KXT) w[i] = make_double2(w[i].x * ib, w[i].y * ib);
+  }
+  __syncthreads();
+  double2(*Ex)[KLD] = small_expm(m, L, hermitian, c);
+  for (int i = tid; i < n; i += KXT) {
+    double2 acc = make_double2(0, 0);
+    for (int l = 0; l < m; ++l) {
+      const double2 p = cmul(Ex[l][0], Q[(int64_t)l * n + i]);
+      acc.x += p.x;
+      acc.y += p.y;
+    }
+    y[i] = make_double2(beta0 * acc.x, beta0 * acc.y);
+  }
+  if (tid == 0) {
+    info[0] = m;
+    info[1] = 0;
+    info[2] = stopped;
+  }
+}
+}  // namespace
+
+extern "C" int64_t ptn_krylov_small_work_bytes(int64_t n, int32_t max_steps) {
+  return (int64_t)sizeof(double2) * n * (max_steps + 1);
+}
+
+extern "C" int ptn_krylov_expm_small(void *stream, int64_t n, const void *H, const void *x,
+                                     void *y, int32_t max_steps, double time_difference,
+                                     double exp_re, double exp_im, double krylov_tol,
+                                     int32_t hermitian, void *work, void *info) {
+  if (n <= 0 || n > KNMAX || max_steps < 1 || max_steps > KMAX || max_steps > n || !H || !x ||
+      !y || !work || !info) {
+    ptn::set_error("ptn_krylov_expm_small: bad arguments (n=%lld max_steps=%d; n <= %d, "
+                   "max_steps <= min(n, %d))", (long long)n, max_steps, KNMAX, KMAX);
+    return (n > KNMAX || max_steps > KMAX) ? PTN_ERANGE : PTN_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double2 c = make_double2(exp_re * time_difference, exp_im * time_difference);
+  hipLaunchKernelGGL(krylov_small_kernel, dim3(1), dim3(KXT), 0, s, (int)n,
+                     static_cast<const double2 *>(H), static_cast<const double2 *>(x),
+                     static_cast<double2 *>(y), static_cast<double2 *>(work), max_steps, c,
+                     krylov_tol, hermitian, static_cast<int *>(info));
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}

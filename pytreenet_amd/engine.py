@@ -1278,18 +1278,20 @@ def network_live_peak(labels, dims: Dict[Label, int], out_labels) -> int:
     return peak
 
 
-def dense_operator(net: "CompiledNetwork") -> torch.Tensor:
+def dense_operator(net: "CompiledNetwork", transposed: bool = False) -> torch.Tensor:
     """The linear map x -> net(x) of a network whose variable is input 0, as a dense
     complex128 matrix: rows over the output labels, columns over input 0's labels
-    (both flattened row-major), i.e. the network contracted without x."""
+    (both flattened row-major), i.e. the network contracted without x.  With
+    ``transposed`` the (cols, rows) transpose is returned instead (the matrix in
+    column-major order), written directly by the contraction."""
     lab_x = list(net.labels[0])
     x = net.tensors[0]
     tensors = list(net.tensors[1:])
     labels = [list(l) for l in net.labels[1:]]
     conj = list(net.conj[1:]) if net.conj is not None else None
-    out = list(net.out_labels) + lab_x
+    out = lab_x + list(net.out_labels) if transposed else list(net.out_labels) + lab_x
     t = contract_network(tensors, labels, out, conj=conj, precision=net.precision,
                          out_dtype=torch.complex128)
-    rows = math.prod(t.shape[:len(net.out_labels)])
     cols = math.prod(x.shape)
-    return t.reshape(rows, cols)
+    rows = t.numel() // max(cols, 1)
+    return t.reshape(cols, rows) if transposed else t.reshape(rows, cols)

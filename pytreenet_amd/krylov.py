@@ -99,6 +99,10 @@ class TimeEvoMode(Enum):
         if isinstance(forward, bool):
             forward = EvoDirection.from_bool(forward)
         if self.action_evolvable():
+            if psi.numel() <= SMALL_N:
+                y = small_time_evolve(psi, hamiltonian, time_difference, forward, options)
+                if y is not None:
+                    return y
             return self.time_evolve_action(psi, lambda t, y: (hamiltonian @ y.reshape(-1)),
                                            time_difference, forward=forward, **options)
         if not self.is_dense():
@@ -351,6 +355,77 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     combine(Q, coeffs, out, nvec=m)
     STATS.record(m)
     return out.view(shape)
+
+
+# Local spaces up to SMALL_N run the whole recurrence in one launch
+# (ptn_krylov_expm_small) on the dense effective Hamiltonian, for at most
+# SMALL_MAX_STEPS steps.
+SMALL_N = 256
+SMALL_MAX_STEPS = 24
+_PINNED_INFO: Dict[int, torch.Tensor] = {}
+
+
+def krylov_exp_small(psi: torch.Tensor, hamiltonian: torch.Tensor, time_difference: float,
+                     exp_factor: complex, krylov_tol: float = 1e-10,
+                     max_krylov_dim: Optional[int] = None, hermitian: bool = True,
+                     transposed: bool = False) -> Optional[torch.Tensor]:
+    """exp(exp_factor * dt * H) psi for a dense device matrix H (n <= 1024) with the
+    recurrence of krylov_exp_action run inside one kernel.  ``transposed``: the
+    argument holds H^T (H in column-major order, as the kernel reads it).
+    Returns None when the stop rules have not ended the recurrence within
+    SMALL_MAX_STEPS steps and the allowed depth is larger (the caller then runs
+    the general form)."""
+    E.require_device(psi, hamiltonian)
+    n = psi.numel()
+    dev = psi.device
+    H = hamiltonian.to(torch.complex128)
+    if H.shape != (n, n):
+        raise ValueError(f"dense operator {tuple(H.shape)} does not act on a vector of {n}")
+    H = H.contiguous() if transposed else H.t().contiguous()
+    x = _flat(psi.to(torch.complex128))
+    max_steps = n if max_krylov_dim is None else min(n, int(max_krylov_dim))
+    steps = min(max_steps, SMALL_MAX_STEPS)
+    lib = _lib.load()
+    work = torch.empty(n * (steps + 1), dtype=torch.complex128, device=dev)
+    info = torch.empty(3, dtype=torch.int32, device=dev)
+    y = torch.empty(n, dtype=torch.complex128, device=dev)
+    stream = ctypes.c_void_p(E.stream_handle())
+    f = complex(exp_factor)
+    _lib.check(lib.ptn_krylov_expm_small(stream, n, _vp(H), _vp(x), _vp(y), steps,
+                                         float(time_difference), f.real, f.imag,
+                                         float(krylov_tol), 1 if hermitian else 0,
+                                         _vp(work), _vp(info)), "ptn_krylov_expm_small")
+    dev_idx = dev.index or 0
+    host = _PINNED_INFO.get(dev_idx)
+    if host is None:
+        host = torch.empty(3, dtype=torch.int32, pin_memory=True)
+        _PINNED_INFO[dev_idx] = host
+    host.copy_(info, non_blocking=True)
+    done = torch.cuda.Event()
+    done.record()
+    done.synchronize()
+    m, zero, stopped = (int(v) for v in host.tolist())
+    if zero:
+        return psi.clone()
+    if not stopped and steps < max_steps:
+        return None
+    STATS.record(m)
+    return y.view(psi.shape)
+
+
+def small_time_evolve(psi: torch.Tensor, hamiltonian: torch.Tensor, time_difference: float,
+                      forward: "EvoDirection", options: Dict[str, Any],
+                      transposed: bool = False) -> Optional[torch.Tensor]:
+    """TimeEvoMode.KRYLOV's exponential of a dense H through krylov_exp_small, with
+    the solver options resolved as time_evolve_action resolves them (None: the
+    recurrence needs more than SMALL_MAX_STEPS steps)."""
+    override = options.get("exp_factor")
+    exp_factor = (forward.exp_factor() if override is None
+                  else (override if forward is EvoDirection.FORWARD else -override))
+    return krylov_exp_small(psi, hamiltonian, time_difference, exp_factor,
+                            krylov_tol=options.get("krylov_tol", 1e-10),
+                            max_krylov_dim=options.get("max_krylov_dim"),
+                            hermitian=options.get("hermitian", True), transposed=transposed)
 
 
 # ---------------------------------------------------------------------------------------

@@ -33,6 +33,7 @@ from . import engine as E
 from .contractions import (SandwichCache, bond_apply, contract_any,
                            contract_ket_ham_with_envs, expectation_value, heff_network,
                            operator_tensor, scalar_product)
+from . import krylov as _K
 from .krylov import EvoDirection, TimeEvoMode, krylov_exp_action
 from .splitting import (SVDParameters, SplitMode, _colmajor_panel, _cstrides, qr_colmajor,
                         svd_colmajor, truncate_singular_values)
@@ -330,6 +331,13 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         opts = dict(TimeEvoMode.default_solver_options(mode))
         opts.update(self.solver_options)
         if mode is TimeEvoMode.KRYLOV:
+            if x.numel() <= _K.SMALL_N and isinstance(action, E.CompiledNetwork):
+                # short local spaces: the recurrence runs in one launch on the dense
+                # H_eff (one contraction instead of one network replay per step)
+                y = _K.small_time_evolve(x, E.dense_operator(action, transposed=True), dt,
+                                         direction, opts, transposed=True)
+                if y is not None:
+                    return y
             return mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
                                            **opts)
         H = E.dense_operator(action)
