@@ -342,6 +342,9 @@ __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) 
 // a tile row are wave-uniform (scalar loads).  Elements outside the matrix or
 // past the K range are read from a zero page.
 constexpr int DBK = 8, DST = 3;
+#ifndef PTN_DIRECT_OCC
+#define PTN_DIRECT_OCC 2
+#endif
 __device__ double2 g_zero_page[64];
 
 __device__ __forceinline__ void lds_dma16(const double2 *src, double2 *lds_row) {
@@ -449,13 +452,17 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
   };
   if (KT > 0) issue(0, sA0, sB0);
   if (KT > 1) issue(1, sA1, sB1);
-  for (int64_t kt = 0; kt < KT; kt += 3) {
+  // whole ring turns first, with no exit inside the unrolled body (an exit
+  // between stages makes the compiler copy the accumulators at every stage and
+  // wait for the MFMA pipeline to drain), then the 0-2 remaining stages
+  int64_t kt = 0;
+  for (; kt + 3 <= KT; kt += 3) {
     step(kt, sA0, sB0, sA2, sB2);
-    if (kt + 1 >= KT) break;
     step(kt + 1, sA1, sB1, sA0, sB0);
-    if (kt + 2 >= KT) break;
     step(kt + 2, sA2, sB2, sA1, sB1);
   }
+  if (kt < KT) step(kt, sA0, sB0, sA2, sB2);
+  if (kt + 1 < KT) step(kt + 1, sA1, sB1, sA0, sB0);
 
   // ---- epilogue (accumulator row of (lane, r) = 4*r + lane/16) ----------------
   double2 *C = static_cast<double2 *>(p.C);
@@ -490,7 +497,7 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
 }
 
 template <int LARGE>
-__global__ __launch_bounds__(NT, 2) void gemm_direct_f64_kernel(
+__global__ __launch_bounds__(NT, PTN_DIRECT_OCC) void gemm_direct_f64_kernel(
     KArgs p, const int64_t *__restrict__ aok, const int64_t *__restrict__ bok) {
   direct_tile(p, aok, bok, remapped_tile(p), blockIdx.y, gridDim.y > 1);
 }
@@ -513,7 +520,7 @@ __device__ __forceinline__ int find_desc(const int64_t *__restrict__ prefix, int
   return lo;
 }
 
-__global__ __launch_bounds__(NT, 2) void gemm_direct_f64_grouped(const KArgs *__restrict__ descs,
+__global__ __launch_bounds__(NT, PTN_DIRECT_OCC) void gemm_direct_f64_grouped(const KArgs *__restrict__ descs,
                                                                  const int64_t *__restrict__ prefix,
                                                                  int count) {
   const int64_t b = blockIdx.x;
