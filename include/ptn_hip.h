@@ -70,7 +70,8 @@ typedef struct ptn_gemm_desc {
   const int64_t *c_off_m, *c_off_n;
   int32_t c_dtype, beta_one;
   double alpha_re, alpha_im;
-  int32_t compute, _pad2;
+  int32_t compute;
+  int32_t c_mmajor;     /* layout hint: 1 if consecutive m are contiguous in C */
   void *ws;            /* optional scratch for split-K partials (may be NULL) */
   int64_t ws_bytes;
 } ptn_gemm_desc;

@@ -57,7 +57,7 @@ class GemmDesc(ctypes.Structure):
         ("C", ctypes.c_void_p), ("c_off_m", ctypes.c_void_p), ("c_off_n", ctypes.c_void_p),
         ("c_dtype", ctypes.c_int32), ("beta_one", ctypes.c_int32),
         ("alpha_re", ctypes.c_double), ("alpha_im", ctypes.c_double),
-        ("compute", ctypes.c_int32), ("_pad2", ctypes.c_int32),
+        ("compute", ctypes.c_int32), ("c_mmajor", ctypes.c_int32),
         ("ws", ctypes.c_void_p), ("ws_bytes", ctypes.c_int64),
     ]
 

@@ -49,7 +49,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BK = PTN_GEMM_BK, NT = 256;
 constexpr int LA = BM * BK / NT;  // A (and B) tile elements per thread per stage
 static_assert(LA >= 1 && BM * BK % NT == 0, "tile / thread count mismatch");
-constexpr int64_t SPLITK_MIN_K = 512;
+constexpr int64_t SPLITK_MIN_K = 128;
 constexpr int64_t SPLITK_TARGET_BLOCKS = 2048;  // 4 waves of 2 blocks x 256 CUs
 constexpr int64_t LARGE_TILES = 2048;
 
@@ -565,6 +565,269 @@ __global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
   }
 }
 
+// the same sum for few outputs and many splits: one wave per output, lane l
+// adds splits l, l + 64, ... in order, then a fixed shuffle tree
+__global__ __launch_bounds__(256) void splitk_reduce_wide(KArgs p, int ksplit) {
+  const int64_t total = p.M * p.N;
+  const int lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= total) return;
+  double re = 0, im = 0;
+  for (int s = lane; s < ksplit; s += 64) {
+    const double2 v = p.ws[(int64_t)s * total + t];
+    re += v.x;
+    im += v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_down(re, off, 64);
+    im += __shfl_down(im, off, 64);
+  }
+  if (lane != 0) return;
+  const int64_t m = t / p.N, n = t - m * p.N;
+  double ore = p.are * re - p.aim * im;
+  double oim = p.are * im + p.aim * re;
+  double2 *dst = static_cast<double2 *>(p.C) + p.com[m] + p.con[n];
+  if (p.beta_one) {
+    const double2 x = *dst;
+    ore += x.x;
+    oim += x.y;
+  }
+  *dst = make_double2(ore, oim);
+}
+
+// ---- wave-level FP64 GEMM for small and thin products -------------------------
+// Each wave computes one (16 BMB) x (16 BNB) block of C over one K range with
+// the operands loaded straight into MFMA registers: for each 4-wide k slab,
+// lane l supplies A[m0 + 16 i + l%16][k + l/16] and B[k + l/16][n0 + 16 j + l%16].
+// No LDS and no barriers, so a wave's cost is its loads and MFMAs alone; the
+// 64 x 64 workgroup tile would spend 16x the MFMA work on a 16 x 16 product
+// and a full prologue / epilogue on a K of 16.  Waves are numbered tile-major
+// (the splits of one tile are consecutive).  ksplit == 1: the wave writes C;
+// otherwise raw partial sums go to ws[split][M][N] for the reduction kernel.
+// The k-offsets of the next group of WU slabs are fetched while the current
+// group's data loads are in flight.  TR computes the transposed product
+// (MFMA operands swapped, same registers): the accumulators then hold C with
+// lanes along m, which makes the stores contiguous when C is m-contiguous.
+constexpr int WU = 4;
+
+template <int BMB, int BNB, bool TR>
+__global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, int64_t nwaves,
+                                                     int ksplit) {
+  const int lane = threadIdx.x & 63;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (gw >= nwaves) return;
+  const int64_t tile = gw / ksplit, split = gw - tile * ksplit;
+  const int64_t tm = tile / wtiles_n, tn = tile - tm * wtiles_n;
+  const int64_t m0 = tm * 16 * BMB, n0 = tn * 16 * BNB;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const int fr = lane & 15, fk = lane >> 4;
+  const double2 *A = static_cast<const double2 *>(p.A);
+  const double2 *B = static_cast<const double2 *>(p.B);
+  int64_t aoff[BMB], boff[BNB];
+  bool aval[BMB], bval[BNB];
+#pragma unroll
+  for (int i = 0; i < BMB; ++i) {
+    const int64_t gm = m0 + 16 * i + fr;
+    aval[i] = gm < p.M;
+    aoff[i] = aval[i] ? p.aom[gm] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < BNB; ++j) {
+    const int64_t gn = n0 + 16 * j + fr;
+    bval[j] = gn < p.N;
+    boff[j] = bval[j] ? p.bon[gn] : 0;
+  }
+  typedef double v4d_t __attribute__((ext_vector_type(4)));
+  v4d_t c0[BMB][BNB], c1[BMB][BNB];
+#pragma unroll
+  for (int i = 0; i < BMB; ++i)
+#pragma unroll
+    for (int j = 0; j < BNB; ++j) {
+      c0[i][j] = v4d_t{0, 0, 0, 0};
+      c1[i][j] = v4d_t{0, 0, 0, 0};
+    }
+  const double asg = p.aconj ? -1.0 : 1.0, bsg = p.bconj ? -1.0 : 1.0;
+  int64_t ka[WU], kb[WU];
+  auto koffs = [&](int64_t k) {
+#pragma unroll
+    for (int u = 0; u < WU; ++u) {
+      const int64_t gk = k + 4 * u + fk;
+      ka[u] = gk < kend ? p.aok[gk] : -1;
+      kb[u] = gk < kend ? p.bok[gk] : -1;
+    }
+  };
+  if (kbeg < kend) koffs(kbeg);
+  for (int64_t k = kbeg; k < kend; k += 4 * WU) {
+    double ar[WU][BMB], ai[WU][BMB], br[WU][BNB], bi[WU][BNB];
+#pragma unroll
+    for (int u = 0; u < WU; ++u) {
+#pragma unroll
+      for (int i = 0; i < BMB; ++i) {
+        const bool ok = aval[i] && ka[u] >= 0;
+        const double2 v = A[ok ? aoff[i] + ka[u] : 0];
+        ar[u][i] = ok ? v.x : 0.0;
+        ai[u][i] = ok ? asg * v.y : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < BNB; ++j) {
+        const bool ok = bval[j] && kb[u] >= 0;
+        const double2 v = B[ok ? kb[u] + boff[j] : 0];
+        br[u][j] = ok ? v.x : 0.0;
+        bi[u][j] = ok ? bsg * v.y : 0.0;
+      }
+    }
+    if (k + 4 * WU < kend) koffs(k + 4 * WU);
+#pragma unroll
+    for (int u = 0; u < WU; ++u) {
+      if (k + 4 * u >= kend) continue;  // slabs past the range (wave-uniform)
+#pragma unroll
+      for (int i = 0; i < BMB; ++i)
+#pragma unroll
+        for (int j = 0; j < BNB; ++j) {
+          if constexpr (TR) {
+            c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[u][j], ar[u][i], c0[i][j], 0, 0, 0);
+            c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[u][j], ar[u][i], c1[i][j], 0, 0, 0);
+            c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bi[u][j], ai[u][i], c0[i][j], 0, 0, 0);
+            c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[u][j], ai[u][i], c1[i][j], 0, 0, 0);
+          } else {
+            c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[u][i], br[u][j], c0[i][j], 0, 0, 0);
+            c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[u][i], bi[u][j], c1[i][j], 0, 0, 0);
+            c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[u][i], bi[u][j], c0[i][j], 0, 0, 0);
+            c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[u][i], br[u][j], c1[i][j], 0, 0, 0);
+          }
+        }
+    }
+  }
+  // accumulator register r of lane l: row 4 r + l/16, column l%16 of the
+  // block (of C, or of C^T when TR)
+  double2 *C = static_cast<double2 *>(p.C);
+#pragma unroll
+  for (int j = 0; j < BNB; ++j) {
+#pragma unroll
+    for (int i = 0; i < BMB; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t cm = TR ? m0 + 16 * i + fr : m0 + 16 * i + 4 * r + fk;
+        const int64_t cn = TR ? n0 + 16 * j + 4 * r + fk : n0 + 16 * j + fr;
+        if (cm >= p.M || cn >= p.N) continue;
+        const double re = c0[i][j][r], im = c1[i][j][r];
+        if (ksplit > 1) {
+          p.ws[(split * p.M + cm) * p.N + cn] = make_double2(re, im);
+          continue;
+        }
+        double ore = p.are * re - p.aim * im;
+        double oim = p.are * im + p.aim * re;
+        double2 *dst = C + p.com[cm] + p.con[cn];
+        if (p.beta_one) {
+          const double2 x = *dst;
+          ore += x.x;
+          oim += x.y;
+        }
+        *dst = make_double2(ore, oim);
+      }
+    }
+  }
+}
+
+// Shapes the wave kernel takes: both sides within two 16-blocks, or one side
+// within one 16-block (the other then runs in 64-wide blocks).  Returns 0 for
+// shapes that stay with the 64 x 64 workgroup tile.
+int wave_shape(int64_t M, int64_t N, int &bmb, int &bnb) {
+  const int64_t bm = (M + 15) / 16, bn = (N + 15) / 16;
+  if (bm <= 2 && bn <= 2) {
+    bmb = (int)bm;
+    bnb = (int)bn;
+    return 1;
+  }
+  if (bn == 1) {
+    bmb = 4;
+    bnb = 1;
+    return 1;
+  }
+  if (bm == 1) {
+    bmb = 1;
+    bnb = 4;
+    return 1;
+  }
+  return 0;
+}
+
+// launch the wave kernel for a validated descriptor; splits K over waves when
+// the tiles cannot fill the chip (each split >= 64 of K, partial traffic no
+// more than half the operand traffic, partials within the workspace)
+template <bool TR>
+void launch_wave_kernel(hipStream_t s, const KArgs &a, int bmb, int bnb, dim3 g, int64_t tn,
+                        int64_t nwaves, int ks) {
+  const dim3 b(256);
+  if (bmb == 1 && bnb == 1)
+    hipLaunchKernelGGL((wave_gemm_f64<1, 1, TR>), g, b, 0, s, a, tn, nwaves, ks);
+  else if (bmb == 2 && bnb == 1)
+    hipLaunchKernelGGL((wave_gemm_f64<2, 1, TR>), g, b, 0, s, a, tn, nwaves, ks);
+  else if (bmb == 1 && bnb == 2)
+    hipLaunchKernelGGL((wave_gemm_f64<1, 2, TR>), g, b, 0, s, a, tn, nwaves, ks);
+  else if (bmb == 2 && bnb == 2)
+    hipLaunchKernelGGL((wave_gemm_f64<2, 2, TR>), g, b, 0, s, a, tn, nwaves, ks);
+  else if (bmb == 4 && bnb == 1)
+    hipLaunchKernelGGL((wave_gemm_f64<4, 1, TR>), g, b, 0, s, a, tn, nwaves, ks);
+  else
+    hipLaunchKernelGGL((wave_gemm_f64<1, 4, TR>), g, b, 0, s, a, tn, nwaves, ks);
+}
+
+int launch_wave(hipStream_t s, KArgs a, int bmb, int bnb, int64_t ws_bytes, bool c_mmajor) {
+  const int64_t tn = (a.N + 16 * bnb - 1) / (16 * bnb);
+  const int64_t tiles = ((a.M + 16 * bmb - 1) / (16 * bmb)) * tn;
+  int64_t ksplit = 1;
+  if (tiles < 1024 && a.K >= 128 && a.ws && ws_bytes > 0) {
+    ksplit = (4096 + tiles - 1) / tiles;
+    const int64_t by_k = a.K / 64;
+    const int64_t by_traffic = a.K * (a.M + a.N) / (2 * a.M * a.N);
+    const int64_t by_ws = ws_bytes / ((int64_t)sizeof(double2) * a.M * a.N);
+    if (ksplit > by_k) ksplit = by_k;
+    if (ksplit > by_traffic) ksplit = by_traffic;
+    if (ksplit > by_ws) ksplit = by_ws;
+    if (ksplit < 1) ksplit = 1;
+  }
+  const int64_t kc = (a.K + ksplit - 1) / ksplit;
+  a.kchunk = a.K > 0 ? ((kc + 4 * WU - 1) / (4 * WU)) * (4 * WU) : 4 * WU;
+  ksplit = a.K > 0 ? (a.K + a.kchunk - 1) / a.kchunk : 1;
+  if (ksplit > 1) {
+    if (!a.ws) {
+      ptn::set_error("ptn_gemm_gather: split-K needs a workspace");
+      return PTN_EINVAL;
+    }
+  } else {
+    a.ws = nullptr;
+  }
+  const int64_t nwaves = tiles * ksplit;
+  const int64_t blocks = (nwaves + 3) / 4;
+  if (blocks > 0x7fffffff) {
+    ptn::set_error("ptn_gemm_gather: %lld waves exceed one launch", (long long)nwaves);
+    return PTN_ERANGE;
+  }
+  const dim3 g((unsigned)blocks);
+  const int ks = (int)ksplit;
+  // lanes along m for the stores when C is m-contiguous (partials: n-contiguous)
+  if (c_mmajor && ksplit == 1)
+    launch_wave_kernel<true>(s, a, bmb, bnb, g, tn, nwaves, ks);
+  else
+    launch_wave_kernel<false>(s, a, bmb, bnb, g, tn, nwaves, ks);
+  PTN_LAUNCH_CHECK();
+  if (ksplit > 1) {
+    const int64_t total = a.M * a.N;
+    if (total <= 16384 && ksplit >= 16) {
+      hipLaunchKernelGGL(splitk_reduce_wide, dim3((unsigned)((total + 3) / 4)), dim3(256), 0, s,
+                         a, ks);
+    } else {
+      int64_t gr = (total + 255) / 256;
+      if (gr > 4096) gr = 4096;
+      hipLaunchKernelGGL((splitk_reduce<double2>), dim3((unsigned)gr), dim3(256), 0, s, a, ks);
+    }
+    PTN_LAUNCH_CHECK();
+  }
+  return PTN_OK;
+}
+
 template <int MODE, typename TA, typename TB, typename TC>
 int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
   const bool large = ksplit == 1 && tiles >= LARGE_TILES;
@@ -675,14 +938,25 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   const int rc0 = to_kargs(d, a, tiles);
   if (rc0 == 1) return PTN_OK;
   if (rc0) return rc0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int bmb = 0, bnb = 0;
+  if (d->compute == PTN_COMPUTE_F64_4M && wave_shape(d->M, d->N, bmb, bnb)) {
+    a.ws = static_cast<double2 *>(d->ws);
+    return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0);
+  }
   a.remap = (tiles % 8 == 0) ? 1 : 0;
   // split-K when the tile grid cannot fill the chip and K is long: each split
-  // keeps >= SPLITK_MIN_K of K, partials go to the caller's workspace
+  // keeps >= SPLITK_MIN_K of K, the partials (ksplit M N) stay within the
+  // operand traffic (K (M + N), at least two splits' worth) and the caller's
+  // workspace
   int ksplit = 1;
   if (tiles < 256 && d->K >= 2 * SPLITK_MIN_K && d->ws && d->ws_bytes > 0) {
     int64_t want = (SPLITK_TARGET_BLOCKS + tiles - 1) / tiles;
     const int64_t by_k = d->K / SPLITK_MIN_K;
+    int64_t by_traffic = d->K * (d->M + d->N) / (d->M * d->N);
+    if (by_traffic < 2) by_traffic = 2;
     if (want > by_k) want = by_k;
+    if (want > by_traffic) want = by_traffic;
     const int64_t by_ws = d->ws_bytes / ((int64_t)sizeof(double2) * d->M * d->N);
     if (want > by_ws) want = by_ws;
     if (want > 1024) want = 1024;
@@ -694,7 +968,6 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   if (ksplit < 1) ksplit = 1;
   a.ws = ksplit > 1 ? static_cast<double2 *>(d->ws) : nullptr;
   if (d->K == 0) a.kchunk = BK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
   if (d->compute == PTN_COMPUTE_F64_4M)
     return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles, ksplit);
   if (d->a_dtype == PTN_C128)

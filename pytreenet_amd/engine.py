@@ -274,6 +274,7 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     K = math.prod(kdims) if kdims else 1
     ka = _inner_is_unit(kdims, key[4]) and not _inner_is_unit(mdims, key[3])
     kb = _inner_is_unit(kdims, key[5]) and not _inner_is_unit(ndims, key[6])
+    cm, cn = _inner_is_unit(mdims, key[7]), _inner_is_unit(ndims, key[8])
     aff = _AFFINE.get(key)
     if aff is None:
         if len(_AFFINE) > 65536:
@@ -284,7 +285,7 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
         aff = _AFFINE[key] = (False if any(x is None for x in st)
                               else tuple((0, x) for x in st))
     _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate, alpha,
-            aff or None, tabs)
+            aff or None, tabs, (cm and not cn, cn and not cm))
     return out
 
 
@@ -362,7 +363,7 @@ def _run_blas(sd: StridedDesc, M: int, N: int, K: int, precision: str):
 
 
 def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, accumulate,
-            alpha, affine=None, tabs=None):
+            alpha, affine=None, tabs=None, c_major=(False, False)):
     """One contraction launch from device offset tables
     ptrs = (a_off_m, a_off_k, b_off_k, b_off_n, c_off_m, c_off_n).
 
@@ -374,7 +375,8 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     progressions; such a launch with a full tile grid runs as one strided
     rocBLAS GEMM.  Otherwise ptn_gemm_gather runs, with the operand of the
     longer free dimension on the M side (64-row tiles waste less on a short N
-    than on a short M)."""
+    than on a short M).  ``c_major`` tells whether consecutive m (n) are
+    contiguous in C: a layout hint for the store order of small products."""
     if affine is not None:
         hit = _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine)
         if hit is not None:
@@ -389,7 +391,9 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
         kmaj_a, kmaj_b = kmaj_b, kmaj_a
         conj_a, conj_b = conj_b, conj_a
         ptrs = (ptrs[3], ptrs[2], ptrs[1], ptrs[0], ptrs[5], ptrs[4])
+        c_major = (c_major[1], c_major[0])
     d = GemmDesc()
+    d.c_mmajor = int(c_major[0])
     d.M, d.N, d.K = M, N, K
     d.A = A.data_ptr()
     d.a_off_m, d.a_off_k = ptrs[0], ptrs[1]
@@ -640,20 +644,24 @@ def _contract_block(X: _Blk, Y: torch.Tensor, ly: Sequence[Label], conj_y: bool,
             _BLK_CACHE[key] = groups
             if len(_BLK_CACHE) > 4096:
                 _BLK_CACHE.popitem(last=False)
-        _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, target)
+        _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, target,
+                     o_strides)
     if block_out:
         return _Blk(data, res, m_blk, G, {l: X.bdims[l] for l in m_blk})
     return out
 
 
-def _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, target):
+def _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, target,
+                 o_strides):
     ka = _inner_is_unit([dims[l] for l in k_d], [sx[l] for l in k_d]) and \
         not _inner_is_unit([dims[l] for l in m_d], [sx[l] for l in m_d])
     kb = _inner_is_unit([dims[l] for l in k_d], [sy[l] for l in k_d]) and \
         not _inner_is_unit([dims[l] for l in n_l], [sy[l] for l in n_l])
+    cm = _inner_is_unit([dims[l] for l in m_d], [o_strides[l] for l in m_d])
+    cn = _inner_is_unit([dims[l] for l in n_l], [o_strides[l] for l in n_l])
     for ptrs, M, K, N, aff, dtabs in groups:
         _launch(X.data, Y, target, ptrs, M, N, K, ka, kb, X.conj, conj_y, precision, False,
-                1.0, aff, dtabs)
+                1.0, aff, dtabs, (cm and not cn, cn and not cm))
 
 
 _BAPPLY_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()

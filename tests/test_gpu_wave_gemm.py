@@ -1,0 +1,76 @@
+"""Small and thin FP64 contractions (the wave-level MFMA GEMM: M or N within one or
+two 16-blocks) against numpy einsum: every block shape, K split over waves with
+both partial-sum reductions, conjugation, alpha / accumulate and gathered
+(permuted, strided) operands."""
+import numpy as np
+import pytest
+import torch
+
+from pytreenet_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _crand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _dev(a):
+    return torch.as_tensor(a, dtype=torch.complex128).to("cuda")
+
+
+CASES = [
+    # (shapeA, labelsA, shapeB, labelsB, out)          (M, N, K)      block shape, split
+    ((16, 20000), "mk", (16, 20000), "nk", "mn"),      # 16 x 16 x 20000: (1,1), wide reduce
+    ((2, 640), "mk", (640, 2), "kn", "nm"),            # 2 x 2 x 640: (1,1), 10 splits
+    ((20, 3000), "mk", (3000, 30), "kn", "mn"),        # (2,2) split
+    ((30, 500), "mk", (10, 500), "nk", "nm"),          # (2,1) split
+    ((10, 500), "mk", (500, 30), "kn", "mn"),          # (1,2) split
+    ((50, 100, 16), "xyk", (16, 16), "kn", "nyx"),     # 5000 x 16 x 16: (4,1), one split
+    ((16, 80), "mk", (100, 80, 50), "xky", "ymx"),     # 16 x 5000 x 80: (1,4)
+    ((50, 100, 200), "xyk", (200, 16), "kn", "yxn"),   # 5000 x 16 x 200: (4,1), 3 splits
+    ((7, 3, 5), "abk", (5, 4), "kc", "cab"),           # 21 x 4 x 5
+    ((3, 2), "mk", (2, 1), "kn", "mn"),                # tiny
+    # C contiguous along the long side: stores from transposed accumulators
+    ((50, 100, 16), "xyk", (16, 16), "kn", "nxy"),     # 5000 x 16 x 16, m-contiguous C
+    ((16, 80), "mk", (100, 80, 50), "xky", "mxy"),     # operands swapped, then m-contiguous
+    ((20, 30), "mk", (30, 25), "kn", "nm"),            # (2,2), m-contiguous C
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[1]},{c[3]}->{c[4]}:{i}" for i, c in
+                                             enumerate(CASES)])
+@pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
+def test_wave_gemm_matches_einsum(gpu, case, conj):
+    sa, la, sb, lb, lo = case
+    _say(f"wave gemm {sa}{la} x {sb}{lb} -> {lo} conj={conj}")
+    rng = np.random.default_rng(sum(sa) + 3 * sum(sb) + 7 * conj[0] + 11 * conj[1])
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    got = E.contract(_dev(a), la, _dev(b), lb, lo, conj_a=conj[0], conj_b=conj[1]).cpu().numpy()
+    ref = np.einsum(f"{la},{lb}->{lo}", a.conj() if conj[0] else a, b.conj() if conj[1] else b)
+    k = max(1, int(np.prod([d for d, l in zip(sa, la) if l in lb])))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("idx", [0, 3, 5, 7, 10, 11])
+def test_wave_gemm_accumulate_alpha_views(gpu, idx):
+    """Strided operand views and output, C += alpha A B."""
+    sa, la, sb, lb, lo = CASES[idx]
+    _say(f"wave gemm accumulate {la},{lb}->{lo}")
+    rng = np.random.default_rng(100 + idx)
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    ta = _dev(np.concatenate([a, a], axis=-1))[..., : sa[-1]]     # strided view of A
+    tb = _dev(b)
+    ref0 = np.einsum(f"{la},{lb}->{lo}", a, b)
+    big = _crand(rng, ref0.shape[::-1])
+    out = _dev(big).permute(*reversed(range(ref0.ndim)))          # transposed output view
+    base = out.cpu().numpy().copy()
+    E.contract(ta, la, tb, lb, lo, out=out, accumulate=True, alpha=0.25 + 1.5j)
+    ref = base + (0.25 + 1.5j) * ref0
+    k = max(1, int(np.prod([d for d, l in zip(sa, la) if l in lb])))
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=0,
+                               atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
