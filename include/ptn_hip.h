@@ -245,6 +245,20 @@ int ptn_trtri_lower(void *stream, int32_t n, const void *L, void *X);
  * util/tensor_splitting.py:60-107.                                          */
 int ptn_qr(void *stream, int64_t m, int64_t n, void *A, void *R, void *tau);
 
+/* QR(KEEP) split of a small tensor in one launch: T (rank <= 8, dims, strides
+ * in elements, any layout) matricised as (other legs, row-major in leg order)
+ * x (leg `leg`), m x n with m n <= 3072 and n <= 256.  Q: contiguous
+ * row-major tensor of T's dims (the new bond at `leg`; columns >= min(m, n)
+ * zero), R: n x n row-major (rows >= min(m, n) zero), Q R = T, diag(R) real
+ * and non-negative.  Householder reflectors, so rank-deficient panels need no
+ * fallback.  Replaces tensor_qr_decomposition with SplitMode.KEEP
+ * (util/tensor_splitting.py:60-106) as the orthogonality-centre moves
+ * (core/ttn.py:1530) and the one-site link updates
+ * (time_evolution/tdvp_algorithms/onesitetdvp.py:97) call it, for the
+ * small nodes of a tree.                                                    */
+int ptn_qr_split_small(void *stream, int32_t rank, const int64_t *dims, const int64_t *strides,
+                       int32_t leg, const void *T, void *Q, void *R);
+
 /* Thin SVD of a column-major m x n complex128 matrix A (destroyed):
  * S (double, k = min(m,n), descending), U (m x k col-major), Vh (k x n
  * col-major).  Replaces np.linalg.svd / scipy gesvd in tensor_svd,

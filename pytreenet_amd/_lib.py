@@ -24,6 +24,7 @@ EXPORTED = (
     "ptn_gemm_grouped_work_bytes", "ptn_gemm_gather_grouped", "ptn_krylov_batched_work_bytes",
     "ptn_krylov_orthonormalize_batched", "ptn_normalize_batched", "ptn_krylov_combine_batched",
     "ptn_ptr_batch_work_bytes", "ptn_fill_batched", "ptn_permute_batched", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
+    "ptn_qr_split_small",
     "ptn_last_error", "ptn_version",
 )
 
@@ -97,6 +98,7 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_svd": (ctypes.c_int, [vp, i64, i64, vp, vp, vp, vp, vp]),
         "ptn_potrf_lower": (ctypes.c_int, [vp, i32, vp, i64, vp, dbl, dbl, vp]),
         "ptn_trtri_lower": (ctypes.c_int, [vp, i32, vp, vp]),
+        "ptn_qr_split_small": (ctypes.c_int, [vp, i32, vp, vp, i32, vp, vp, vp]),
         "ptn_block_apply": (ctypes.c_int, [vp, i64, i32, i32, i32, i32, vp, i32, vp, vp, vp,
                                            i32, vp, vp, i32, vp, i32, vp, vp, vp]),
         "ptn_gemm_grouped_work_bytes": (i64, [i32]),

@@ -526,3 +526,243 @@ extern "C" int ptn_gemm_strided(void *stream, const ptn_gemm_strided_desc *d) {
   }
   return PTN_OK;
 }
+
+// ---- QR split of a small tensor in one workgroup ------------------------------
+// The m x n panel (rows: the tensor's other legs, row-major in leg order;
+// columns: leg `leg`) is gathered into LDS, factorised by Householder QR
+// (reflectors as LAPACK's zlarfg / zgeqr2 build them: beta real, H^H applied
+// to the trailing columns), R (its upper triangle) is written, then Q is formed
+// in place (zung2r order: H_{k-1} first).  Row r of R and column r of Q are
+// negated where beta_r < 0, so R has a real non-negative diagonal: for a
+// full-rank panel the unique QR of that gauge.  KEEP shape: Q has the input's
+// dims (column c >= k = min(m, n) zero), R is n x n (rows >= k zero).
+namespace {
+constexpr int QST = 256;
+constexpr int QS_MAXE = 3072;
+constexpr int QS_MAXN = 256;
+constexpr int QS_MAXR = 8;
+
+struct QrSmallArgs {
+  int64_t dims[QS_MAXR];
+  int64_t istr[QS_MAXR];  // input strides (elements)
+  int64_t ostr[QS_MAXR];  // Q strides: contiguous row-major over dims
+  int rank, leg;
+  int m, n, k;
+};
+
+__device__ __forceinline__ double2 qcmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 qcjmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ double qs_block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  for (int w = 0; w < QST / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// input and Q offsets of panel row i (the other legs' flattened index)
+__device__ __forceinline__ void qs_row_offsets(const QrSmallArgs &a, int64_t i, int64_t &io,
+                                               int64_t &oo) {
+  io = 0;
+  oo = 0;
+  for (int d = a.rank - 1; d >= 0; --d) {
+    if (d == a.leg) continue;
+    const int64_t idx = i % a.dims[d];
+    i /= a.dims[d];
+    io += idx * a.istr[d];
+    oo += idx * a.ostr[d];
+  }
+}
+
+__global__ __launch_bounds__(QST) void qr_split_small_kernel(QrSmallArgs a, const double2 *T,
+                                                             double2 *Q, double2 *R) {
+  __shared__ double2 P[QS_MAXE];
+  __shared__ double2 sw[QS_MAXN], stau[QS_MAXN];
+  __shared__ double ssgn[QS_MAXN];
+  __shared__ double red[QST / 64];
+  __shared__ double2 sscale;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m = a.m, n = a.n, k = a.k;
+  for (int e = tid; e < m * n; e += QST) {
+    const int i = e % m, j = e / m;
+    int64_t io, oo;
+    qs_row_offsets(a, i, io, oo);
+    P[e] = T[io + (int64_t)j * a.istr[a.leg]];
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    double loc = 0;
+    for (int i = j + 1 + tid; i < m; i += QST) loc += P[i + j * m].x * P[i + j * m].x +
+                                                      P[i + j * m].y * P[i + j * m].y;
+    const double xn2 = qs_block_sum(loc, red);
+    if (tid == 0) {
+      const double2 al = P[j + j * m];
+      if (xn2 == 0.0 && al.y == 0.0) {
+        stau[j] = make_double2(0, 0);
+        sscale = make_double2(0, 0);
+      } else {
+        const double nr = sqrt(al.x * al.x + al.y * al.y + xn2);
+        const double beta = al.x >= 0 ? -nr : nr;
+        stau[j] = make_double2((beta - al.x) / beta, -al.y / beta);
+        // 1 / (alpha - beta)
+        const double dr = al.x - beta, di = al.y, dd = dr * dr + di * di;
+        sscale = make_double2(dr / dd, -di / dd);
+        P[j + j * m] = make_double2(beta, 0.0);
+      }
+    }
+    __syncthreads();
+    const double2 tau = stau[j];
+    if (tau.x == 0.0 && tau.y == 0.0) continue;  // H_j = I (uniform)
+    const double2 sc = sscale;
+    for (int i = j + 1 + tid; i < m; i += QST) P[i + j * m] = qcmul(sc, P[i + j * m]);
+    __syncthreads();
+    // w_c = v^H P[j:m, c], v_j = 1
+    for (int c = j + 1 + wave; c < n; c += QST / 64) {
+      double2 acc = make_double2(0, 0);
+      for (int i = j + lane; i < m; i += 64) {
+        const double2 v = i == j ? make_double2(1, 0) : P[i + j * m];
+        const double2 p = qcjmul(v, P[i + c * m]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        acc.x += __shfl_down(acc.x, off, 64);
+        acc.y += __shfl_down(acc.y, off, 64);
+      }
+      if (lane == 0) sw[c] = acc;
+    }
+    __syncthreads();
+    // P[j:m, c] -= conj(tau) v w_c
+    const double2 ctau = make_double2(tau.x, -tau.y);
+    const int rows = m - j, cols = n - j - 1;
+    for (int e = tid; e < rows * cols; e += QST) {
+      const int i = j + e % rows, c = j + 1 + e / rows;
+      const double2 v = i == j ? make_double2(1, 0) : P[i + j * m];
+      const double2 t = qcmul(ctau, qcmul(v, sw[c]));
+      P[i + c * m].x -= t.x;
+      P[i + c * m].y -= t.y;
+    }
+    __syncthreads();
+  }
+  // gauge: real non-negative diagonal of R
+  for (int r = tid; r < k; r += QST) ssgn[r] = P[r + r * m].x < 0.0 ? -1.0 : 1.0;
+  __syncthreads();
+  for (int e = tid; e < n * n; e += QST) {
+    const int r = e / n, c = e % n;
+    double2 v = make_double2(0, 0);
+    if (r < k && c >= r) {
+      v = P[r + c * m];
+      v.x *= ssgn[r];
+      v.y *= ssgn[r];
+    }
+    R[e] = v;
+  }
+  __syncthreads();
+  // Q = H_0 ... H_{k-1} [I; 0], formed in place from the last reflector back
+  for (int j = k - 1; j >= 0; --j) {
+    const double2 tau = stau[j];
+    if (j < k - 1) {
+      for (int c = j + 1 + wave; c < k; c += QST / 64) {
+        double2 acc = make_double2(0, 0);
+        for (int i = j + lane; i < m; i += 64) {
+          const double2 v = i == j ? make_double2(1, 0) : P[i + j * m];
+          const double2 p = qcjmul(v, P[i + c * m]);
+          acc.x += p.x;
+          acc.y += p.y;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          acc.x += __shfl_down(acc.x, off, 64);
+          acc.y += __shfl_down(acc.y, off, 64);
+        }
+        if (lane == 0) sw[c] = acc;
+      }
+      __syncthreads();
+      const int rows = m - j, cols = k - j - 1;
+      for (int e = tid; e < rows * cols; e += QST) {
+        const int i = j + e % rows, c = j + 1 + e / rows;
+        const double2 v = i == j ? make_double2(1, 0) : P[i + j * m];
+        const double2 t = qcmul(tau, qcmul(v, sw[c]));
+        P[i + c * m].x -= t.x;
+        P[i + c * m].y -= t.y;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < m; i += QST) {
+      double2 v;
+      if (i < j)
+        v = make_double2(0, 0);
+      else if (i == j)
+        v = make_double2(1.0 - tau.x, -tau.y);
+      else {
+        const double2 t = qcmul(tau, P[i + j * m]);
+        v = make_double2(-t.x, -t.y);
+      }
+      P[i + j * m] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * n; e += QST) {
+    const int i = e % m, c = e / m;
+    int64_t io, oo;
+    qs_row_offsets(a, i, io, oo);
+    double2 v = make_double2(0, 0);
+    if (c < k) {
+      v = P[i + c * m];
+      v.x *= ssgn[c];
+      v.y *= ssgn[c];
+    }
+    Q[oo + (int64_t)c * a.ostr[a.leg]] = v;
+  }
+}
+}  // namespace
+
+extern "C" int ptn_qr_split_small(void *stream, int32_t rank, const int64_t *dims,
+                                  const int64_t *strides, int32_t leg, const void *T, void *Q,
+                                  void *R) {
+  if (rank < 1 || rank > QS_MAXR || leg < 0 || leg >= rank || !dims || !strides || !T || !Q ||
+      !R) {
+    ptn::set_error("ptn_qr_split_small: bad arguments (rank=%d leg=%d)", rank, leg);
+    return PTN_EINVAL;
+  }
+  QrSmallArgs a;
+  int64_t total = 1;
+  for (int d = 0; d < rank; ++d) {
+    if (dims[d] < 1) {
+      ptn::set_error("ptn_qr_split_small: dimension %d is %lld", d, (long long)dims[d]);
+      return PTN_EINVAL;
+    }
+    a.dims[d] = dims[d];
+    a.istr[d] = strides[d];
+    total *= dims[d];
+  }
+  int64_t acc = 1;
+  for (int d = rank - 1; d >= 0; --d) {
+    a.ostr[d] = acc;
+    acc *= dims[d];
+  }
+  const int64_t n = dims[leg], m = total / n;
+  if (total > QS_MAXE || n > QS_MAXN) {
+    ptn::set_error("ptn_qr_split_small: panel %lld x %lld beyond %d elements / %d columns",
+                   (long long)m, (long long)n, QS_MAXE, QS_MAXN);
+    return PTN_ERANGE;
+  }
+  a.rank = rank;
+  a.leg = leg;
+  a.m = (int)m;
+  a.n = (int)n;
+  a.k = (int)(m < n ? m : n);
+  hipLaunchKernelGGL(qr_split_small_kernel, dim3(1), dim3(QST), 0,
+                     static_cast<hipStream_t>(stream), a, static_cast<const double2 *>(T),
+                     static_cast<double2 *>(Q), static_cast<double2 *>(R));
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}

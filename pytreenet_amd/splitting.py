@@ -228,20 +228,43 @@ def cholesky_qr3_leg(t: torch.Tensor, leg: int):
     return q, R, ok
 
 
-QR_STATS = {"cholqr3": 0, "householder": 0}
+QR_STATS = {"cholqr3": 0, "householder": 0, "small": 0}
+# tensors up to this many elements are split by the one-launch Householder kernel
+QR_SMALL_MAX = 3072
+
+
+def qr_split_small(t: torch.Tensor, leg: int) -> Tuple[torch.Tensor, torch.Tensor]:
+    """QR(KEEP) of ``t`` matricised as (other legs) x (leg ``leg``) in one launch
+    (``ptn_qr_split_small``): Q with t's shape (new bond at ``leg``), R (new, old);
+    diag(R) real and non-negative."""
+    E.require_device(t)
+    t = t if t.dtype == torch.complex128 else t.to(torch.complex128)
+    n = t.shape[leg]
+    q = torch.empty(t.shape, dtype=torch.complex128, device=t.device)
+    r = torch.empty((n, n), dtype=torch.complex128, device=t.device)
+    arr = ctypes.c_int64 * t.dim()
+    _lib.check(_lib.load().ptn_qr_split_small(
+        ctypes.c_void_p(E.stream_handle()), t.dim(), arr(*t.shape), arr(*t.stride()), leg,
+        ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(q.data_ptr()),
+        ctypes.c_void_p(r.data_ptr())), "ptn_qr_split_small")
+    return q, r
 
 
 def qr_split_to_neighbour(ttn, node_id: str, neighbour_id: str):
     """QR(KEEP) of a node off its leg to ``neighbour_id``.
 
     Returns (Q in the node's canonical layout with the new bond at the old
-    leg's position, R with legs (new, old)).  Panels with rows >= columns use
-    shifted CholeskyQR3; wide or rank-deficient panels use Householder QR.
+    leg's position, R with legs (new, old)).  Small tensors run the one-launch
+    Householder kernel; larger panels with rows >= columns use shifted
+    CholeskyQR3, wide or rank-deficient ones Householder QR.
     """
     t = ttn.tensors[node_id]
     leg = ttn.nodes[node_id].neighbour_index(neighbour_id)
     n = t.shape[leg]
     m = t.numel() // n
+    if t.numel() <= QR_SMALL_MAX and n <= 256 and t.dim() <= 8:
+        QR_STATS["small"] += 1
+        return qr_split_small(t, leg)
     if m >= n and n <= 1024:
         q, r, ok = cholesky_qr3_leg(t, leg)
         if ok:

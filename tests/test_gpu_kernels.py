@@ -396,3 +396,55 @@ def test_svd_qr_preconditioned(gpu, m, n, rank):
                                                                           else "qr_wide"] + 1
     else:
         assert S.SVD_STATS["qr_fallback"] == before["qr_fallback"] + 1
+
+
+QR_SMALL_CASES = [((2, 2), 0), ((2, 2), 1), ((16, 2, 2, 2, 2), 0), ((16, 2, 2, 2, 2), 1),
+                  ((3, 8), 1), ((48, 64), 1), ((12, 256), 1), ((5, 7, 3), 1)]
+
+
+@pytest.mark.parametrize("shape,leg", QR_SMALL_CASES)
+def test_qr_split_small(gpu, shape, leg):
+    """One-launch Householder QR(KEEP): numpy's QR in the gauge diag(R) >= 0, the
+    KEEP zero padding, and the same factors as CholeskyQR3 on full-rank tall panels."""
+    _say(f"qr split small {shape} leg={leg}")
+    rng = np.random.default_rng(sum(shape) + leg)
+    t = _crand(rng, shape)
+    q, r = S.qr_split_small(_dev(t), leg)
+    q, r = q.cpu().numpy(), r.cpu().numpy()
+    n = shape[leg]
+    others = [i for i in range(len(shape)) if i != leg]
+    A = np.transpose(t, others + [leg]).reshape(-1, n)
+    m = A.shape[0]
+    k = min(m, n)
+    Qm = np.transpose(q, others + [leg]).reshape(-1, n)
+    qn, rn = np.linalg.qr(A)
+    d = np.diag(rn) / np.abs(np.diag(rn))
+    np.testing.assert_allclose(Qm[:, :k], qn * d[None, :], atol=1e-12)
+    np.testing.assert_allclose(r[:k], rn * d.conj()[:, None], atol=1e-12 * np.abs(A).max())
+    assert np.all(Qm[:, k:] == 0) and np.all(r[k:] == 0)
+    assert np.all(np.abs(np.tril(r, -1)) == 0)
+    np.testing.assert_allclose(Qm @ r, A, atol=1e-12 * np.abs(A).max())
+    if m >= n:
+        qc, rc, ok = S.cholesky_qr3_leg(_dev(t), leg)
+        assert ok
+        np.testing.assert_allclose(q, qc.cpu().numpy(), atol=1e-11)
+        np.testing.assert_allclose(r, rc.cpu().numpy(), atol=1e-11 * np.abs(A).max())
+
+
+def test_qr_split_small_rank_deficient_and_views(gpu):
+    _say("qr split small rank-deficient / strided view")
+    rng = np.random.default_rng(77)
+    u, v = _crand(rng, (12, 1)), _crand(rng, (1, 6))
+    t = (u @ v).reshape(3, 4, 6)                       # rank 1 over (3*4) x 6
+    q, r = S.qr_split_small(_dev(t), 2)
+    Qm, R = q.cpu().numpy().reshape(12, 6), r.cpu().numpy()
+    np.testing.assert_allclose(Qm @ R, t.reshape(12, 6), atol=1e-12 * np.abs(t).max())
+    # Householder Q stays orthonormal on a rank-deficient panel
+    np.testing.assert_allclose(Qm.conj().T @ Qm, np.eye(6), atol=1e-12)
+    assert np.all(np.diag(R).real >= 0) and np.all(np.diag(R).imag == 0)
+    base = _crand(rng, (6, 5, 4))
+    view = _dev(base).permute(2, 0, 1)                 # (4, 6, 5), non-contiguous
+    q, r = S.qr_split_small(view, 0)
+    tv = base.transpose(2, 0, 1)
+    rec = np.einsum("kab,kj->jab", q.cpu().numpy(), r.cpu().numpy())
+    np.testing.assert_allclose(rec, tv, atol=1e-12 * np.abs(tv).max())
