@@ -131,8 +131,8 @@ def env_block(ket_node: Node, ket: torch.Tensor, op_node: Node, op: torch.Tensor
         tensors.append(envs[i])
         labels.append(_env_labels(i))
         conj.append(False)
-    return E.contract_network(tensors, labels, _env_labels(to), conj=conj, precision=precision,
-                              out_dtype=E.storage_dtype(precision))
+    return E.contract_network_cached(tensors, labels, _env_labels(to), conj=conj,
+                                     precision=precision, out_dtype=E.storage_dtype(precision))
 
 
 def heff_network(ket_node: Node, op_node: Node, op: torch.Tensor,

@@ -173,6 +173,9 @@ def clear_caches():
     _CACHE.d.clear()
     _PLANS.clear()
     _AFFINE.clear()
+    _NET_CACHE.clear()
+    _NET_CACHE_BYTES[0] = 0
+    _NET_TOO_BIG.clear()
 
 
 _SPLITK_WS: Dict[torch.device, torch.Tensor] = {}
@@ -1286,6 +1289,65 @@ def network_live_peak(labels, dims: Dict[Label, int], out_labels) -> int:
     return peak
 
 
+# Small networks that recur with the same structure and input layouts (the
+# environment updates and dense effective operators of the small nodes of a
+# tree, every sweep) are recorded once and replayed: the host then issues the
+# recorded launches without planning or offset-table lookups.  A recording
+# keeps its intermediates, so only networks whose live intermediates stay
+# within NET_CACHE_MAX_BYTES are kept, up to NET_CACHE_BUDGET bytes in all.
+NET_CACHE_MAX_BYTES = 64 << 20
+NET_CACHE_BUDGET = 2 << 30
+_NET_CACHE: "OrderedDict[tuple, Tuple[CompiledNetwork, int]]" = OrderedDict()
+_NET_CACHE_BYTES = [0]
+_NET_TOO_BIG: set = set()
+
+
+def contract_network_cached(tensors: Sequence, labels: Sequence[Sequence[Label]],
+                            out_labels: Sequence[Label], *, conj: Optional[Sequence[bool]] = None,
+                            precision: str = FP64,
+                            out_dtype: torch.dtype = torch.complex128) -> torch.Tensor:
+    """``contract_network`` through a recorded replay when the network is small.
+
+    Dense inputs are the replay's variables (matched by shape, stride and dtype);
+    a block-sparse operator tensor is part of the key, held by the recording."""
+    var = [i for i, t in enumerate(tensors) if not isinstance(t, BlockSparse)]
+    if not var or len(tensors) < 2:
+        return contract_network(tensors, labels, out_labels, conj=conj, precision=precision,
+                                out_dtype=out_dtype)
+    key = (tuple(tuple(l) for l in labels), tuple(out_labels),
+           tuple(conj) if conj is not None else None, precision, out_dtype,
+           tuple(("blk", id(t)) if isinstance(t, BlockSparse) else
+                 (tuple(t.shape), tuple(t.stride()), t.dtype, t.device) for t in tensors))
+    hit = _NET_CACHE.get(key)
+    if hit is None:
+        if key in _NET_TOO_BIG:
+            return contract_network(tensors, labels, out_labels, conj=conj,
+                                    precision=precision, out_dtype=out_dtype)
+        dims: Dict[Label, int] = {}
+        for t, lab in zip(tensors, labels):
+            for l, d in zip(lab, t.shape):
+                dims[l] = d
+        esize = torch.empty((), dtype=storage_dtype(precision)).element_size()
+        nbytes = esize * network_live_peak(labels, dims, out_labels)
+        if nbytes > NET_CACHE_MAX_BYTES:
+            if len(_NET_TOO_BIG) > 65536:
+                _NET_TOO_BIG.clear()
+            _NET_TOO_BIG.add(key)
+            return contract_network(tensors, labels, out_labels, conj=conj,
+                                    precision=precision, out_dtype=out_dtype)
+        net = CompiledNetwork(tensors, labels, out_labels, conj=conj, precision=precision,
+                              out_dtype=out_dtype, variable=var)
+        _NET_CACHE[key] = (net, nbytes)
+        _NET_CACHE_BYTES[0] += nbytes
+        while _NET_CACHE_BYTES[0] > NET_CACHE_BUDGET and len(_NET_CACHE) > 1:
+            _, (_, nb) = _NET_CACHE.popitem(last=False)
+            _NET_CACHE_BYTES[0] -= nb
+    else:
+        net = hit[0]
+        _NET_CACHE.move_to_end(key)
+    return net(*[tensors[i] for i in var])
+
+
 def dense_operator(net: "CompiledNetwork", transposed: bool = False) -> torch.Tensor:
     """The linear map x -> net(x) of a network whose variable is input 0, as a dense
     complex128 matrix: rows over the output labels, columns over input 0's labels
@@ -1298,7 +1360,7 @@ def dense_operator(net: "CompiledNetwork", transposed: bool = False) -> torch.Te
     labels = [list(l) for l in net.labels[1:]]
     conj = list(net.conj[1:]) if net.conj is not None else None
     out = lab_x + list(net.out_labels) if transposed else list(net.out_labels) + lab_x
-    t = contract_network(tensors, labels, out, conj=conj, precision=net.precision,
+    t = contract_network_cached(tensors, labels, out, conj=conj, precision=net.precision,
                          out_dtype=torch.complex128)
     cols = math.prod(x.shape)
     rows = t.numel() // max(cols, 1)

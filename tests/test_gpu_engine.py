@@ -55,3 +55,35 @@ def test_recorded_tables_survive_cache_eviction(gpu):
         assert np.abs(first).max() > 0
     finally:
         E._CACHE.cap = cap
+
+
+@pytest.mark.parametrize("blocked", [False, True])
+def test_cached_network_replay_matches_direct(gpu, blocked):
+    """contract_network_cached replays a recorded environment-update network: every
+    call, with new input values (the ket used twice, once conjugated), equals a
+    fresh contract_network; a new input layout records a new network."""
+    print(f"cached network replay blocked={blocked}", flush=True)
+    rng = np.random.default_rng(5 + blocked)
+    c = lambda *s: torch.as_tensor(rng.standard_normal(s) + 1j * rng.standard_normal(s)).cuda()
+    w = np.zeros((3, 4, 2, 2), dtype=complex)           # (w_parent, w_child, out, in)
+    w[0, 0] = np.eye(2)
+    w[1, 2] = [[0, 1], [1, 0]]
+    w[2, 3] = [[1, 0], [0, -1]]
+    W = torch.as_tensor(w).cuda()
+    op = E.BlockSparse.from_dense(W, [0, 1]) if blocked else W
+    labels = [["a0", "a1", "p"], ["w0", "w1", "q", "p"], ["b0", "b1", "q"], ["a1", "w1", "b1"]]
+    out_l = ["a0", "w0", "b0"]
+    conj = [False, False, True, False]
+    E.clear_caches()
+    for it in range(3):
+        ket, env = c(6, 5, 2), c(5, 4, 5)
+        got = E.contract_network_cached([ket, op, ket, env], labels, out_l, conj=conj)
+        ref = E.contract_network([ket, op, ket, env], labels, out_l, conj=conj)
+        assert torch.allclose(got, ref, rtol=0, atol=1e-13 * ref.abs().max().item()), it
+    assert len(E._NET_CACHE) == 1
+    ket2 = c(5, 6, 2).permute(1, 0, 2)                  # same shape, other strides
+    env = c(5, 4, 5)
+    got = E.contract_network_cached([ket2, op, ket2, env], labels, out_l, conj=conj)
+    ref = E.contract_network([ket2, op, ket2, env], labels, out_l, conj=conj)
+    assert torch.allclose(got, ref, rtol=0, atol=1e-13 * ref.abs().max().item())
+    assert len(E._NET_CACHE) == 2
