@@ -31,12 +31,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
-# measured HBM bytes per LARGE FP64 launch of each backend (rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE on this command, scaled by the copy calibration; the
-# summaries are under profiles/); None where not yet measured
-TRAFFIC_BYTES_PER_LAUNCH = {"gather": 3.80e8, "blas": None}
-KERNEL_NAME = {"gather": "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, F64_4M)",
-               "blas": "rocblas_zgemm (ptn_gemm_strided)"}
+# measured HBM bytes per LARGE launch of each (backend, precision) (rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE on this command, scaled by the copy calibration;
+# the summaries are under profiles/); None where not measured
+TRAFFIC_BYTES_PER_LAUNCH = {("gather", "fp64"): 2.54e9, ("blas", "fp64"): None,
+                            ("gather", "fp32"): None, ("blas", "fp32"): None}
+KERNEL_NAME = {("gather", "fp64"): "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, FP64 3M)",
+               ("blas", "fp64"): "rocblas_zgemm (ptn_gemm_strided)",
+               ("gather", "fp32"): "gemm_gather_kernel<F32_3M, ..., LARGE=1> (ptn_gemm_gather)",
+               ("blas", "fp32"): "rocblas_cgemm (ptn_gemm_strided)"}
+# real flops executed per complex multiply-add (8 = four-product form)
+EXECUTED_PER_CMAC = {("gather", "fp64"): 6.0, ("blas", "fp64"): 8.0,
+                     ("gather", "fp32"): 6.0, ("blas", "fp32"): 8.0}
 METRIC = "TDVP sweeps/s & H_eff TFLOP/s at χ=256, mixed vs FP64 GPU at ΔE≤1e-8"
 
 
@@ -90,7 +96,7 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool):
     return {"elapsed": elapsed, "energy": energy, "alg": alg,
             "applies": alg.apply_log[nlog:],
             "gemm_flops": fl1 - fl0, "launches": n1 - n0,
-            "kernels": E.PROFILER.summary(phase=f"{precision}-timed", compute="fp64")
+            "kernels": E.PROFILER.summary(phase=f"{precision}-timed", compute=precision)
             if profile else []}
 
 
@@ -233,10 +239,9 @@ def main():
            "fp64_gemm_tflops": r64["gemm_flops"] / r64["elapsed"] / 1e12,
            "fp64_launches_per_step": r64["launches"] / args.steps,
            "fp64_applies_per_step": sum(m for _, _, m in r64["applies"]) / args.steps}
-    kern = r64["kernels"]
     mixed_ok = False
     if not args.no_mixed:
-        r32 = run_tdvp(st, H, "fp32", args.warmup, args.steps, profile=False)
+        r32 = run_tdvp(st, H, "fp32", args.warmup, args.steps, profile=True)
         sps32 = args.steps / r32["elapsed"]
         de = abs(r32["energy"] - r64["energy"])
         mixed_ok = de <= 1e-8
@@ -244,12 +249,18 @@ def main():
                     "mixed_abs_energy_error": de, "mixed_speedup": sps32 / sps64,
                     "mixed_gemm_tflops": r32["gemm_flops"] / r32["elapsed"] / 1e12})
     value = res["mixed_sweeps_per_s"] if mixed_ok else sps64
+    # the roofline follows the path whose rate is `value`
+    prec = "fp32" if mixed_ok else "fp64"
+    run = r32 if mixed_ok else r64
+    kern = run["kernels"]
     roofline = None
     if kern:
-        # dominant kernel: of the LARGE FP64 launches (tile grid >= 2048) in the timed
-        # steps, the backend with the larger total time -- rocBLAS strided GEMM
+        # dominant kernel: of the LARGE launches (gemm_kernel_class "large") in the
+        # timed steps, the backend with the larger total time -- rocBLAS strided GEMM
         # ("blas") or the hand-written gather-GEMM's LARGE instantiation ("gather");
-        # the other one is reported beside it
+        # the other one is reported beside it.  `achieved` counts 8 M N K flops per
+        # launch (a complex multiply-add as 8 real flops); `executed_*` count what
+        # the MFMAs run (6 M N K for the three-product form)
         by = {}
         for k in kern:
             by.setdefault(k["backend"], []).append(k)
@@ -262,14 +273,16 @@ def main():
             fl = sum(k["flops_per_launch"] * k["launches"] for k in ks)
             ms = sum(k["total_ms"] for k in ks)
             ach = fl / (ms * 1e-3)
-            return {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12,
-                    "unit": "TFLOP/s", "frac": ach / PEAK["fp64"],
+            ex = ach * EXECUTED_PER_CMAC[(b, prec)] / 8.0
+            return {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[prec] / 1e12,
+                    "unit": "TFLOP/s", "frac": ach / PEAK[prec],
+                    "executed_tflops": ex / 1e12, "executed_frac": ex / PEAK[prec],
                     # HBM bytes per launch: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
                     # (scales from --calibrate-copy), averaged over the launches
-                    "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(b),
-                    "kernel": KERNEL_NAME[b], "flops_per_launch": fl / n, "avg_ms": ms / n,
-                    "launches": n, "timed_steps": args.steps,
-                    "share_of_step": ms * 1e-3 / r64["elapsed"],
+                    "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((b, prec)),
+                    "kernel": KERNEL_NAME[(b, prec)], "flops_per_launch": fl / n,
+                    "avg_ms": ms / n, "launches": n, "timed_steps": args.steps,
+                    "share_of_step": ms * 1e-3 / run["elapsed"],
                     "shapes": [{"M": k["M"], "N": k["N"], "K": k["K"],
                                 "launches": k["launches"], "avg_ms": round(k["avg_ms"], 3),
                                 "tflops": round(k["flops_per_launch"] / k["avg_ms"] / 1e9, 2)}
@@ -277,14 +290,14 @@ def main():
         roofline = block(order[0])
         if len(order) > 1:
             roofline["other"] = block(order[1])
-        allk = E.PROFILER.summary(compute="fp64", backend=order[0])
+        allk = E.PROFILER.summary(compute=prec, backend=order[0])
         n_all = sum(k["launches"] for k in allk)
         roofline["all_phases"] = {"launches": n_all,
                                   "avg_ms": sum(k["total_ms"] for k in allk) / n_all,
-                                  "note": "every LARGE FP64 launch of this backend in this "
-                                          "process (setup, warmup, timed, evaluation, QR of "
-                                          "the mixed run): the set rocprofv3 --stats "
-                                          "averages over"}
+                                  "note": f"every LARGE {prec} launch of this backend in "
+                                          "this process (setup, warmup, timed, evaluation, "
+                                          "and the other run's launches of this precision): "
+                                          "the set rocprofv3 --stats averages over"}
     E.PROFILER.stop()
     cpu = None
     if not args.no_cpu:

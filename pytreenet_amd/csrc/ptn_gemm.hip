@@ -342,6 +342,9 @@ __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) 
 // a tile row are wave-uniform (scalar loads).  Elements outside the matrix or
 // past the K range are read from a zero page.
 constexpr int DBK = 8, DST = 3;
+#ifndef PTN_F64_3M
+#define PTN_F64_3M 1
+#endif
 #ifndef PTN_DIRECT_OCC
 #define PTN_DIRECT_OCC 2
 #endif
@@ -392,13 +395,23 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
   };
 
   typedef double v4d_t __attribute__((ext_vector_type(4)));
+  // PTN_F64_3M: Gauss's three-product form, T1 = sum ar br, T2 = sum ai bi,
+  // T3 = sum (ar + ai)(br + bi); re = T1 - T2, im = T3 - T1 - T2 (3 MFMAs per
+  // block product instead of 4; normwise error of the same order as the
+  // four-product form).  Otherwise re and im accumulate directly (4 MFMAs).
   v4d_t c0[2][2], c1[2][2];
+#if PTN_F64_3M
+  v4d_t c2[2][2];
+#endif
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       c0[i][j] = v4d_t{0, 0, 0, 0};
       c1[i][j] = v4d_t{0, 0, 0, 0};
+#if PTN_F64_3M
+      c2[i][j] = v4d_t{0, 0, 0, 0};
+#endif
     }
   const double asg = p.aconj ? -1.0 : 1.0, bsg = p.bconj ? -1.0 : 1.0;
   const int fr = lane & 15, fk = lane >> 4;
@@ -417,6 +430,22 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
         br[i] = b.x;
         bi[i] = bsg * b.y;
       }
+#if PTN_F64_3M
+      double as[2], bs[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        as[i] = ar[i] + ai[i];
+        bs[i] = br[i] + bi[i];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], c0[i][j], 0, 0, 0);
+          c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], c1[i][j], 0, 0, 0);
+          c2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], c2[i][j], 0, 0, 0);
+        }
+#else
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -426,6 +455,7 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
           c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], c0[i][j], 0, 0, 0);
           c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], c1[i][j], 0, 0, 0);
         }
+#endif
     }
   };
 
@@ -477,7 +507,12 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
       for (int r = 0; r < 4; ++r) {
         const int64_t cm = m0 + wm * 32 + i * 16 + r * 4 + fk;
         if (cm >= p.M) continue;
+#if PTN_F64_3M
+        const double t1 = c0[i][j][r], t2 = c1[i][j][r];
+        const double re = t1 - t2, im = c2[i][j][r] - t1 - t2;
+#else
         const double re = c0[i][j][r], im = c1[i][j][r];
+#endif
         if (splitk) {
           p.ws[(split * p.M + cm) * p.N + cn] = make_double2(re, im);
           continue;
@@ -567,6 +602,7 @@ __global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
 
 // the same sum for few outputs and many splits: one wave per output, lane l
 // adds splits l, l + 64, ... in order, then a fixed shuffle tree
+template <typename TC>
 __global__ __launch_bounds__(256) void splitk_reduce_wide(KArgs p, int ksplit) {
   const int64_t total = p.M * p.N;
   const int lane = threadIdx.x & 63;
@@ -586,13 +622,14 @@ __global__ __launch_bounds__(256) void splitk_reduce_wide(KArgs p, int ksplit) {
   const int64_t m = t / p.N, n = t - m * p.N;
   double ore = p.are * re - p.aim * im;
   double oim = p.are * im + p.aim * re;
-  double2 *dst = static_cast<double2 *>(p.C) + p.com[m] + p.con[n];
+  TC *dst = static_cast<TC *>(p.C) + p.com[m] + p.con[n];
   if (p.beta_one) {
-    const double2 x = *dst;
-    ore += x.x;
-    oim += x.y;
+    double xr, xi;
+    ptn::cload(dst, xr, xi);
+    ore += xr;
+    oim += xi;
   }
-  *dst = make_double2(ore, oim);
+  ptn::cstore(dst, ore, oim);
 }
 
 // ---- wave-level FP64 GEMM for small and thin products -------------------------
@@ -610,7 +647,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_wide(KArgs p, int ksplit) {
 // lanes along m, which makes the stores contiguous when C is m-contiguous.
 constexpr int WU = 4;
 
-template <int BMB, int BNB, bool TR>
+template <int BMB, int BNB, bool TR, typename TA, typename TB, typename TC>
 __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, int64_t nwaves,
                                                      int ksplit) {
   const int lane = threadIdx.x & 63;
@@ -622,8 +659,8 @@ __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, 
   const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
   const int fr = lane & 15, fk = lane >> 4;
-  const double2 *A = static_cast<const double2 *>(p.A);
-  const double2 *B = static_cast<const double2 *>(p.B);
+  const TA *A = static_cast<const TA *>(p.A);
+  const TB *B = static_cast<const TB *>(p.B);
   int64_t aoff[BMB], boff[BNB];
   bool aval[BMB], bval[BNB];
 #pragma unroll
@@ -665,16 +702,18 @@ __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, 
 #pragma unroll
       for (int i = 0; i < BMB; ++i) {
         const bool ok = aval[i] && ka[u] >= 0;
-        const double2 v = A[ok ? aoff[i] + ka[u] : 0];
-        ar[u][i] = ok ? v.x : 0.0;
-        ai[u][i] = ok ? asg * v.y : 0.0;
+        double vr, vi;
+        ptn::cload(A + (ok ? aoff[i] + ka[u] : 0), vr, vi);
+        ar[u][i] = ok ? vr : 0.0;
+        ai[u][i] = ok ? asg * vi : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < BNB; ++j) {
         const bool ok = bval[j] && kb[u] >= 0;
-        const double2 v = B[ok ? kb[u] + boff[j] : 0];
-        br[u][j] = ok ? v.x : 0.0;
-        bi[u][j] = ok ? bsg * v.y : 0.0;
+        double vr, vi;
+        ptn::cload(B + (ok ? kb[u] + boff[j] : 0), vr, vi);
+        br[u][j] = ok ? vr : 0.0;
+        bi[u][j] = ok ? bsg * vi : 0.0;
       }
     }
     if (k + 4 * WU < kend) koffs(k + 4 * WU);
@@ -701,7 +740,7 @@ __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, 
   }
   // accumulator register r of lane l: row 4 r + l/16, column l%16 of the
   // block (of C, or of C^T when TR)
-  double2 *C = static_cast<double2 *>(p.C);
+  TC *C = static_cast<TC *>(p.C);
 #pragma unroll
   for (int j = 0; j < BNB; ++j) {
 #pragma unroll
@@ -718,13 +757,14 @@ __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, 
         }
         double ore = p.are * re - p.aim * im;
         double oim = p.are * im + p.aim * re;
-        double2 *dst = C + p.com[cm] + p.con[cn];
+        TC *dst = C + p.com[cm] + p.con[cn];
         if (p.beta_one) {
-          const double2 x = *dst;
-          ore += x.x;
-          oim += x.y;
+          double xr, xi;
+          ptn::cload(dst, xr, xi);
+          ore += xr;
+          oim += xi;
         }
-        *dst = make_double2(ore, oim);
+        ptn::cstore(dst, ore, oim);
       }
     }
   }
@@ -756,25 +796,37 @@ int wave_shape(int64_t M, int64_t N, int &bmb, int &bnb) {
 // launch the wave kernel for a validated descriptor; splits K over waves when
 // the tiles cannot fill the chip (each split >= 64 of K, partial traffic no
 // more than half the operand traffic, partials within the workspace)
-template <bool TR>
+template <bool TR, typename TA, typename TB, typename TC>
 void launch_wave_kernel(hipStream_t s, const KArgs &a, int bmb, int bnb, dim3 g, int64_t tn,
                         int64_t nwaves, int ks) {
   const dim3 b(256);
   if (bmb == 1 && bnb == 1)
-    hipLaunchKernelGGL((wave_gemm_f64<1, 1, TR>), g, b, 0, s, a, tn, nwaves, ks);
+    hipLaunchKernelGGL((wave_gemm_f64<1, 1, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
   else if (bmb == 2 && bnb == 1)
-    hipLaunchKernelGGL((wave_gemm_f64<2, 1, TR>), g, b, 0, s, a, tn, nwaves, ks);
+    hipLaunchKernelGGL((wave_gemm_f64<2, 1, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
   else if (bmb == 1 && bnb == 2)
-    hipLaunchKernelGGL((wave_gemm_f64<1, 2, TR>), g, b, 0, s, a, tn, nwaves, ks);
+    hipLaunchKernelGGL((wave_gemm_f64<1, 2, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
   else if (bmb == 2 && bnb == 2)
-    hipLaunchKernelGGL((wave_gemm_f64<2, 2, TR>), g, b, 0, s, a, tn, nwaves, ks);
+    hipLaunchKernelGGL((wave_gemm_f64<2, 2, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
   else if (bmb == 4 && bnb == 1)
-    hipLaunchKernelGGL((wave_gemm_f64<4, 1, TR>), g, b, 0, s, a, tn, nwaves, ks);
+    hipLaunchKernelGGL((wave_gemm_f64<4, 1, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
   else
-    hipLaunchKernelGGL((wave_gemm_f64<1, 4, TR>), g, b, 0, s, a, tn, nwaves, ks);
+    hipLaunchKernelGGL((wave_gemm_f64<1, 4, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
 }
 
-int launch_wave(hipStream_t s, KArgs a, int bmb, int bnb, int64_t ws_bytes, bool c_mmajor) {
+template <typename TA, typename TB, typename TC>
+void launch_wave_typed(hipStream_t s, const KArgs &a, int bmb, int bnb, dim3 g, int64_t tn,
+                       int64_t nwaves, int ks, bool tr) {
+  if (tr)
+    launch_wave_kernel<true, TA, TB, TC>(s, a, bmb, bnb, g, tn, nwaves, ks);
+  else
+    launch_wave_kernel<false, TA, TB, TC>(s, a, bmb, bnb, g, tn, nwaves, ks);
+}
+
+// the operand dtypes the library's contractions use: all complex128 (FP64), or
+// complex64 with at most one complex128 side (the mixed form)
+int launch_wave(hipStream_t s, KArgs a, int bmb, int bnb, int64_t ws_bytes, bool c_mmajor,
+                int adt, int bdt, int cdt) {
   const int64_t tn = (a.N + 16 * bnb - 1) / (16 * bnb);
   const int64_t tiles = ((a.M + 16 * bmb - 1) / (16 * bmb)) * tn;
   int64_t ksplit = 1;
@@ -808,20 +860,40 @@ int launch_wave(hipStream_t s, KArgs a, int bmb, int bnb, int64_t ws_bytes, bool
   const dim3 g((unsigned)blocks);
   const int ks = (int)ksplit;
   // lanes along m for the stores when C is m-contiguous (partials: n-contiguous)
-  if (c_mmajor && ksplit == 1)
-    launch_wave_kernel<true>(s, a, bmb, bnb, g, tn, nwaves, ks);
-  else
-    launch_wave_kernel<false>(s, a, bmb, bnb, g, tn, nwaves, ks);
+  const bool tr = c_mmajor && ksplit == 1;
+  const int code = adt * 4 + bdt * 2 + cdt;  // 0: complex128, 1: complex64
+  switch (code) {
+    case 0: launch_wave_typed<double2, double2, double2>(s, a, bmb, bnb, g, tn, nwaves, ks, tr);
+            break;
+    case 7: launch_wave_typed<float2, float2, float2>(s, a, bmb, bnb, g, tn, nwaves, ks, tr);
+            break;
+    case 6: launch_wave_typed<float2, float2, double2>(s, a, bmb, bnb, g, tn, nwaves, ks, tr);
+            break;
+    case 3: launch_wave_typed<double2, float2, float2>(s, a, bmb, bnb, g, tn, nwaves, ks, tr);
+            break;
+    case 5: launch_wave_typed<float2, double2, float2>(s, a, bmb, bnb, g, tn, nwaves, ks, tr);
+            break;
+    default:
+      ptn::set_error("ptn_gemm_gather: operand dtype combination %d not supported", code);
+      return PTN_EINVAL;
+  }
   PTN_LAUNCH_CHECK();
   if (ksplit > 1) {
     const int64_t total = a.M * a.N;
+    const bool c64 = cdt == PTN_C64;
     if (total <= 16384 && ksplit >= 16) {
-      hipLaunchKernelGGL(splitk_reduce_wide, dim3((unsigned)((total + 3) / 4)), dim3(256), 0, s,
-                         a, ks);
+      const dim3 gr((unsigned)((total + 3) / 4));
+      if (c64)
+        hipLaunchKernelGGL((splitk_reduce_wide<float2>), gr, dim3(256), 0, s, a, ks);
+      else
+        hipLaunchKernelGGL((splitk_reduce_wide<double2>), gr, dim3(256), 0, s, a, ks);
     } else {
       int64_t gr = (total + 255) / 256;
       if (gr > 4096) gr = 4096;
-      hipLaunchKernelGGL((splitk_reduce<double2>), dim3((unsigned)gr), dim3(256), 0, s, a, ks);
+      if (c64)
+        hipLaunchKernelGGL((splitk_reduce<float2>), dim3((unsigned)gr), dim3(256), 0, s, a, ks);
+      else
+        hipLaunchKernelGGL((splitk_reduce<double2>), dim3((unsigned)gr), dim3(256), 0, s, a, ks);
     }
     PTN_LAUNCH_CHECK();
   }
@@ -940,9 +1012,15 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   if (rc0) return rc0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   int bmb = 0, bnb = 0;
-  if (d->compute == PTN_COMPUTE_F64_4M && wave_shape(d->M, d->N, bmb, bnb)) {
+  // small and thin products run on the wave kernel in both compute modes (FP64
+  // MFMA, complex64 operands widened on load): they are bound by memory and
+  // latency, where the wider arithmetic costs nothing
+  const int code = d->a_dtype * 4 + d->b_dtype * 2 + d->c_dtype;
+  const bool wave_types = code == 0 || code == 7 || code == 6 || code == 3 || code == 5;
+  if (wave_types && wave_shape(d->M, d->N, bmb, bnb)) {
     a.ws = static_cast<double2 *>(d->ws);
-    return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0);
+    return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0, d->a_dtype,
+                       d->b_dtype, d->c_dtype);
   }
   a.remap = (tiles % 8 == 0) ? 1 : 0;
   // split-K when the tile grid cannot fill the chip and K is long: each split

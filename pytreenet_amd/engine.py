@@ -116,10 +116,11 @@ class LaunchProfiler:
     """Executed-flop counters and HIP-event timings of contraction launches.
 
     Events are recorded on the stream the kernels are launched on (torch's
-    current stream) around every launch with a tile grid >= LARGE_TILES: the
-    dominant H_eff / environment GEMMs.  A gather-GEMM of that size runs the
-    library's LARGE instantiation (never split-K), which profilers report as a
-    kernel of its own; a strided one runs as a rocBLAS GEMM.
+    current stream) around every launch that runs the gather-GEMM's LARGE
+    instantiation (``gemm_kernel_class`` "large": a tile grid >= LARGE_TILES,
+    never split-K), which profilers report as a kernel of its own, and around
+    every strided launch with such a grid (a rocBLAS GEMM): the dominant H_eff /
+    environment GEMMs.
     """
 
     LARGE_TILES = 2048   # tile-grid size from which the library uses its LARGE instantiation
@@ -306,11 +307,15 @@ _RECORDER: Optional[_Recorder] = None
 
 
 # strided GEMMs go to rocBLAS when the tile grid fills the chip; the gather-GEMM
-# keeps the gathered, short and split-K (long K, few tiles) cases
+# keeps the gathered, short and split-K (long K, few tiles) cases.  FP64 stays on
+# the gather-GEMM unless BLAS_FP64 is set: its three-product kernel outruns
+# rocBLAS zgemm on the strided shapes of the sweep (78.5 vs 68.8 TFLOP/s
+# algorithmic at M=131072, N=1280, K=256)
 BLAS_MIN_TILES = 256
 BLAS_MIN_DIM = 64
 BLAS_MAX_K_PER_TILE = 1 << 14
 USE_BLAS = True
+BLAS_FP64 = False
 
 
 _BLAS_OFF = 0   # > 0 while recording networks that are replayed as grouped launches
@@ -318,6 +323,8 @@ _BLAS_OFF = 0   # > 0 while recording networks that are replayed as grouped laun
 
 def _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine):
     if not USE_BLAS or _BLAS_OFF or K < 1 or min(M, N) < BLAS_MIN_DIM:
+        return None
+    if precision == FP64 and not BLAS_FP64:
         return None
     tiles = -(-M // 64) * -(-N // 64)
     if tiles < BLAS_MIN_TILES or K > BLAS_MAX_K_PER_TILE:
@@ -422,6 +429,17 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     _run_gemm(d, M, N, K, precision)
 
 
+def gemm_kernel_class(M: int, N: int, precision: str) -> str:
+    """The kernel ``ptn_gemm_gather`` runs a launch of this shape on (its dispatch
+    rule): "wave" (M or N within one 16-block, or both within two; either compute
+    mode), "large" (a tile grid of >= 2048 64 x 64 tiles: the LARGE instantiation,
+    never split-K) or "small" (the same kernel on a smaller grid, split-K allowed)."""
+    bm, bn = -(-M // 16), -(-N // 16)
+    if (bm <= 2 and bn <= 2) or bm == 1 or bn == 1:
+        return "wave"
+    return "large" if -(-M // 64) * -(-N // 64) >= PROFILER.LARGE_TILES else "small"
+
+
 def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
     lib = _lib.load()
     prof = PROFILER
@@ -429,8 +447,7 @@ def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
         fl = 8.0 * M * N * K
         prof.flops += fl
         prof.launches += 1
-        tiles = -(-M // 64) * -(-N // 64)
-        if tiles >= prof.LARGE_TILES:
+        if gemm_kernel_class(M, N, precision) == "large":
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()

@@ -336,8 +336,16 @@ def test_gemm_strided_layouts(gpu, dtype):
     assert ran == 36
 
 
-def test_contract_blas_matches_gather(gpu):
-    """Engine contractions routed to rocBLAS equal the gather-GEMM ones."""
+@pytest.fixture
+def blas_fp64():
+    E.BLAS_FP64 = True
+    yield
+    E.BLAS_FP64 = False
+
+
+def test_contract_blas_matches_gather(gpu, blas_fp64):
+    """Engine contractions routed to rocBLAS (FP64 strided dispatch switched on)
+    equal the gather-GEMM ones."""
     rng = np.random.default_rng(17)
     x = _dev(_crand(rng, (64, 64, 64, 2)))
     env = _dev(_crand(rng, (64, 5, 64)))

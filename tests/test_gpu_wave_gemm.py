@@ -74,3 +74,25 @@ def test_wave_gemm_accumulate_alpha_views(gpu, idx):
     k = max(1, int(np.prod([d for d, l in zip(sa, la) if l in lb])))
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=0,
                                atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2, 5, 6, 7, 10])
+@pytest.mark.parametrize("dts", [("c64", "c64", "c64"), ("c128", "c64", "c64"),
+                                 ("c64", "c64", "c128"), ("c64", "c128", "c64")])
+def test_wave_gemm_mixed_form(gpu, idx, dts):
+    """Small and thin products of the mixed form run on the wave kernel too: the
+    complex64 operands are widened on load and accumulated in FP64, so the result
+    is the exact product of the given operands up to FP64 rounding (and the final
+    rounding of a complex64 output)."""
+    sa, la, sb, lb, lo = CASES[idx]
+    _say(f"wave gemm mixed {la},{lb}->{lo} {dts}")
+    rng = np.random.default_rng(200 + idx)
+    tdt = {"c64": torch.complex64, "c128": torch.complex128}
+    a = torch.as_tensor(_crand(rng, sa)).to(tdt[dts[0]])
+    b = torch.as_tensor(_crand(rng, sb)).to(tdt[dts[1]])
+    got = E.contract(a.cuda(), la, b.cuda(), lb, lo, precision=E.FP32,
+                     out_dtype=tdt[dts[2]]).cpu().numpy()
+    ref = np.einsum(f"{la},{lb}->{lo}", a.numpy().astype(complex), b.numpy().astype(complex))
+    k = max(1, int(np.prod([d for d, l in zip(sa, la) if l in lb])))
+    tol = (2e-7 if dts[2] == "c64" else 1e-14 * np.sqrt(k)) * np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=tol)
