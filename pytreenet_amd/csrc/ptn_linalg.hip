@@ -202,11 +202,13 @@ __global__ __launch_bounds__(CT) void potrf_lower_kernel(int n, const double2 *G
 // kernel above.
 constexpr int PB = 16;
 constexpr int PT = 512;   // threads of the blocked kernel (256 VGPRs per lane)
+constexpr int PSTAGE = 4096;   // panel elements staged in LDS (rows <= 256)
 
 __global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 *G, int64_t ldg,
                                                            double2 *L, double shift_factor,
                                                            double min_pivot, int *status) {
   __shared__ double2 D[PB][PB + 1];
+  __shared__ double2 Pn[PSTAGE];
   __shared__ double red[PT / 64];
   __shared__ double sh_shift, sh_inv;
   __shared__ int sh_fail;
@@ -236,71 +238,106 @@ __global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 
   __syncthreads();
   for (int J = 0; J < n; J += PB) {
     const int jb = (n - J) < PB ? (n - J) : PB;
-    // (a) diagonal block in LDS
-    for (int t = tid; t < PB * PB; t += PT) {
-      const int i = t / PB, k = t % PB;
-      D[i][k] = (i < jb && k <= i) ? L[(int64_t)(J + i) * n + J + k] : make_double2(0, 0);
-    }
-    __syncthreads();
-    for (int j = 0; j < jb; ++j) {
-      if (tid == 0) {
-        double d = D[j][j].x;
-        if (!(d > min_pivot)) {  // also catches NaN
-          sh_fail = 1;
+    // (a) diagonal block, factorised by the first 16 lanes of wave 0 with no
+    //     block barrier: lane i holds row i in registers, the pivot and the
+    //     entries of column j are broadcast by 16-wide shuffles
+    if (tid < PB) {
+      const int i = tid;
+      double2 row[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k)
+        row[k] = (i < jb && k <= i) ? L[(int64_t)(J + i) * n + J + k] : make_double2(0, 0);
+      int fail = 0;
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const bool act = j < jb;  // columns past a short last block do nothing
+        double d = __shfl(row[j].x, j, PB);
+        if (act && !(d > min_pivot)) {  // also catches NaN
+          fail = 1;
           d = 1.0;
         }
-        const double r = sqrt(d);
-        D[j][j] = make_double2(r, 0.0);
-        sh_inv = 1.0 / r;
-      }
-      __syncthreads();
-      if (tid > j && tid < jb) {
-        D[tid][j].x *= sh_inv;
-        D[tid][j].y *= sh_inv;
-      }
-      __syncthreads();
-      if (tid < PB * PB) {
-        const int i = tid / PB, k = tid % PB;
-        if (i < jb && k > j && k <= i) {
-          const double2 a = D[i][j], b = D[k][j];
-          D[i][k].x -= a.x * b.x + a.y * b.y;  // a * conj(b)
-          D[i][k].y -= a.y * b.x - a.x * b.y;
+        if (!act) d = 1.0;
+        const double r = sqrt(d), inv = 1.0 / r;
+        if (act && i == j) row[j] = make_double2(r, 0.0);
+        if (act && i > j) {
+          row[j].x *= inv;
+          row[j].y *= inv;
+        }
+#pragma unroll
+        for (int k = j + 1; k < PB; ++k) {
+          // D[k][j] from lane k, then D[i][k] -= D[i][j] conj(D[k][j]) for k <= i
+          const double br = __shfl(row[j].x, k, PB), bi = __shfl(row[j].y, k, PB);
+          if (act && k <= i && i < jb) {
+            row[k].x -= row[j].x * br + row[j].y * bi;
+            row[k].y -= row[j].y * br - row[j].x * bi;
+          }
         }
       }
-      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const double2 v = (i < jb && k <= i) ? row[k] : make_double2(0, 0);
+        D[i][k] = v;
+        if (i < jb && k <= i) L[(int64_t)(J + i) * n + J + k] = v;
+      }
+      if (fail) sh_fail = 1;
     }
-    for (int t = tid; t < jb * jb; t += PT) {
-      const int i = t / jb, k = t % jb;
-      if (k <= i) L[(int64_t)(J + i) * n + J + k] = D[i][k];
-    }
+    __syncthreads();
     // (b) panel rows below the block: x L_JJ^H = a, i.e. x_k = (a_k - sum_{l<k} x_l
     //     conj(L_kl)) / L_kk.  Sixteen lanes per row, lane k owns x_k; x_l is
     //     broadcast within the 16-lane group once it is final.
+    //     Each 16-lane group solves PRB rows together (their loads in flight at
+    //     once); lane lk keeps row lk of the block and its pivot inverse in
+    //     registers.
     {
       const int lk = tid & (PB - 1), grp = tid / PB;
-      for (int i0 = J + jb; i0 < n; i0 += PT / PB) {
-        const int i = i0 + grp;
-        const bool ok = i < n && lk < jb;
-        double2 v = ok ? L[(int64_t)i * n + J + lk] : make_double2(0, 0);
-        for (int l = 0; l < jb; ++l) {
-          if (lk == l) {
-            const double inv = 1.0 / D[l][l].x;
-            v.x *= inv;
-            v.y *= inv;
-          }
-          const double xr = __shfl(v.x, l, PB), xi = __shfl(v.y, l, PB);
-          if (lk > l) {
-            const double2 b = D[lk][l];   // zero outside the block
-            v.x -= xr * b.x + xi * b.y;
-            v.y -= xi * b.x - xr * b.y;
+      constexpr int PRB = 8, GROUPS = PT / PB;
+      const double dinv = lk < jb ? 1.0 / D[lk][lk].x : 0.0;
+      double2 drow[PB];
+#pragma unroll
+      for (int l = 0; l < PB; ++l) drow[l] = D[lk][l];   // zero outside the block
+      for (int i0 = J + jb; i0 < n; i0 += PRB * GROUPS) {
+        double2 v[PRB];
+#pragma unroll
+        for (int rb = 0; rb < PRB; ++rb) {
+          const int i = i0 + rb * GROUPS + grp;
+          v[rb] = (i < n && lk < jb) ? L[(int64_t)i * n + J + lk] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int l = 0; l < PB; ++l) {
+          if (l >= jb) continue;  // uniform
+#pragma unroll
+          for (int rb = 0; rb < PRB; ++rb) {
+            if (lk == l) {
+              v[rb].x *= dinv;
+              v[rb].y *= dinv;
+            }
+            const double xr = __shfl(v[rb].x, l, PB), xi = __shfl(v[rb].y, l, PB);
+            if (lk > l) {
+              v[rb].x -= xr * drow[l].x + xi * drow[l].y;
+              v[rb].y -= xi * drow[l].x - xr * drow[l].y;
+            }
           }
         }
-        if (ok) L[(int64_t)i * n + J + lk] = v;
+#pragma unroll
+        for (int rb = 0; rb < PRB; ++rb) {
+          const int i = i0 + rb * GROUPS + grp;
+          if (i < n && lk < jb) L[(int64_t)i * n + J + lk] = v[rb];
+        }
       }
     }
     __syncthreads();
-    // (c) trailing update of the lower triangle, 4x4 tiles
+    // (c) trailing update of the lower triangle, 4x4 tiles.  The panel columns
+    //     (rows below the block) are staged in LDS when they fit, so the tile
+    //     loops read LDS instead of L2; columns past jb are zero
     const int r0 = J + jb, rem = n - r0;
+    const bool staged = rem > 0 && rem * PB <= PSTAGE;
+    if (staged) {
+      for (int t = tid; t < rem * PB; t += PT) {
+        const int r = t / PB, l = t % PB;
+        Pn[t] = l < jb ? L[(int64_t)(r0 + r) * n + J + l] : make_double2(0, 0);
+      }
+      __syncthreads();
+    }
     if (rem > 0) {
       const int nt = (rem + 3) / 4;
       const int ntiles = nt * (nt + 1) / 2;
@@ -320,13 +357,8 @@ __global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 
           rows[r] = r0 + ti * 4 + r;
           cols[r] = r0 + tk * 4 + r;
         }
-        for (int l = 0; l < jb; ++l) {
-          double2 a[4], b[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            a[r] = rows[r] < n ? L[(int64_t)rows[r] * n + J + l] : make_double2(0, 0);
-            b[r] = cols[r] < n ? L[(int64_t)cols[r] * n + J + l] : make_double2(0, 0);
-          }
+
+        auto madd = [&](const double2 (&a)[4], const double2 (&b)[4]) {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -334,18 +366,45 @@ __global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 
               acc[r][c].x += a[r].x * b[c].x + a[r].y * b[c].y;
               acc[r][c].y += a[r].y * b[c].x - a[r].x * b[c].y;
             }
+        };
+        if (staged) {
+#pragma unroll 2
+          for (int l = 0; l < PB; ++l) {
+            double2 a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              a[r] = rows[r] < n ? Pn[(rows[r] - r0) * PB + l] : make_double2(0, 0);
+              b[r] = cols[r] < n ? Pn[(cols[r] - r0) * PB + l] : make_double2(0, 0);
+            }
+            madd(a, b);
+          }
+        } else {
+          for (int l = 0; l < jb; ++l) {
+            double2 a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              a[r] = rows[r] < n ? L[(int64_t)rows[r] * n + J + l] : make_double2(0, 0);
+              b[r] = cols[r] < n ? L[(int64_t)cols[r] * n + J + l] : make_double2(0, 0);
+            }
+            madd(a, b);
+          }
         }
+        // all 16 current values are loaded before any store, so the loads fly
+        // together instead of each waiting behind the previous store
+        double2 cur[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            cur[r][c] = (rows[r] < n && cols[c] <= rows[r])
+                            ? L[(int64_t)rows[r] * n + cols[c]] : make_double2(0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            if (rows[r] < n && cols[c] <= rows[r]) {
-              double2 *p = L + (int64_t)rows[r] * n + cols[c];
-              double2 v = *p;
-              v.x -= acc[r][c].x;
-              v.y -= acc[r][c].y;
-              *p = v;
-            }
+            if (rows[r] < n && cols[c] <= rows[r])
+              L[(int64_t)rows[r] * n + cols[c]] =
+                  make_double2(cur[r][c].x - acc[r][c].x, cur[r][c].y - acc[r][c].y);
           }
       }
     }
