@@ -646,6 +646,11 @@ __global__ __launch_bounds__(256) void splitk_reduce_wide(KArgs p, int ksplit) {
 // (MFMA operands swapped, same registers): the accumulators then hold C with
 // lanes along m, which makes the stores contiguous when C is m-contiguous.
 constexpr int WU = 4;
+// 16-blocks per wave along the long side of a thin product (1, 2 or 4): one
+// block keeps the most waves in flight (18 vs 28 us at M=131072, N=K=16)
+#ifndef PTN_WAVE_THIN
+#define PTN_WAVE_THIN 1
+#endif
 
 template <int BMB, int BNB, bool TR, typename TA, typename TB, typename TC>
 __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, int64_t nwaves,
@@ -770,10 +775,13 @@ __global__ __launch_bounds__(256) void wave_gemm_f64(KArgs p, int64_t wtiles_n, 
   }
 }
 
-// Shapes the wave kernel takes: both sides within two 16-blocks, or one side
-// within one 16-block (the other then runs in 64-wide blocks).  Returns 0 for
-// shapes that stay with the 64 x 64 workgroup tile.
-int wave_shape(int64_t M, int64_t N, int &bmb, int &bnb) {
+// Shapes the wave kernel takes: both sides within two 16-blocks; one side
+// within one 16-block (the other then runs in blocks of PTN_WAVE_THIN); or,
+// for K <= 256, N within three or five 16-blocks (blocks of 1 x 3 or 1 x 5:
+// the 64 x 64 tile would waste most of its second column of tiles and spend a
+// whole prologue on a few K steps), or M within three with N longer (3 x 1).  Returns 0 for shapes that stay
+// with the 64 x 64 workgroup tile.
+int wave_shape(int64_t M, int64_t N, int64_t K, int &bmb, int &bnb) {
   const int64_t bm = (M + 15) / 16, bn = (N + 15) / 16;
   if (bm <= 2 && bn <= 2) {
     bmb = (int)bm;
@@ -781,13 +789,23 @@ int wave_shape(int64_t M, int64_t N, int &bmb, int &bnb) {
     return 1;
   }
   if (bn == 1) {
-    bmb = 4;
+    bmb = PTN_WAVE_THIN;
     bnb = 1;
     return 1;
   }
   if (bm == 1) {
     bmb = 1;
-    bnb = 4;
+    bnb = PTN_WAVE_THIN;
+    return 1;
+  }
+  if (K <= 256 && (bn == 3 || bn == 5) && bm > bn) {
+    bmb = 1;
+    bnb = (int)bn;
+    return 1;
+  }
+  if (K <= 256 && bm == 3 && bn > bm) {
+    bmb = 3;
+    bnb = 1;
     return 1;
   }
   return 0;
@@ -810,8 +828,14 @@ void launch_wave_kernel(hipStream_t s, const KArgs &a, int bmb, int bnb, dim3 g,
     hipLaunchKernelGGL((wave_gemm_f64<2, 2, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
   else if (bmb == 4 && bnb == 1)
     hipLaunchKernelGGL((wave_gemm_f64<4, 1, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
-  else
+  else if (bmb == 1 && bnb == 4)
     hipLaunchKernelGGL((wave_gemm_f64<1, 4, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
+  else if (bmb == 1 && bnb == 3)
+    hipLaunchKernelGGL((wave_gemm_f64<1, 3, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
+  else if (bmb == 3 && bnb == 1)
+    hipLaunchKernelGGL((wave_gemm_f64<3, 1, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
+  else
+    hipLaunchKernelGGL((wave_gemm_f64<1, 5, TR, TA, TB, TC>), g, b, 0, s, a, tn, nwaves, ks);
 }
 
 template <typename TA, typename TB, typename TC>
@@ -1017,7 +1041,7 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   // latency, where the wider arithmetic costs nothing
   const int code = d->a_dtype * 4 + d->b_dtype * 2 + d->c_dtype;
   const bool wave_types = code == 0 || code == 7 || code == 6 || code == 3 || code == 5;
-  if (wave_types && wave_shape(d->M, d->N, bmb, bnb)) {
+  if (wave_types && wave_shape(d->M, d->N, d->K, bmb, bnb)) {
     a.ws = static_cast<double2 *>(d->ws);
     return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0, d->a_dtype,
                        d->b_dtype, d->c_dtype);

@@ -429,13 +429,16 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     _run_gemm(d, M, N, K, precision)
 
 
-def gemm_kernel_class(M: int, N: int, precision: str) -> str:
+def gemm_kernel_class(M: int, N: int, K: int, precision: str) -> str:
     """The kernel ``ptn_gemm_gather`` runs a launch of this shape on (its dispatch
-    rule): "wave" (M or N within one 16-block, or both within two; either compute
-    mode), "large" (a tile grid of >= 2048 64 x 64 tiles: the LARGE instantiation,
+    rule): "wave" (M or N within one 16-block, or both within two, or, for
+    K <= 256, N within three or five 16-blocks below a longer M, or M within three
+    below a longer N; either compute mode), "large" (a tile grid of >= 2048 64 x 64 tiles: the LARGE instantiation,
     never split-K) or "small" (the same kernel on a smaller grid, split-K allowed)."""
     bm, bn = -(-M // 16), -(-N // 16)
     if (bm <= 2 and bn <= 2) or bm == 1 or bn == 1:
+        return "wave"
+    if K <= 256 and ((bn in (3, 5) and bm > bn) or (bm == 3 and bn > bm)):
         return "wave"
     return "large" if -(-M // 64) * -(-N // 64) >= PROFILER.LARGE_TILES else "small"
 
@@ -447,7 +450,7 @@ def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
         fl = 8.0 * M * N * K
         prof.flops += fl
         prof.launches += 1
-        if gemm_kernel_class(M, N, precision) == "large":
+        if gemm_kernel_class(M, N, K, precision) == "large":
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()

@@ -39,6 +39,10 @@ CASES = [
     ((50, 100, 16), "xyk", (16, 16), "kn", "nxy"),     # 5000 x 16 x 16, m-contiguous C
     ((16, 80), "mk", (100, 80, 50), "xky", "mxy"),     # operands swapped, then m-contiguous
     ((20, 30), "mk", (30, 25), "kn", "nm"),            # (2,2), m-contiguous C
+    ((40, 30, 16), "xyk", (16, 80), "kn", "xyn"),      # 1200 x 80 x 16: (1,5)
+    ((48, 200), "mk", (200, 30, 20), "kxy", "xym"),    # 600 x 48 x 200 after the swap: (1,3)
+    ((40, 100), "mk", (100, 60), "kn", "mn"),          # 40 x 60 x 100: (3,1)
+    ((40, 30, 100), "xyk", (100, 40), "kn", "nxy"),    # 1200 x 40 x 100: (1,3), m-contiguous C
 ]
 
 
@@ -56,7 +60,7 @@ def test_wave_gemm_matches_einsum(gpu, case, conj):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("idx", [0, 3, 5, 7, 10, 11])
+@pytest.mark.parametrize("idx", [0, 3, 5, 7, 10, 11, 13, 15, 16])
 def test_wave_gemm_accumulate_alpha_views(gpu, idx):
     """Strided operand views and output, C += alpha A B."""
     sa, la, sb, lb, lo = CASES[idx]
@@ -76,7 +80,7 @@ def test_wave_gemm_accumulate_alpha_views(gpu, idx):
                                atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("idx", [0, 1, 2, 5, 6, 7, 10])
+@pytest.mark.parametrize("idx", [0, 1, 2, 5, 6, 7, 10, 13, 14, 15])
 @pytest.mark.parametrize("dts", [("c64", "c64", "c64"), ("c128", "c64", "c64"),
                                  ("c64", "c64", "c128"), ("c64", "c128", "c64")])
 def test_wave_gemm_mixed_form(gpu, idx, dts):
