@@ -19,6 +19,8 @@ gauge-invariant quantity).  Node metadata lives on the host.
 """
 from __future__ import annotations
 
+import json
+import os
 import uuid
 from typing import Dict, Iterator, List, Optional, Tuple, Union
 
@@ -233,6 +235,57 @@ class TreeTensorNetwork:
         return new
 
     __deepcopy__ = lambda self, memo: self.copy()
+
+    # -- interchange files (core/ttn.py:1748-1837) ----------------------------------
+    def save(self, filepath: str):
+        """Write ``{filepath}.npz`` (every tensor under its node id, in the canonical
+        leg order (parent, children..., open)) and ``{filepath}.json`` (root id,
+        orthogonality centre, and per node its identifier, parent, children and
+        shape).  These are the reference's two files: its ``TreeTensorNetwork.load``
+        reads them, and ``load`` below reads the reference's."""
+        arrays = {k: t.detach().cpu().numpy() for k, t in self._tensors.items()}
+        np.savez(f"{filepath}.npz", **arrays)
+        data = {"root_id": self._root_id,
+                "orthogonality_center_id": self.orthogonality_center_id,
+                "nodes": {nid: {"identifier": n.identifier, "parent": n.parent,
+                                "children": list(n.children),
+                                "shape": [int(d) for d in self._tensors[nid].shape]}
+                          for nid, n in self._nodes.items()}}
+        with open(f"{filepath}.json", "w") as f:
+            json.dump(data, f, indent=2)
+
+    @classmethod
+    def load(cls, filepath: str, device=None) -> "TreeTensorNetwork":
+        """Read the two files ``save`` writes (or the reference's).  Tensors go to
+        ``device`` (default: the ROCm device when present) as complex128; every
+        bond is checked to have the same dimension on both sides."""
+        if not (os.path.exists(f"{filepath}.npz") and os.path.exists(f"{filepath}.json")):
+            raise FileNotFoundError(f"Could not find required files at {filepath}!")
+        with open(f"{filepath}.json") as f:
+            data = json.load(f)
+        ttn = cls()
+        with np.load(f"{filepath}.npz") as arrays:
+            for nid, nd in data["nodes"].items():
+                node = Node(identifier=nd["identifier"])
+                node.parent = nd["parent"]
+                node.children = list(nd["children"])
+                t = as_device_tensor(arrays[nid], device=device)
+                node.shape = tuple(t.shape)
+                ttn._nodes[nid] = node
+                dict.__setitem__(ttn._tensors, nid, t)
+        ttn._root_id = data["root_id"]
+        ttn.orthogonality_center_id = data["orthogonality_center_id"]
+        for nid, node in ttn._nodes.items():
+            for c in node.children:
+                child = ttn._nodes[c]
+                if child.parent != nid:
+                    raise ValueError(f"{c} is listed as a child of {nid} but names "
+                                     f"{child.parent} as its parent")
+                if node.shape[node.neighbour_index(c)] != child.shape[0]:
+                    raise ValueError(f"bond {nid}-{c} has dimension "
+                                     f"{node.shape[node.neighbour_index(c)]} at {nid} and "
+                                     f"{child.shape[0]} at {c}")
+        return ttn
 
     # -- tree queries (core/tree_structure.py) ----------------------------------------
     def neighbours(self, nid: str) -> List[str]:

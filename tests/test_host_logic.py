@@ -162,3 +162,41 @@ def test_network_planner_block_sparse_root_cost():
     plan = plan_network(labels, dims, [("b", i) for i in range(3)] + ["q"],
                         (1, [("w", i) for i in range(3)], bs.tuples))
     assert abs(plan.flops / 1e9 - 1312.1) < 0.1
+
+
+def test_ttn_save_load_round_trip(tmp_path):
+    """The reference's two interchange files: tensors under node ids in canonical leg
+    order (.npz) plus the structure (.json); the round trip keeps everything."""
+    from pytreenet_amd.random_states import build_state, random_state_tensors
+    ids, parent, children = complete_tree(2, 2)
+    tensors = random_state_tensors(ids, parent, children, 4, seed=3)
+    st = build_state(ids, parent, children, tensors, device="cpu")
+    st.orthogonality_center_id = ids[1]
+    path = str(tmp_path / "state")
+    st.save(path)
+    meta = json.load(open(path + ".json"))
+    assert meta["root_id"] == st.root_id and meta["orthogonality_center_id"] == ids[1]
+    assert set(meta["nodes"]) == set(ids)
+    for nid in ids:
+        assert meta["nodes"][nid]["children"] == list(st.nodes[nid].children)
+        assert meta["nodes"][nid]["shape"] == list(st.tensors[nid].shape)
+    back = ptn.TreeTensorNetworkState.load(path, device="cpu")
+    assert isinstance(back, ptn.TreeTensorNetworkState)
+    assert back.root_id == st.root_id and back.orthogonality_center_id == ids[1]
+    for nid in ids:
+        assert back.nodes[nid].parent == st.nodes[nid].parent
+        assert back.nodes[nid].children == st.nodes[nid].children
+        assert np.array_equal(back.tensors[nid].numpy(), st.tensors[nid].numpy())
+
+
+def test_ttn_load_rejects_missing_and_inconsistent_files(tmp_path):
+    with pytest.raises(FileNotFoundError):
+        ptn.TreeTensorNetworkState.load(str(tmp_path / "nothing"), device="cpu")
+    np.savez(str(tmp_path / "bad.npz"), r=np.ones((3, 2), complex), c=np.ones((2, 2), complex))
+    json.dump({"root_id": "r", "orthogonality_center_id": None,
+               "nodes": {"r": {"identifier": "r", "parent": None, "children": ["c"],
+                               "shape": [3, 2]},
+                         "c": {"identifier": "c", "parent": "r", "children": [],
+                               "shape": [2, 2]}}}, open(str(tmp_path / "bad.json"), "w"))
+    with pytest.raises(ValueError, match="bond r-c"):
+        ptn.TreeTensorNetworkState.load(str(tmp_path / "bad"), device="cpu")
