@@ -66,6 +66,7 @@ struct KArgs {
   int64_t kchunk;   // K range per split (multiple of BK)
   double2 *ws;      // split-K partials [ksplit][M][N], null when gridDim.y == 1
   int aconj, akmaj, bconj, bkmaj, beta_one, remap;
+  int cmaj;         // consecutive m contiguous in C (store-order hint)
 };
 
 template <int MODE>
@@ -355,6 +356,10 @@ __device__ __forceinline__ void lds_dma16(const double2 *src, double2 *lds_row) 
                                    (__attribute__((address_space(3))) void *)lds_row, 16, 0, 0);
 }
 
+// TR computes the transposed block product (MFMA operands swapped, same
+// registers): the accumulators then run along m, so the epilogue stores are
+// contiguous when C is m-contiguous.
+template <bool TR>
 __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__restrict__ aok,
                                             const int64_t *__restrict__ bok, int64_t bid,
                                             int64_t split, bool splitk) {
@@ -416,6 +421,13 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
   const double asg = p.aconj ? -1.0 : 1.0, bsg = p.bconj ? -1.0 : 1.0;
   const int fr = lane & 15, fk = lane >> 4;
 
+  // x supplies the A side of the block product, y the B side
+  auto mf = [](double x, double y, v4d_t c) {
+    if constexpr (TR)
+      return __builtin_amdgcn_mfma_f64_16x16x4f64(y, x, c, 0, 0, 0);
+    else
+      return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  };
   auto compute = [&](double2 (*sa)[BM], double2 (*sb)[BN]) {
 #pragma unroll
     for (int kk = 0; kk < DBK / 4; ++kk) {
@@ -441,19 +453,19 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], c0[i][j], 0, 0, 0);
-          c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], c1[i][j], 0, 0, 0);
-          c2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], c2[i][j], 0, 0, 0);
+          c0[i][j] = mf(ar[i], br[j], c0[i][j]);
+          c1[i][j] = mf(ai[i], bi[j], c1[i][j]);
+          c2[i][j] = mf(as[i], bs[j], c2[i][j]);
         }
 #else
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], c0[i][j], 0, 0, 0);
-          c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], c1[i][j], 0, 0, 0);
-          c0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], c0[i][j], 0, 0, 0);
-          c1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], c1[i][j], 0, 0, 0);
+          c0[i][j] = mf(ar[i], br[j], c0[i][j]);
+          c1[i][j] = mf(ar[i], bi[j], c1[i][j]);
+          c0[i][j] = mf(-ai[i], bi[j], c0[i][j]);
+          c1[i][j] = mf(ai[i], br[j], c1[i][j]);
         }
 #endif
     }
@@ -498,15 +510,15 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
   double2 *C = static_cast<double2 *>(p.C);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int64_t cn = n0 + wn * 32 + j * 16 + fr;
-    if (cn >= p.N) continue;
-    const int64_t coff = p.con[cn];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t cm = m0 + wm * 32 + i * 16 + r * 4 + fk;
-        if (cm >= p.M) continue;
+        // accumulator register r of lane l: row 4 r + l/16, column l%16 of the
+        // block of C (of C^T when TR)
+        const int64_t cm = TR ? m0 + wm * 32 + i * 16 + fr : m0 + wm * 32 + i * 16 + r * 4 + fk;
+        const int64_t cn = TR ? n0 + wn * 32 + j * 16 + r * 4 + fk : n0 + wn * 32 + j * 16 + fr;
+        if (cm >= p.M || cn >= p.N) continue;
 #if PTN_F64_3M
         const double t1 = c0[i][j][r], t2 = c1[i][j][r];
         const double re = t1 - t2, im = c2[i][j][r] - t1 - t2;
@@ -519,7 +531,7 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
         }
         double ore = p.are * re - p.aim * im;
         double oim = p.are * im + p.aim * re;
-        double2 *dst = C + p.com[cm] + coff;
+        double2 *dst = C + p.com[cm] + p.con[cn];
         if (p.beta_one) {
           const double2 x = *dst;
           ore += x.x;
@@ -531,10 +543,10 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
   }
 }
 
-template <int LARGE>
+template <int LARGE, bool TR>
 __global__ __launch_bounds__(NT, PTN_DIRECT_OCC) void gemm_direct_f64_kernel(
     KArgs p, const int64_t *__restrict__ aok, const int64_t *__restrict__ bok) {
-  direct_tile(p, aok, bok, remapped_tile(p), blockIdx.y, gridDim.y > 1);
+  direct_tile<TR>(p, aok, bok, remapped_tile(p), blockIdx.y, gridDim.y > 1);
 }
 
 // ---- grouped launches -----------------------------------------------------------
@@ -561,7 +573,7 @@ __global__ __launch_bounds__(NT, PTN_DIRECT_OCC) void gemm_direct_f64_grouped(co
   const int64_t b = blockIdx.x;
   const int d = find_desc(prefix, count, b);
   const KArgs p = descs[d];
-  direct_tile(p, p.aok, p.bok, b - prefix[d], 0, false);
+  direct_tile<false>(p, p.aok, p.bok, b - prefix[d], 0, false);
 }
 
 template <int MODE, typename TA, typename TB, typename TC>
@@ -928,12 +940,20 @@ template <int MODE, typename TA, typename TB, typename TC>
 int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
   const bool large = ksplit == 1 && tiles >= LARGE_TILES;
   if constexpr (MODE == PTN_COMPUTE_F64_4M) {
-    if (large)
-      hipLaunchKernelGGL((gemm_direct_f64_kernel<1>), dim3((unsigned)tiles, 1), dim3(NT), 0, s,
-                         a, a.aok, a.bok);
+    // stores along m when C is m-contiguous (not for split-K partials)
+    const bool tr = a.cmaj && ksplit == 1;
+    if (large && tr)
+      hipLaunchKernelGGL((gemm_direct_f64_kernel<1, true>), dim3((unsigned)tiles, 1), dim3(NT),
+                         0, s, a, a.aok, a.bok);
+    else if (large)
+      hipLaunchKernelGGL((gemm_direct_f64_kernel<1, false>), dim3((unsigned)tiles, 1), dim3(NT),
+                         0, s, a, a.aok, a.bok);
+    else if (tr)
+      hipLaunchKernelGGL((gemm_direct_f64_kernel<0, true>), dim3((unsigned)tiles, 1), dim3(NT),
+                         0, s, a, a.aok, a.bok);
     else
-      hipLaunchKernelGGL((gemm_direct_f64_kernel<0>), dim3((unsigned)tiles, ksplit), dim3(NT), 0,
-                         s, a, a.aok, a.bok);
+      hipLaunchKernelGGL((gemm_direct_f64_kernel<0, false>), dim3((unsigned)tiles, ksplit),
+                         dim3(NT), 0, s, a, a.aok, a.bok);
   } else {
     if (large)
       hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
@@ -1022,6 +1042,7 @@ int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
     return PTN_ERANGE;
   }
   a.remap = 0;
+  a.cmaj = d->c_mmajor != 0;
   a.ws = nullptr;
   a.kchunk = d->K > 0 ? ((d->K + BK - 1) / BK) * BK : BK;
   return PTN_OK;

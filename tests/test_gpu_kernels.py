@@ -456,3 +456,28 @@ def test_qr_split_small_rank_deficient_and_views(gpu):
     tv = base.transpose(2, 0, 1)
     rec = np.einsum("kab,kj->jab", q.cpu().numpy(), r.cpu().numpy())
     np.testing.assert_allclose(rec, tv, atol=1e-12 * np.abs(tv).max())
+
+
+@pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
+@pytest.mark.parametrize("case", [
+    ((200, 70), "mk", (70, 150), "kn", "nm"),            # 64x64 tiles, m-contiguous C
+    ((4, 50, 9, 7), "abkc", (7, 9, 130), "ckn", "nab"),   # gathered, m-contiguous C
+    ((100, 3000), "mk", (3000, 90), "kn", "nm"),          # split-K grid: no transposed stores
+])
+def test_direct_gemm_m_contiguous_output(gpu, case, conj):
+    """The 64 x 64 kernel with the C-layout hint set (lanes along m in the
+    epilogue), plain and accumulating with alpha."""
+    sa, la, sb, lb, lo = case
+    _say(f"direct gemm m-contiguous C {la},{lb}->{lo} conj={conj}")
+    rng = np.random.default_rng(sum(sa) + sum(sb) + 3 * conj[0] + 5 * conj[1])
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    ref = np.einsum(f"{la},{lb}->{lo}", a.conj() if conj[0] else a, b.conj() if conj[1] else b)
+    got = E.contract(_dev(a), la, _dev(b), lb, lo, conj_a=conj[0], conj_b=conj[1]).cpu().numpy()
+    k = int(np.prod([d for d, l in zip(sa, la) if l in lb]))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
+    base = _crand(rng, ref.shape)
+    out = _dev(base)
+    E.contract(_dev(a), la, _dev(b), lb, lo, conj_a=conj[0], conj_b=conj[1], out=out,
+               accumulate=True, alpha=-0.5 + 0.25j)
+    np.testing.assert_allclose(out.cpu().numpy(), base + (-0.5 + 0.25j) * ref, rtol=0,
+                               atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
