@@ -348,6 +348,49 @@ __global__ __launch_bounds__(RT) void subtract_proj(int64_t n, const double2 *Q,
   subtract_proj_body(n, Q, ldq, nvec, h, w, partial, pstride, blockIdx.x, gridDim.x);
 }
 
+// The first projection and the second pass's dot products in one sweep (for
+// nvec <= FUSE_V): w' = w - sum_v h[v] Q[v] is stored and partial[b][v] =
+// sum over block b's range of conj(Q[v]) w', with the Q values of an element
+// held in registers between the two uses.  Same per-element order and the same
+// block reduction as subtract_proj followed by dots_partial.
+constexpr int FUSE_V = 16;
+__global__ __launch_bounds__(RT) void subtract_dots(int64_t n, const double2 *Q, int64_t ldq,
+                                                    int nvec, const double2 *h, double2 *w,
+                                                    double2 *partial, int pstride) {
+  __shared__ double2 hs[FUSE_V];
+  for (int v = threadIdx.x; v < nvec; v += RT) hs[v] = h[v];
+  __syncthreads();
+  double2 acc[FUSE_V];
+#pragma unroll
+  for (int v = 0; v < FUSE_V; ++v) acc[v] = make_double2(0, 0);
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    double2 wi = w[i];
+    double2 q[FUSE_V];
+#pragma unroll
+    for (int v = 0; v < FUSE_V; ++v)
+      q[v] = v < nvec ? Q[(int64_t)v * ldq + i] : make_double2(0, 0);
+#pragma unroll
+    for (int v = 0; v < FUSE_V; ++v) {
+      if (v < nvec) {
+        const double2 hv = hs[v];
+        wi.x -= hv.x * q[v].x - hv.y * q[v].y;
+        wi.y -= hv.x * q[v].y + hv.y * q[v].x;
+      }
+    }
+    w[i] = wi;
+#pragma unroll
+    for (int v = 0; v < FUSE_V; ++v) {
+      if (v < nvec) {
+        const double2 t = cmul_conj_a(q[v], wi);
+        acc[v].x += t.x;
+        acc[v].y += t.y;
+      }
+    }
+  }
+  block_reduce_store<FUSE_V>(acc, nvec, partial + (int64_t)blockIdx.x * pstride);
+}
+
 __device__ __forceinline__ void combine_body(int64_t n, const double2 *Q, int64_t ldq, int nvec,
                                              const double2 *c, double2 *out, int blk, int nblk) {
   __shared__ double2 cs[MAXV];
@@ -558,7 +601,29 @@ extern "C" int ptn_krylov_orthonormalize(void *stream, int64_t n, const void *Qv
   double2 *scratch = partial + (int64_t)MAXBLK * pstride;
   double2 *h1 = scratch, *h2 = scratch + (MAXV + 1), *inv = scratch + 2 * (MAXV + 1);
   const int nb = nblocks_for(n);
-  for (int pass = 0; pass < 2; ++pass) {
+  if (nvec <= FUSE_V) {
+    // three sweeps: dots (+ |w|^2), subtract fused with the second dots,
+    // subtract (+ |w'|^2)
+    hipLaunchKernelGGL(dots_partial, dim3(nb), dim3(RT), 0, s, n, Q, ldq, 0,
+                       nvec < KV ? nvec : KV, w, partial, pstride, nvec <= KV ? 1 : -1);
+    PTN_LAUNCH_CHECK();
+    if (nvec > KV) {
+      hipLaunchKernelGGL(dots_partial, dim3(nb), dim3(RT), 0, s, n, Q, ldq, KV, nvec - KV, w,
+                         partial, pstride, 1);
+      PTN_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(reduce_cols, dim3(nvec + 1), dim3(RT), 0, s, partial, pstride, nb, h1);
+    PTN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(subtract_dots, dim3(nb), dim3(RT), 0, s, n, Q, ldq, nvec, h1, w, partial,
+                       pstride);
+    PTN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_cols, dim3(nvec), dim3(RT), 0, s, partial, pstride, nb, h2);
+    PTN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(subtract_proj, dim3(nb), dim3(RT), 0, s, n, Q, ldq, nvec, h2, w, partial,
+                       pstride);
+    PTN_LAUNCH_CHECK();
+  }
+  for (int pass = 0; pass < 2 && nvec > FUSE_V; ++pass) {
     double2 *h = pass == 0 ? h1 : h2;
     for (int v0 = 0; v0 < nvec; v0 += KV) {
       const int cnt = (nvec - v0) < KV ? (nvec - v0) : KV;
