@@ -284,6 +284,20 @@ int ptn_block_apply(void *stream, int64_t R, int32_t nt, int32_t ng, int32_t nj,
                     const void *W, int32_t conj_w, void *out, int32_t out_dtype,
                     const int64_t *o_r, const int64_t *o_g, const int64_t *o_j);
 
+/* ptn_block_apply with the inputs staged through LDS: rows come in tiles of
+ * 256 consecutive rows along the innermost row dimension, row r of a tile at
+ * y_r[r0] + (r - r0) * row_stride, and every input of a tile lies within
+ * [y_r[r0] + min_offset, y_r[r0] + min_offset + span).  Requires R a multiple
+ * of 256 and span * sizeof(Y element) + nt nj ni * 16 <= 60 KiB.  Same results
+ * as ptn_block_apply; the workgroup reads the span with coalesced loads.   */
+int ptn_block_apply_staged(void *stream, int64_t R, int32_t nt, int32_t ng, int32_t nj,
+                           int32_t ni, const void *Y, int32_t y_dtype, const int64_t *y_r,
+                           const int64_t *y_t, const int64_t *y_i, int32_t conj_y,
+                           const int32_t *g_begin, const void *W, int32_t conj_w, void *out,
+                           int32_t out_dtype, const int64_t *o_r, const int64_t *o_g,
+                           const int64_t *o_j, int64_t row_stride, int64_t min_offset,
+                           int32_t span);
+
 const char *ptn_last_error(void);
 int ptn_version(void);
 

@@ -24,7 +24,7 @@ EXPORTED = (
     "ptn_gemm_grouped_work_bytes", "ptn_gemm_gather_grouped", "ptn_krylov_batched_work_bytes",
     "ptn_krylov_orthonormalize_batched", "ptn_normalize_batched", "ptn_krylov_combine_batched",
     "ptn_ptr_batch_work_bytes", "ptn_fill_batched", "ptn_permute_batched", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
-    "ptn_qr_split_small",
+    "ptn_qr_split_small", "ptn_block_apply_staged",
     "ptn_last_error", "ptn_version",
 )
 
@@ -101,6 +101,9 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_qr_split_small": (ctypes.c_int, [vp, i32, vp, vp, i32, vp, vp, vp]),
         "ptn_block_apply": (ctypes.c_int, [vp, i64, i32, i32, i32, i32, vp, i32, vp, vp, vp,
                                            i32, vp, vp, i32, vp, i32, vp, vp, vp]),
+        "ptn_block_apply_staged": (ctypes.c_int, [vp, i64, i32, i32, i32, i32, vp, i32, vp, vp,
+                                                  vp, i32, vp, vp, i32, vp, i32, vp, vp, vp,
+                                                  i64, i64, i32]),
         "ptn_gemm_grouped_work_bytes": (i64, [i32]),
         "ptn_gemm_gather_grouped": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc), i32, vp]),
         "ptn_krylov_batched_work_bytes": (i64, [i32]),

@@ -688,6 +688,9 @@ def _group_gemms(X, Y, conj_y, dims, m_d, k_d, n_l, sx, sy, groups, precision, t
 
 
 _BAPPLY_CACHE: "OrderedDict[tuple, tuple]" = OrderedDict()
+BLOCK_TILE = 256                 # rows per workgroup of ptn_block_apply_staged
+BLOCK_STAGE_BYTES = 60 * 1024    # its LDS for the operator blocks and the staged inputs
+USE_BLOCK_STAGED = True
 
 
 def _block_apply(X, Y, conj_y, dims, m_d, k_d, n_l, k_blk, cols_k, cols_m, G, inv, sy, target,
@@ -722,12 +725,24 @@ def _block_apply(X, Y, conj_y, dims, m_d, k_d, n_l, k_blk, cols_k, cols_m, G, in
         gb32 = torch.from_numpy(g_begin).to(dev)
         y_i = _offsets([dims[l] for l in k_d], [sy[l] for l in k_d])
         o_j = _offsets([dims[l] for l in m_d], [o_strides[l] for l in m_d])
+        # LDS staging of the inputs: tiles of BLOCK_TILE rows along the innermost
+        # row dimension, when the inputs of a tile fit one bounded span
+        staged = None
+        d0 = dims[n_l[-1]] if n_l else 0
+        s0 = sy[n_l[-1]] if n_l else 0
+        if n_l and d0 % BLOCK_TILE == 0 and s0 > 0:
+            offs = (y_t[:, None] + y_i[None, :]).reshape(-1)
+            lo, hi = int(offs.min()), int(offs.max())
+            span = (BLOCK_TILE - 1) * s0 + hi - lo + 1
+            ybytes = 16 if Y.dtype == torch.complex128 else 8
+            if span * ybytes + len(y_t) * nj * ni * 16 <= BLOCK_STAGE_BYTES:
+                staged = (s0, lo, span)
         hit = (big, sm, gb32, tuple(perm), order, len(y_r), len(order), len(G), nj, ni, y_i, o_j,
-               len(y_t))
+               len(y_t), staged)
         _BAPPLY_CACHE[key] = hit
         if len(_BAPPLY_CACHE) > 1024:
             _BAPPLY_CACHE.popitem(last=False)
-    big, sm, gb32, perm, order, R, nt, ng, nj, ni, y_i, o_j, nyt = hit
+    big, sm, gb32, perm, order, R, nt, ng, nj, ni, y_i, o_j, nyt, staged = hit
     # the operator values: cached on the persistent BlockSparse, never on structure alone
     wkey = (perm, order.tobytes())
     w = None
@@ -746,6 +761,8 @@ def _block_apply(X, Y, conj_y, dims, m_d, k_d, n_l, k_blk, cols_k, cols_m, G, in
             int(X.conj), None, _DTYPE_CODE[target.dtype],
             ctypes.c_void_p(big.data_ptr() + 8 * R), ctypes.c_void_p(sm.data_ptr() + 8 * nyt),
             arr(*[int(v) for v in o_j] + [0] * (4 - nj))]
+    if staged is not None and USE_BLOCK_STAGED:
+        args += [staged[0], staged[1], staged[2]]
     keep = (big, sm, gb32, w)
     if _RECORDER is not None:
         _RECORDER.ops.append(("block", args, Y, target, keep))
@@ -756,8 +773,13 @@ def _run_block(args, Y, target):
     args = list(args)
     args[5] = ctypes.c_void_p(Y.data_ptr())
     args[14] = ctypes.c_void_p(target.data_ptr())
-    _lib.check(_lib.load().ptn_block_apply(ctypes.c_void_p(stream_handle()), *args),
-               "ptn_block_apply")
+    lib = _lib.load()
+    if len(args) > 19:   # with the LDS-staging parameters
+        _lib.check(lib.ptn_block_apply_staged(ctypes.c_void_p(stream_handle()), *args),
+                   "ptn_block_apply_staged")
+    else:
+        _lib.check(lib.ptn_block_apply(ctypes.c_void_p(stream_handle()), *args),
+                   "ptn_block_apply")
 
 
 @dataclass

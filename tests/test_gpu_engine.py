@@ -87,3 +87,41 @@ def test_cached_network_replay_matches_direct(gpu, blocked):
     ref = E.contract_network([ket2, op, ket2, env], labels, out_l, conj=conj)
     assert torch.allclose(got, ref, rtol=0, atol=1e-13 * ref.abs().max().item())
     assert len(E._NET_CACHE) == 2
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("layout", ["abwp", "bawp"])
+def test_block_apply_staged_matches_unstaged_and_dense(gpu, dtype, layout):
+    """The operator-tensor step with rows along a 256-long innermost dimension runs
+    ptn_block_apply_staged (inputs of each 256-row tile staged through LDS); it
+    equals ptn_block_apply and the dense contraction."""
+    print(f"block apply staged {dtype} {layout}", flush=True)
+    rng = np.random.default_rng(9)
+    w = np.zeros((5, 2, 2), dtype=complex)          # (w, out, in): a few live blocks
+    w[0] = [[1, 0], [0, -1]]
+    w[2] = [[0, 1], [1, 0]]
+    w[4] = [[0.5, -1j], [1j, 0.25]]
+    W = torch.as_tensor(w).cuda()
+    Wb = E.BlockSparse.from_dense(W, [0])
+    dims = {"a": 16, "b": 256, "w": 5, "p": 2}
+    y = rng.standard_normal([dims[c] for c in layout]) + \
+        1j * rng.standard_normal([dims[c] for c in layout])
+    Y = torch.as_tensor(y).to(dtype).cuda()
+    labels = [list(layout), ["w", "q", "p"]]
+    out_l = ["a", "b", "q"]
+    prec = E.FP64 if dtype == torch.complex128 else E.FP32
+    od = torch.complex128 if dtype == torch.complex128 else torch.complex64
+    try:
+        E.USE_BLOCK_STAGED = True
+        E._BAPPLY_CACHE.clear()
+        got = E.contract_network([Y, Wb], labels, out_l, precision=prec, out_dtype=od)
+        E.USE_BLOCK_STAGED = False
+        E._BAPPLY_CACHE.clear()
+        ref = E.contract_network([Y, Wb], labels, out_l, precision=prec, out_dtype=od)
+    finally:
+        E.USE_BLOCK_STAGED = True
+        E._BAPPLY_CACHE.clear()
+    assert torch.equal(got, ref)
+    dense = np.einsum(f"{layout},wqp->abq", Y.cpu().numpy().astype(complex), w)
+    tol = 1e-13 if dtype == torch.complex128 else 1e-6
+    np.testing.assert_allclose(got.cpu().numpy(), dense, rtol=0, atol=tol * np.abs(dense).max())
