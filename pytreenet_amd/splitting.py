@@ -16,6 +16,8 @@ reference semantics) pick the kept dimension.
 """
 from __future__ import annotations
 
+from contextlib import contextmanager
+
 import ctypes
 import warnings
 from dataclasses import dataclass
@@ -224,8 +226,42 @@ def cholesky_qr3_leg(t: torch.Tensor, leg: int):
     P = E.contract(Ls[0], ["a", "b"], Ls[1], ["b", "c"], ["a", "c"])
     P = E.contract(P, ["a", "c"], Ls[2], ["c", "d"], ["a", "d"])
     R = E.permute_copy(P, [1, 0], conj=True)
+    if _DEFER.active:
+        # checked once at the end of the time step (deferred_qr_checks)
+        _DEFER.flags.append(status)
+        return q, R, True
     ok = int(status.sum().cpu().item()) == 0
     return q, R, ok
+
+
+class _DeferredChecks:
+    """Pivot statuses of the CholeskyQR3 splits of one time step, read back once."""
+
+    def __init__(self):
+        self.active = False
+        self.flags: List[torch.Tensor] = []
+
+    def failed(self) -> bool:
+        if not self.flags:
+            return False
+        return bool(torch.cat(self.flags).sum().item() != 0)
+
+
+_DEFER = _DeferredChecks()
+
+
+@contextmanager
+def deferred_qr_checks():
+    """Within the block, a CholeskyQR3 split returns its Q and R without reading
+    its pivot status back (which would stall the device queue); the statuses
+    are collected and ``failed()`` reads them all with one copy.  The caller
+    must redo the work of the block with the checks on when a split failed."""
+    prev = (_DEFER.active, _DEFER.flags)
+    _DEFER.active, _DEFER.flags = True, []
+    try:
+        yield _DEFER
+    finally:
+        _DEFER.active, _DEFER.flags = prev
 
 
 QR_STATS = {"cholqr3": 0, "householder": 0, "small": 0}

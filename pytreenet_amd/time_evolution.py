@@ -376,9 +376,53 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         raise NotImplementedError
         yield
 
+    # A time step runs with the CholeskyQR pivot checks deferred to its end (no
+    # device -> host stall after every split).  If a split turned out rank
+    # deficient, the step is restored from its start and run again with every
+    # split checked as it happens (and the Householder fallback where needed).
+    # The state, the cache and the logs are only ever rebound to new tensors,
+    # so the snapshot holds references, not copies.
+    DEFER_QR_CHECKS = True
+    steps_redone = 0
+
+    def _snapshot(self):
+        cache = self.partial_tree_cache
+        return (dict(self.state.tensors), self.state.orthogonality_center_id, cache,
+                dict(cache), len(self.apply_log), len(getattr(self, "kept_dims", [])))
+
+    def _restore(self, snap):
+        tensors, center, cache, entries, nlog, nkept = snap
+        for k, v in tensors.items():
+            self.state.tensors[k] = v
+        self.state.orthogonality_center_id = center
+        cache.clear()
+        cache.update(entries)
+        self.partial_tree_cache = cache
+        del self.apply_log[nlog:]
+        if hasattr(self, "kept_dims"):
+            del self.kept_dims[nkept:]
+
     def run_one_time_step(self, **kwargs):
         E.refresh_stream()
-        self._drive(self._time_step_gen())
+        if not self.DEFER_QR_CHECKS:
+            self._drive(self._time_step_gen())
+            return
+        from .splitting import deferred_qr_checks
+        snap = self._snapshot()
+        with deferred_qr_checks() as checks:
+            try:
+                self._drive(self._time_step_gen())
+            except Exception:
+                # work downstream of a collapsed split may fail in its own way
+                if not checks.failed():
+                    raise
+                failed = True
+            else:
+                failed = checks.failed()
+        if failed:
+            TDVPAlgorithm.steps_redone += 1
+            self._restore(snap)
+            self._drive(self._time_step_gen())
 
     def _update_site(self, node_id: str, time_step_factor: float = 1):
         return self._drive(self._update_site_gen(node_id, time_step_factor))

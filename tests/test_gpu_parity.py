@@ -260,3 +260,49 @@ def test_tdvp_block_apply_shared_structure_matches_oracle(gpu):
         for nid in ops:
             z = alg.state.single_site_operator_expectation_value(nid, PAULI_Z)
             assert abs(z - otd.expect_site(nid, PAULI_Z)) < REL * abs(otd.energy()), nid
+
+
+@pytest.mark.parametrize("padded", [False, True])
+def test_deferred_qr_checks_match_synchronous_checks(gpu, padded):
+    """A time step runs with the CholeskyQR pivot checks deferred to its end.  With
+    zero-padded (rank-deficient) bonds every CholeskyQR split of the root collapses:
+    the step is redone with synchronous checks and the Householder fallback, and
+    equals the run that checks synchronously from the start.  Full-rank states are
+    not redone and give the same result either way."""
+    _say(f"deferred QR checks padded={padded}")
+    from pytreenet_amd.time_evolution import TDVPAlgorithm
+    ids, parent, children = complete_tree(3, 2)
+    tensors = random_state_tensors(ids, parent, children, 16, 41)
+    if padded:
+        for nid, t in tensors.items():
+            t = t.copy()
+            for ax in range(t.ndim - 1):
+                idx = [slice(None)] * t.ndim
+                idx[ax] = slice(3, None)
+                t[tuple(idx)] = 0
+            tensors[nid] = t
+    from pytreenet_amd.splitting import QR_STATS
+    results = []
+    redone = []
+    collapsed = 0  # steps of the synchronous run with a CholeskyQR3 split that fell back
+    for defer in (True, False):
+        st = build_state(ids, parent, children, tensors, device="cuda")
+        H = ptn.random_xyz_model(st, 41)
+        cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV)
+        alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.1, [], config=cfg,
+                                         solver_options={"max_krylov_dim": 10})
+        alg.DEFER_QR_CHECKS = defer
+        n0 = TDVPAlgorithm.steps_redone
+        for _ in range(2):
+            h0 = QR_STATS["householder"]
+            alg.run_one_time_step()
+            # the only tensor above the one-launch QR's size is the root, which is tall
+            collapsed += (not defer) and QR_STATS["householder"] > h0
+        redone.append(TDVPAlgorithm.steps_redone - n0)
+        results.append({k: v.cpu().numpy() for k, v in alg.state.tensors.items()})
+    assert redone[1] == 0
+    assert redone[0] == collapsed
+    assert (collapsed >= 1) == padded
+    for k in ids:
+        np.testing.assert_allclose(results[0][k], results[1][k], rtol=0,
+                                   atol=REL * np.abs(results[1][k]).max())
