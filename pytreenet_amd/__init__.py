@@ -2,13 +2,13 @@
 
 The public names mirror the reference package's star exports
 (``pytreenet/__init__.py``) for the time-evolution path: TTNS/TTNO objects,
-the environment cache and contractions, the Krylov exponential, QR/SVD splits
-and the TDVP algorithms.  All arithmetic runs in libptnhip.so (HIP, gfx950);
+the environment cache and contractions, the Krylov exponential, QR/SVD splits,
+the TDVP algorithms and TEBD.  All arithmetic runs in libptnhip.so (HIP, gfx950);
 there is no CPU fallback.
 """
 from ._lib import PtnBackendError
 from .ttn import Node, TreeTensorNetwork, TreeTensorNetworkState, TTNS
-from .ttno import (TensorProduct, TreeTensorNetworkOperator, TTNO, pauli_matrices,
+from .ttno import (NumericOperator, TensorProduct, TreeTensorNetworkOperator, TTNO, pauli_matrices,
                    transverse_field_ising, xyz_model, xxz_model, random_xyz_model,
                    random_field_heisenberg)
 from .contractions import (PartialTreeCachDict, SandwichCache, contract_any,
@@ -22,7 +22,8 @@ from .splitting import (SplitMode, SVDParameters, ContractionMode, tensor_qr_dec
 from .time_evolution import (TDVPConfig, TwoSiteTDVPConfig, TDVPAlgorithm, OneSiteTDVP,
                              FirstOrderOneSiteTDVP, SecondOrderOneSiteTDVP, TwoSiteTDVP,
                              SecondOrderTwoSiteTDVP, Results, find_orthogonalisation_path,
-                             BatchedTDVP)
+                             BatchedTDVP, TTNTimeEvolution)
+from .tebd import TEBD, SWAPlist, TrotterSplitting, TrotterStep, swap_gate
 from .random_states import random_tree_state, complete_tree, build_state
 from . import engine
 

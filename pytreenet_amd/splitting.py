@@ -447,3 +447,84 @@ def scale_leg(t: torch.Tensor, leg: int, s: np.ndarray) -> torch.Tensor:
     lo = list(range(t.dim()))
     lo[leg] = "new"
     return E.contract(d, ["new", "old"], t, lt, lo)
+
+
+# ---------------------------------------------------------------------------------------
+# two-site merge and truncated split (two-site TDVP, TEBD)
+# ---------------------------------------------------------------------------------------
+def merge_two_sites(ttn, n1: str, n2: str) -> torch.Tensor:
+    """The tensors of neighbours n1 and n2 contracted over their shared bond.
+
+    Leg order of the result: n1's other virtual legs in n1's order, n2's other
+    virtual legs in n2's order, then n1's and n2's physical legs.  This is the
+    merged node of ``TTN.contract_nodes`` (``core/ttn.py:1072-1115``) with the
+    open legs last in (n1, n2) order, as ``legs_before_combination``
+    (``core/ttn.py:1163-1210``) records them; the virtual legs keep each node's
+    own order so that ``split_two_sites`` can put every leg back in place."""
+    A1, A2 = ttn.tensors[n1], ttn.tensors[n2]
+    i12 = ttn.nodes[n1].neighbour_index(n2)
+    i21 = ttn.nodes[n2].neighbour_index(n1)
+    l1 = [("o1", i) for i in range(A1.dim() - 1)] + ["p1"]
+    l1[i12] = "bond"
+    l2 = [("o2", i) for i in range(A2.dim() - 1)] + ["p2"]
+    l2[i21] = "bond"
+    outer1 = [x for x in l1[:-1] if x != "bond"]
+    outer2 = [x for x in l2[:-1] if x != "bond"]
+    return E.contract(A1, l1, A2, l2, outer1 + outer2 + ["p1", "p2"])
+
+
+def split_two_sites(ttn, x: torch.Tensor, n1: str, n2: str, svd_params,
+                    contr_mode: "ContractionMode" = None) -> np.ndarray:
+    """Split a merged two-site tensor (legs as ``merge_two_sites`` returns them)
+    back into n1 and n2 by a truncated SVD, and write both node tensors.
+
+    U goes to n1 and S Vh to n2 (``ContractionMode.VCONTR``, the default of
+    ``contr_truncated_svd_splitting``, ``util/tensor_splitting.py:541-572``, as
+    used by ``TTN.split_node_svd``, ``core/ttn.py:1476-1497``); UCONTR puts S on
+    U, EQUAL splits sqrt(S) over both.  The new bond takes the place of the old
+    one in each node.  Returns the kept singular values (host)."""
+    if contr_mode is None:
+        contr_mode = ContractionMode.VCONTR
+    A1, A2 = ttn.tensors[n1], ttn.tensors[n2]
+    i12 = ttn.nodes[n1].neighbour_index(n2)
+    i21 = ttn.nodes[n2].neighbour_index(n1)
+    k1, k2 = A1.dim() - 2, A2.dim() - 2
+    dims = list(x.shape)
+    udims = dims[:k1] + [dims[k1 + k2]]
+    vdims = dims[k1:k1 + k2] + [dims[k1 + k2 + 1]]
+    u_legs = list(range(k1)) + [k1 + k2]
+    v_legs = list(range(k1, k1 + k2)) + [k1 + k2 + 1]
+    m, n = int(np.prod(udims)), int(np.prod(vdims))
+    panel = _colmajor_panel(x, u_legs, v_legs)
+    svals, U, Vh, kfull = svd_colmajor(panel, m, n)
+    s_new, _ = truncate_singular_values(svals, svd_params)
+    r = len(s_new)
+    dev = x.device
+    s_u = s_v = None
+    if contr_mode == ContractionMode.VCONTR:
+        s_v = s_new
+    elif contr_mode == ContractionMode.UCONTR:
+        s_u = s_new
+    else:
+        s_u = s_v = np.sqrt(s_new)
+    # U (m x kfull, column-major) -> n1's layout with the new bond at i12
+    shape1 = [A1.shape[i] for i in range(A1.dim())]
+    shape1[i12] = r
+    new1 = torch.empty(shape1, dtype=torch.complex128, device=dev)
+    st1 = list(new1.stride())
+    dst_u = [st1[i12]] + [st1[j] for j in range(A1.dim()) if j != i12]
+    E.strided_copy([r] + udims, U, [m] + _cstrides(udims), new1, dst_u)
+    if s_u is not None:
+        new1 = scale_leg(new1, i12, s_u)
+    # Vh (kfull x n, column-major) -> n2's layout with the new bond at i21
+    vh_view = torch.as_strided(Vh, [r] + vdims, [1] + [kfull * c for c in _cstrides(vdims)])
+    lv = ["old"] + [("v", i) for i in range(len(vdims))]
+    lo = [("v", i) for i in range(len(vdims) - 1)]
+    lo.insert(i21, "new")
+    lo.append(("v", len(vdims) - 1))
+    scale = s_v if s_v is not None else np.ones(r)
+    sdiag = torch.as_tensor(np.diag(np.asarray(scale, dtype=np.complex128))).to(dev)
+    new2 = E.contract(sdiag, ["new", "old"], vh_view, lv, lo)
+    ttn.tensors[n1] = new1
+    ttn.tensors[n2] = new2
+    return np.asarray(s_new)

@@ -35,8 +35,7 @@ from .contractions import (SandwichCache, bond_apply, contract_any,
                            operator_tensor, scalar_product)
 from . import krylov as _K
 from .krylov import EvoDirection, TimeEvoMode, krylov_exp_action
-from .splitting import (SVDParameters, SplitMode, _colmajor_panel, _cstrides, qr_colmajor,
-                        svd_colmajor, truncate_singular_values)
+from .splitting import SVDParameters, SplitMode, merge_two_sites, split_two_sites
 from .ttn import TreeTensorNetworkState
 from .ttno import TensorProduct, TreeTensorNetworkOperator
 
@@ -142,6 +141,10 @@ class TimeEvolution:
         return {"0": operators}
 
     @property
+    def initial_state(self):
+        return self._initial_state
+
+    @property
     def time_step_size(self) -> float:
         return self._time_step_size
 
@@ -180,14 +183,10 @@ class TimeEvolution:
                 self.results.set_time(idx[i], i * self._time_step_size)
 
 
-class TTNOBasedTimeEvolution(TimeEvolution):
+class TTNTimeEvolution(TimeEvolution):
+    """Time evolution of a TTNS with operator evaluation and the optional
+    bond-dimension / norm / Loschmidt records (``time_evolution/ttn_time_evolution.py:40-179``)."""
     config_class = TTNTimeEvolutionConfig
-
-    def __init__(self, initial_state, hamiltonian, time_step_size, final_time, operators,
-                 config=None, solver_options=None):
-        super().__init__(initial_state, time_step_size, final_time, operators,
-                         solver_options=solver_options, config=config)
-        self.hamiltonian = hamiltonian
 
     def evaluate_operator(self, operator) -> complex:
         return self.state.operator_expectation_value(operator)
@@ -202,6 +201,14 @@ class TTNOBasedTimeEvolution(TimeEvolution):
         if getattr(cfg, "record_loschmidt_amplitude", False):
             self.results.set_element("loschmidt_amplitude", index,
                                      scalar_product(self.state, self._initial_state))
+
+
+class TTNOBasedTimeEvolution(TTNTimeEvolution):
+    def __init__(self, initial_state, hamiltonian, time_step_size, final_time, operators,
+                 config=None, solver_options=None):
+        super().__init__(initial_state, time_step_size, final_time, operators,
+                         solver_options=solver_options, config=config)
+        self.hamiltonian = hamiltonian
 
 
 # ---------------------------------------------------------------------------------------
@@ -612,52 +619,14 @@ class TwoSiteTDVP(TDVPAlgorithm):
         s = self.state
         n1, n2 = target_node_id, next_node_id
         assert s.orthogonality_center_id == n1
-        i12 = s.nodes[n1].neighbour_index(n2)
-        i21 = s.nodes[n2].neighbour_index(n1)
-        A1, A2 = s.tensors[n1], s.tensors[n2]
-        k1, k2 = A1.dim() - 2, A2.dim() - 2
-        l1 = [("o1", i) for i in range(k1 + 1)] + ["p1"]
-        l1[i12] = "bond"
-        l1 = [x for x in l1]
-        l2 = [("o2", i) for i in range(k2 + 1)] + ["p2"]
-        l2[i21] = "bond"
-        outer1 = [x for x in l1[:-1] if x != "bond"]
-        outer2 = [x for x in l2[:-1] if x != "bond"]
-        x = E.contract(A1, l1, A2, l2, outer1 + outer2 + ["p1", "p2"])
+        x = merge_two_sites(s, n1, n2)
         make, _, _ = self._two_site_action(n1, n2)
         x = yield from self._evolve(x, make(x), self.time_step_size * time_step_factor,
                                     EvoDirection.FORWARD)
         self.apply_log.append(("two_site", (n1, n2), self._last_applies))
         # truncated SVD, S absorbed into V (VCONTR)
-        dims = list(x.shape)
-        udims = dims[:k1] + [dims[k1 + k2]]
-        vdims = dims[k1:k1 + k2] + [dims[k1 + k2 + 1]]
-        u_legs = list(range(k1)) + [k1 + k2]
-        v_legs = list(range(k1, k1 + k2)) + [k1 + k2 + 1]
-        m, n = int(np.prod(udims)), int(np.prod(vdims))
-        panel = _colmajor_panel(x, u_legs, v_legs)
-        svals, U, Vh, kfull = svd_colmajor(panel, m, n)
-        s_new, _ = truncate_singular_values(svals, self.config)
-        r = len(s_new)
-        self.kept_dims.append(r)
-        dev = x.device
-        # U (m x kfull col-major) -> node 1 layout with the new bond at i12
-        shape1 = [A1.shape[i] for i in range(A1.dim())]
-        shape1[i12] = r
-        new1 = torch.empty(shape1, dtype=torch.complex128, device=dev)
-        st1 = list(new1.stride())
-        dst_u = [st1[i12]] + [st1[j] for j in range(A1.dim()) if j != i12]
-        E.strided_copy([r] + udims, U, [m] + _cstrides(udims), new1, dst_u)
-        # S Vh (kfull x n col-major) -> node 2 layout with the new bond at i21
-        vh_view = torch.as_strided(Vh, [r] + vdims, [1] + [kfull * c for c in _cstrides(vdims)])
-        sdiag = torch.as_tensor(np.diag(np.asarray(s_new, dtype=np.complex128))).to(dev)
-        lv = ["old"] + [("v", i) for i in range(len(vdims))]
-        lo = [("v", i) for i in range(len(vdims) - 1)]
-        lo.insert(i21, "new")
-        lo.append(("v", len(vdims) - 1))
-        new2 = E.contract(sdiag, ["new", "old"], vh_view, lv, lo)
-        s.tensors[n1] = new1
-        s.tensors[n2] = new2
+        s_new = split_two_sites(s, x, n1, n2, self.config)
+        self.kept_dims.append(len(s_new))
         s.orthogonality_center_id = n2
         self.update_tree_cache(n1, n2)
 

@@ -40,8 +40,70 @@ def pauli_matrices() -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
     return PAULI_X, PAULI_Y, PAULI_Z
 
 
+class NumericOperator:
+    """An operator on one or more nodes (``operators/operator.py:20-108``).
+
+    ``operator`` is a host array: a matrix (out, in) or a tensor with legs
+    (out_1..out_n, in_1..in_n), the legs in ``node_identifiers`` order."""
+
+    def __init__(self, operator: np.ndarray, node_identifiers):
+        operator = np.asarray(operator)
+        if operator.ndim % 2 != 0:
+            raise ValueError("an operator needs as many output legs as input legs")
+        self.operator = operator
+        self.node_identifiers = ([node_identifiers] if isinstance(node_identifiers, str)
+                                 else list(node_identifiers))
+
+    def to_matrix(self) -> "NumericOperator":
+        half = self.operator.ndim // 2
+        rows = int(np.prod(self.operator.shape[:half]))
+        return NumericOperator(self.operator.reshape(rows, -1), list(self.node_identifiers))
+
+    def to_tensor(self, dim: Optional[int] = None, ttn=None) -> "NumericOperator":
+        """The operator with one output and one input leg per node: every node of
+        dimension ``dim``, or each node's open dimension in ``ttn``."""
+        if dim is None and ttn is None:
+            raise ValueError("`dim` and `ttn` cannot both be `None`!")
+        if dim is not None:
+            if dim <= 0:
+                raise ValueError("dimension has to be positive")
+            shape = [dim] * len(self.node_identifiers)
+        else:
+            shape = [ttn.nodes[n].open_dimension() for n in self.node_identifiers]
+        return NumericOperator(self.operator.reshape(shape * 2), list(self.node_identifiers))
+
+    def is_unitary(self) -> bool:
+        m = self.to_matrix().operator
+        return bool(np.allclose(m @ m.conj().T, np.eye(m.shape[0])))
+
+
 class TensorProduct(dict):
-    """{node_id: operator matrix} -- a product of single-site operators."""
+    """{node_id: operator matrix} -- a product of single-site operators
+    (``operators/tensorproduct.py``)."""
+
+    def into_operator(self, conversion_dict: Optional[Dict[str, np.ndarray]] = None,
+                      order: Optional[List[str]] = None) -> NumericOperator:
+        """The Kronecker product of the factors, the first identifier most significant
+        (``tensorproduct.py:170-205``).  Symbolic (string) factors are looked up in
+        ``conversion_dict``."""
+        order = list(self.keys()) if order is None else list(order)
+        total = np.ones((1, 1), dtype=complex)
+        for nid in order:
+            op = self[nid]
+            if isinstance(op, str):
+                if conversion_dict is None:
+                    raise TypeError("If the tensor_product contains symbolic operators, "
+                                    "there must be a dictionary for conversion!")
+                op = conversion_dict[op]
+            total = np.kron(total, np.asarray(op))
+        return NumericOperator(total, list(self.keys()))
+
+    def exp(self, factor=1) -> NumericOperator:
+        """exp(factor * product) as a matrix on the nodes in key order
+        (``tensorproduct.py:207-224``)."""
+        from scipy.linalg import expm
+        total = self.into_operator()
+        return NumericOperator(expm(factor * total.operator), total.node_identifiers)
 
 
 class TreeTensorNetworkOperator(TreeTensorNetwork):
