@@ -286,18 +286,27 @@ def qr_split_small(t: torch.Tensor, leg: int) -> Tuple[torch.Tensor, torch.Tenso
     return q, r
 
 
-def qr_split_to_neighbour(ttn, node_id: str, neighbour_id: str):
-    """QR(KEEP) of a node off its leg to ``neighbour_id``.
+def qr_split_to_neighbour(ttn, node_id: str, neighbour_id: str, mode=SplitMode.KEEP):
+    """QR of a node off its leg to ``neighbour_id``.
 
     Returns (Q in the node's canonical layout with the new bond at the old
     leg's position, R with legs (new, old)).  Small tensors run the one-launch
     Householder kernel; larger panels with rows >= columns use shifted
-    CholeskyQR3, wide or rank-deficient ones Householder QR.
+    CholeskyQR3, wide or rank-deficient ones Householder QR.  The modes differ
+    only for a bond wider than the product m of the node's other dimensions:
+    KEEP (also for ``mode=None``) zero-pads Q and R to the old bond, REDUCED
+    shrinks the bond to m.
     """
     t = ttn.tensors[node_id]
     leg = ttn.nodes[node_id].neighbour_index(neighbour_id)
     n = t.shape[leg]
     m = t.numel() // n
+    if mode is SplitMode.REDUCED and m < n:
+        others = [i for i in range(t.dim()) if i != leg]
+        q, r = tensor_qr_decomposition(t, others, [leg], mode=SplitMode.REDUCED)
+        perm = list(range(q.dim() - 1))
+        perm.insert(leg, q.dim() - 1)
+        return E.permute_copy(q, perm), r
     if t.numel() <= QR_SMALL_MAX and n <= 256 and t.dim() <= 8:
         QR_STATS["small"] += 1
         return qr_split_small(t, leg)

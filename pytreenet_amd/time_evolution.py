@@ -266,6 +266,42 @@ def find_orthogonalisation_path(update_path: List[str], ttn,
 # ---------------------------------------------------------------------------------------
 # TDVP
 # ---------------------------------------------------------------------------------------
+# FASTEST evolves with the dense effective Hamiltonian (like the reference) up to
+# this local dimension and with the Krylov recurrence above it, where a dense
+# matrix (dim^2 complex) is out of reach
+DENSE_MAX_DIM = 4096
+
+
+def local_time_evolution(x: torch.Tensor, action, dt: float, direction: EvoDirection,
+                         mode: TimeEvoMode, solver_options: Optional[Dict[str, Any]] = None,
+                         dense_max_dim: int = DENSE_MAX_DIM) -> torch.Tensor:
+    """exp(f dt H_eff) x for a site, bond or two-site action ``action``
+    (``time_evo_util/effective_time_evolution.py:28-140``).
+
+    KRYLOV (and FASTEST above ``dense_max_dim``) runs the matrix-free Krylov
+    exponential; the dense modes build H_eff as a matrix (the network without x,
+    ``effective_hamiltonians.py:20-470``) and exponentiate it exactly
+    (``time_evolution.py:617-655``, ``util/std_utils.py:134-175``).
+    """
+    if mode is TimeEvoMode.FASTEST and x.numel() > dense_max_dim:
+        mode = TimeEvoMode.KRYLOV
+    opts = dict(TimeEvoMode.default_solver_options(mode))
+    opts.update(solver_options or {})
+    if mode is TimeEvoMode.KRYLOV:
+        if x.numel() <= _K.SMALL_N and isinstance(action, E.CompiledNetwork):
+            # short local spaces: the recurrence runs in one launch on the dense
+            # H_eff (one contraction instead of one network replay per step)
+            y = _K.small_time_evolve(x, E.dense_operator(action, transposed=True), dt,
+                                     direction, opts, transposed=True)
+            if y is not None:
+                return y
+        return mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
+                                       **opts)
+    H = E.dense_operator(action)
+    return mode.time_evolve(x, H, dt, forward=direction,
+                            **{k: v for k, v in opts.items() if k == "exp_factor"})
+
+
 class KrylovRequest:
     """One pending exp(f dt H) x of a sweep: x, the H action, dt, direction."""
     __slots__ = ("x", "action", "dt", "direction", "result", "applies")
@@ -319,37 +355,11 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
     def update_tree_cache(self, node_id: str, next_node_id: str):
         self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
 
-    # FASTEST evolves with the dense effective Hamiltonian (like the reference) up to
-    # this local dimension and with the Krylov recurrence above it, where a dense
-    # matrix (dim^2 complex) is out of reach
     DENSE_MAX_DIM = 4096
 
     def _krylov(self, x: torch.Tensor, action, dt: float, direction: EvoDirection):
-        """exp(f dt H_eff) x for the site, bond or two-site action ``action``.
-
-        KRYLOV (and FASTEST above DENSE_MAX_DIM) runs the matrix-free Krylov
-        exponential; the dense modes build H_eff as a matrix (the network without x,
-        ``effective_hamiltonians.py:20-470``) and exponentiate it exactly
-        (``time_evolution.py:617-655``, ``util/std_utils.py:134-175``).
-        """
-        mode = self._mode
-        if mode is TimeEvoMode.FASTEST and x.numel() > self.DENSE_MAX_DIM:
-            mode = TimeEvoMode.KRYLOV
-        opts = dict(TimeEvoMode.default_solver_options(mode))
-        opts.update(self.solver_options)
-        if mode is TimeEvoMode.KRYLOV:
-            if x.numel() <= _K.SMALL_N and isinstance(action, E.CompiledNetwork):
-                # short local spaces: the recurrence runs in one launch on the dense
-                # H_eff (one contraction instead of one network replay per step)
-                y = _K.small_time_evolve(x, E.dense_operator(action, transposed=True), dt,
-                                         direction, opts, transposed=True)
-                if y is not None:
-                    return y
-            return mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
-                                           **opts)
-        H = E.dense_operator(action)
-        return mode.time_evolve(x, H, dt, forward=direction,
-                                **{k: v for k, v in opts.items() if k == "exp_factor"})
+        return local_time_evolution(x, action, dt, direction, self._mode, self.solver_options,
+                                    self.DENSE_MAX_DIM)
 
     def krylov_options(self) -> Dict[str, Any]:
         opts = dict(TimeEvoMode.default_solver_options(self._mode))

@@ -29,6 +29,7 @@ import torch
 
 from . import engine as E
 from ._lib import PtnBackendError
+from .splitting import SplitMode
 
 
 def default_device() -> torch.device:
@@ -342,28 +343,48 @@ class TreeTensorNetwork:
         return self._nodes[b].parent == a
 
     # -- gauge moves (core/canonical_form.py, core/ttn.py:1530-1582) -----------------
-    def _qr_towards(self, nid: str, nb: str):
+    def _qr_towards(self, nid: str, nb: str, mode=None):
         from .splitting import qr_split_to_neighbour, absorb_into_neighbour
-        q, r = qr_split_to_neighbour(self, nid, nb)
+        q, r = qr_split_to_neighbour(self, nid, nb, mode=mode)
         self._tensors[nid] = q
         absorb_into_neighbour(self, nb, nid, r)
 
-    def canonical_form(self, orthogonality_center_id: str, mode=None):
-        """QR-canonicalise towards ``orthogonality_center_id`` (KEEP padding)."""
+    def canonical_form(self, orthogonality_center_id: str, mode=SplitMode.REDUCED):
+        """QR-canonicalise towards ``orthogonality_center_id``
+        (``core/canonical_form.py``; ``core/ttn.py:1673-1697``: a state that already
+        has a centre only moves it there).  ``mode`` is the QR mode of every
+        split: REDUCED (the default, as in the reference) lets a bond shrink to
+        the product of the node's other dimensions, KEEP keeps every bond."""
+        if self.orthogonality_center_id is not None:
+            self.move_orthogonalization_center(orthogonality_center_id, mode=mode)
+            return
         dist = self.distance_to_node(orthogonality_center_id)
         for d in range(max(dist.values()), 0, -1):
             for nid in [n for n in dist if dist[n] == d]:
                 nb = min(self.neighbours(nid), key=lambda x: dist[x])
-                self._qr_towards(nid, nb)
+                self._qr_towards(nid, nb, mode)
         self.orthogonality_center_id = orthogonality_center_id
 
-    def move_orthogonalization_center(self, new_center_id: str, mode=None):
+    def move_orthogonalization_center(self, new_center_id: str, mode=SplitMode.REDUCED):
+        """QR splits along the path from the current centre (``core/ttn.py:1530-1582``);
+        ``mode`` as in ``canonical_form``."""
         if self.orthogonality_center_id is None:
             raise AssertionError("The TTN is not in canonical form!")
         path = self.path_from_to(self.orthogonality_center_id, new_center_id)
         for a, b in zip(path[:-1], path[1:]):
-            self._qr_towards(a, b)
+            self._qr_towards(a, b, mode)
         self.orthogonality_center_id = new_center_id
+
+    def ensure_root_orth_center(self, mode=SplitMode.REDUCED) -> bool:
+        """Make the root the orthogonality centre (``core/ttn.py:1631-1645``);
+        True when it already was."""
+        if self.orthogonality_center_id == self._root_id:
+            return True
+        if self.orthogonality_center_id is None:
+            self.canonical_form(self._root_id, mode=mode)
+        else:
+            self.move_orthogonalization_center(self._root_id, mode=mode)
+        return False
 
     def bond_dims(self) -> Dict[Tuple[str, str], int]:
         out = {}
