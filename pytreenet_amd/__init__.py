@@ -3,7 +3,7 @@
 The public names mirror the reference package's star exports
 (``pytreenet/__init__.py``) for the time-evolution path: TTNS/TTNO objects,
 the environment cache and contractions, the Krylov exponential, QR/SVD splits,
-the TDVP algorithms, TEBD and the BUG integrators.  All arithmetic runs in libptnhip.so (HIP, gfx950);
+the TDVP algorithms, TEBD, the BUG integrators and DMRG.  All arithmetic runs in libptnhip.so (HIP, gfx950);
 there is no CPU fallback.
 """
 from ._lib import PtnBackendError
@@ -26,6 +26,7 @@ from .time_evolution import (TDVPConfig, TwoSiteTDVPConfig, TDVPAlgorithm, OneSi
 from .tebd import TEBD, SWAPlist, TrotterSplitting, TrotterStep, swap_gate
 from .bug import (BUG, FixedBUG, BUGConfig, FixedBUGConfig, CommonBUGConfig, svd_truncation,
                   find_path_leaf_to_leaf)
+from .dmrg import DMRGAlgorithm
 from .random_states import random_tree_state, complete_tree, build_state
 from . import engine
 

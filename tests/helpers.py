@@ -4,7 +4,11 @@ from __future__ import annotations
 import numpy as np
 
 from oracle.tree import Topology
-from pytreenet_amd.random_states import complete_tree, random_state_tensors
+from oracle import dense as OD
+from oracle import bug as OB
+from oracle.tdvp import OracleState
+from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
+from pytreenet_amd.ttno import build_edge_ttno_tensors
 
 
 def topology_from_spec(ids, parent, children) -> Topology:
@@ -38,3 +42,37 @@ def small_tree(branching=2, depth=2, chi=4, seed=7):
     ids, parent, children = complete_tree(branching, depth)
     tensors = random_state_tensors(ids, parent, children, chi, seed)
     return ids, parent, children, tensors
+
+
+X = np.array([[0, 1], [1, 0]], dtype=complex)
+Y = np.array([[0, -1j], [1j, 0]], dtype=complex)
+Z = np.array([[1, 0], [0, -1]], dtype=complex)
+
+
+def _xyz_terms(topo, seed):
+    rng = np.random.default_rng(seed)
+    site = {n: rng.uniform(-1, 1) * Z for n in topo.node_ids}
+    edges = []
+    for n in topo.node_ids:
+        p = topo.parent[n]
+        if p is not None:
+            j = rng.uniform(-1, 1, 3)
+            edges += [(p, n, -j[0], X, X), (p, n, -j[1], Y, Y), (p, n, -j[2], Z, Z)]
+    return site, edges
+
+
+def oracle_problem(branching, depth, chi, seed):
+    """(topology, normalised oracle state with the root as centre, product-built TTNO
+    tensors, dense Hamiltonian) of a random XYZ model with random fields on a
+    complete tree."""
+    ids, parent, children = complete_tree(branching, depth)
+    tensors = random_state_tensors(ids, parent, children, chi, seed)
+    topo = topology_from_spec(ids, parent, children)
+    site, edges = _xyz_terms(topo, seed)
+    st = build_state(ids, parent, children, tensors, device="cpu")
+    W = build_edge_ttno_tensors(st, site, edges)
+    H = OD.edge_hamiltonian(topo, site, edges)
+    ost = OracleState(topo, tensors)
+    ost.canonical_form(topo.root)
+    ost.normalise()
+    return topo, ost, W, H

@@ -19,7 +19,7 @@ from oracle.tree import Topology
 from pytreenet_amd.bug import find_path_leaf_to_leaf
 from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
 from pytreenet_amd.ttno import build_edge_ttno_tensors
-from tests.helpers import topology_from_spec
+from tests.helpers import oracle_problem
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 X = np.array([[0, 1], [1, 0]], dtype=complex)
@@ -48,37 +48,11 @@ def test_leaf_to_leaf_paths_match_reference_kats(case):
     assert topo.path_from_to(*case["diameter_leaves"]) == case["main_path"]
 
 
-def _model(topo, seed):
-    rng = np.random.default_rng(seed)
-    site = {n: rng.uniform(-1, 1) * Z for n in topo.node_ids}
-    edges = []
-    for n in topo.node_ids:
-        p = topo.parent[n]
-        if p is not None:
-            j = rng.uniform(-1, 1, 3)
-            edges += [(p, n, -j[0], X, X), (p, n, -j[1], Y, Y), (p, n, -j[2], Z, Z)]
-    return site, edges
-
-
-def _setup(branching, depth, chi, seed):
-    ids, parent, children = complete_tree(branching, depth)
-    tensors = random_state_tensors(ids, parent, children, chi, seed)
-    topo = topology_from_spec(ids, parent, children)
-    site, edges = _model(topo, seed)
-    st = build_state(ids, parent, children, tensors, device="cpu")
-    W = build_edge_ttno_tensors(st, site, edges)
-    H = OD.edge_hamiltonian(topo, site, edges)
-    ost = OracleState(topo, tensors)
-    OB.move_center(ost, topo.root, keep=True) if ost.center else ost.canonical_form(topo.root)
-    ost.normalise()
-    return topo, ost, W, H
-
-
 @pytest.mark.parametrize("fixed", [False, True])
 def test_oracle_bug_exact_at_full_bond_dims(fixed):
     """Complete bases (7-site binary tree, bonds at their caps): both integrators
     equal exact evolution."""
-    topo, st, W, H = _setup(2, 2, 64, 5)
+    topo, st, W, H = oracle_problem(2, 2, 64, 5)
     psi0 = OD.state_to_vector(topo, st.A)
     for _ in range(2):
         st = OB.bug_time_step(st, W, 0.05, dict(max_bond_dim=100, rel_tol=1e-15,
@@ -92,7 +66,7 @@ def test_oracle_bug_galerkin_step_conserves_norm_and_energy():
     """Below full rank the rank-adaptive Galerkin step (before truncation) stays in
     a subspace that contains the initial state: norm and energy are conserved,
     and the bonds grow (at most doubled)."""
-    topo, st, W, H = _setup(2, 2, 2, 7)    # 7 sites, level-1 bonds 2 of 8
+    topo, st, W, H = oracle_problem(2, 2, 2, 7)    # 7 sites, level-1 bonds 2 of 8
     psi0 = OD.state_to_vector(topo, st.A)
     e0 = np.vdot(psi0, H @ psi0).real
     out = OB.bug_step(st, W, 0.1)
