@@ -162,7 +162,7 @@ def test_batched_fill_permute(gpu):
         assert torch.all(d == (0.5 - 2.0j))
 
 
-def _instances(count, depth, chis, seed0, two_site=True, precision="fp64"):
+def _instances(count, depth, chis, seed0, two_site=True, precision="fp64", max_bond_dim=12):
     ids, parent, children = complete_tree(2, depth)
     algs = []
     for b in range(count):
@@ -172,7 +172,8 @@ def _instances(count, depth, chis, seed0, two_site=True, precision="fp64"):
                          device="cuda")
         H = random_field_heisenberg(st, 2.0, seed=seed0 + 100 + b)
         if two_site:
-            cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, max_bond_dim=12,
+            cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV,
+                                        max_bond_dim=max_bond_dim,
                                         rel_tol=1e-10, total_tol=1e-10, precision=precision)
             cls = ptn.SecondOrderTwoSiteTDVP
         else:
@@ -214,3 +215,21 @@ def test_batched_tdvp_matches_single(gpu, two_site):
         if two_site:
             assert a.kept_dims == b.kept_dims
         assert [x[2] for x in a.apply_log] == [x[2] for x in b.apply_log]
+
+
+def test_batched_config4_shape_matches_single(gpu):
+    """The batching configuration's instance shape: 31-node binary tree,
+    random-field Heisenberg (W=2), second-order two-site TDVP with truncation at
+    1e-10 and chi_max=128, starting at bond dimension 8 so the bonds grow unevenly:
+    three instances in lockstep equal their single runs after one step."""
+    _say("batched config C4 shape")
+    single = _instances(3, 4, [8], 41, True, max_bond_dim=128)
+    batch = _instances(3, 4, [8], 41, True, max_bond_dim=128)
+    for a in single:
+        a.run_one_time_step()
+    ptn.BatchedTDVP(batch).run(1)
+    for a, b in zip(single, batch):
+        oa, ob = _observables(a), _observables(b)
+        np.testing.assert_allclose(ob, oa, rtol=0, atol=1e-12 * max(1.0, np.abs(oa).max()))
+        assert a.kept_dims == b.kept_dims
+        assert max(a.kept_dims) > 8

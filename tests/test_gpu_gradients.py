@@ -122,3 +122,50 @@ def test_autograd_function_and_descent(gpu):
     np.testing.assert_allclose(Jt.grad.cpu().numpy(), model.coupling_gradient(gop), atol=1e-12)
     hist = G.ground_state_descent(st, model.ttno(J), steps=20, lr=0.05)
     assert hist[-1] < hist[0] - 1e-3
+
+
+def test_config5_full_size_gradients(gpu):
+    """The differentiation configuration at full size: 63-node binary tree, chi=128,
+    TFIM with per-edge couplings.  dE/dJ_e equals -<X_p X_c>/<psi|psi> (evaluated as
+    tensor-product expectation values, another contraction path) and central finite
+    differences (h=1e-6) of the device energy; sampled dE/dA entries equal central
+    differences of the device energy."""
+    _say("config C5 full size gradients")
+    st, model, J, _ = _problem(seed=11, chi=128, depth=5)
+    assert max(st.bond_dims().values()) == 128 and len(st.nodes) == 63
+    H = model.ttno(J)
+    energy, gsite, gop, den = G.rayleigh_quotient_gradients(st, H)
+    gJ = model.coupling_gradient(gop)
+    nrm = ptn.scalar_product(st, st).real
+
+    def e_of(state, ham):
+        return (ptn.expectation_value(state, ham) / ptn.scalar_product(state, state)).real
+
+    assert abs(e_of(st, H) - energy) < 1e-12 * abs(energy)
+    edges = [(n.parent, nid) for nid, n in st.nodes.items() if n.parent is not None]
+    h = 1e-6
+    for e in (0, 17, 61):
+        p, c = edges[e]
+        xx = ptn.TensorProduct({p: PAULI_X, c: PAULI_X})
+        ref = -(st.operator_expectation_value(xx) / nrm).real
+        assert abs(gJ[e] - ref) < 1e-10 * max(1, abs(ref)), e
+        Jp, Jm = J.copy(), J.copy()
+        Jp[e] += h
+        Jm[e] -= h
+        fd = (e_of(st, model.ttno(Jp)) - e_of(st, model.ttno(Jm))) / (2 * h)
+        assert abs(fd - gJ[e]) < 1e-6 * max(1, abs(gJ[e])), (e, fd, gJ[e])
+    rng = np.random.default_rng(1)
+    for nid in ("site0", "site3", "site30"):
+        t0 = st.tensors[nid]
+        g = gsite[nid]
+        idx = tuple(int(rng.integers(0, s)) for s in t0.shape)
+        for part, expect in ((1.0, 2 * g[idx].real.item()), (1j, 2 * g[idx].imag.item())):
+            vals = []
+            for sgn in (1, -1):
+                t = t0.clone()
+                t[idx] += sgn * h * part
+                st.tensors[nid] = t
+                vals.append(e_of(st, H))
+            st.tensors[nid] = t0
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - expect) < 1e-6 * max(1.0, abs(expect)), (nid, idx, fd, expect)

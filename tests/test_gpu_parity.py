@@ -306,3 +306,70 @@ def test_deferred_qr_checks_match_synchronous_checks(gpu, padded):
     for k in ids:
         np.testing.assert_allclose(results[0][k], results[1][k], rtol=0,
                                    atol=REL * np.abs(results[1][k]).max())
+
+
+def test_config2_tree_matches_oracle_at_reduced_bond_dim(gpu):
+    """The 63-node binary tree of the plain FP64 configuration (XXZ, Jxy=1, Jz=0.8,
+    second-order one-site TDVP, dt=0.02) at bond dimension 8, where the oracle runs
+    in seconds: energies, overlaps with the initial state and <Z_i> after each of 3
+    steps, to 1e-12."""
+    _say("config C2 tree, chi 8, vs oracle")
+    alg, otd, ops = _run_pair("second1", 2, 5, 8, 63, "xxz", 0.02, 3)
+    psi0 = {k: v.copy() for k, v in otd.s.A.items()}
+    st0 = alg.state.copy()
+    for step in range(3):
+        alg.run_one_time_step()
+        otd.run_one_time_step()
+        e_ref = otd.energy()
+        assert _rel(ptn.expectation_value(alg.state, alg.hamiltonian), e_ref) < REL, step
+        ov = ptn.scalar_product(st0, alg.state)
+        ov_ref = OC.overlap(otd.topo, otd.s.A, psi0)     # sum psi0 * conj(psi_t)
+        assert abs(ov - ov_ref) < REL * abs(ov_ref), step
+        for nid in ["site0", "site5", "site40", "site62"]:
+            z = alg.state.single_site_operator_expectation_value(nid, PAULI_Z)
+            assert abs(z - otd.expect_site(nid, PAULI_Z)) < REL * abs(e_ref), (step, nid)
+
+
+def test_config2_full_size_conserves_norm_and_energy(gpu):
+    """The plain FP64 configuration at full size (63-node binary tree, bond dimension
+    64, XXZ, dt=0.02), checked through what one-site TDVP conserves: the norm and
+    the energy stay at their initial values over the configuration's 50 steps (up to
+    the Krylov tolerance), and |<psi_0|psi_t>| <= 1 decreases from 1."""
+    _say("config C2 full size, 50 steps")
+    st, H, topo, tensors, W = _setup(2, 5, 64, 63, "xxz")
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV)
+    alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.02, 1.0, [], config=cfg,
+                                     solver_options={"max_krylov_dim": 10})
+    c = alg.state.orthogonality_center_id
+    alg.state.tensors[c] = alg.state.tensors[c] / alg.state.norm()
+    st0 = alg.state.copy()
+    e0 = ptn.expectation_value(alg.state, H).real
+    assert max(alg.state.bond_dims().values()) == 64 and alg.num_time_steps == 50
+    for step in range(50):
+        alg.run_one_time_step()
+        assert abs(alg.state.norm() - 1.0) < 1e-12, step
+        assert abs(ptn.expectation_value(alg.state, H).real - e0) < 1e-10 * max(1, abs(e0)), step
+    ov = abs(ptn.scalar_product(st0, alg.state))
+    assert ov <= 1 + 1e-12 and ov < 1 - 1e-8
+
+
+def test_config3_full_size_conserves_norm_and_energy(gpu):
+    """The chi=256 configuration at full size (ternary tree N=40, root (256, 256, 256, 2),
+    random XYZ, dt=0.05): one second-order one-site step keeps the norm and the
+    energy (the FP64 path conserves both up to the Krylov tolerance)."""
+    _say("config C3 full size, 1 step")
+    import bench
+    st, H, _ = bench.make_problem(256, torch.device("cuda"))
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV)
+    alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.05, [], config=cfg,
+                                     solver_options={"max_krylov_dim": 10, "krylov_tol": 1e-10})
+    c = alg.state.orthogonality_center_id
+    alg.state.tensors[c] = alg.state.tensors[c] / alg.state.norm()
+    assert tuple(alg.state.tensors[alg.state.root_id].shape) == (256, 256, 256, 2)
+    e0 = ptn.expectation_value(alg.state, H).real
+    alg.run_one_time_step()
+    assert abs(alg.state.norm() - 1.0) < 1e-12
+    assert abs(ptn.expectation_value(alg.state, H).real - e0) < 1e-10 * max(1, abs(e0))
+    del alg, st, H
+    E.clear_caches()
+    torch.cuda.empty_cache()
