@@ -82,6 +82,27 @@ __global__ __launch_bounds__(BT) void block_apply_kernel(BlockArgs a, const TY *
 // elements from y_r[r0] + minoff, which the workgroup copies with coalesced
 // loads before each thread reads its row from LDS.  Dynamic LDS: the operator
 // blocks (nt nj ni complex128) and then the span.
+// LDS-only workgroup barrier: this thread's LDS accesses are complete, then
+// s_barrier.  Unlike __syncthreads() it does not wait for the global stores in
+// flight (vmcnt also counts stores here), so a tile's output stores drain
+// while the next tile's inputs load.  The memory clobber keeps the compiler
+// from moving memory accesses across it.
+constexpr int STAGE_ILP = 16;   // staged loads per thread in flight
+template <typename T>
+struct StageScalar;
+template <>
+struct StageScalar<double2> {
+  typedef double T;
+};
+template <>
+struct StageScalar<float2> {
+  typedef float T;
+};
+
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 template <typename TY, typename TO>
 __global__ __launch_bounds__(BT) void block_apply_staged(BlockArgs a, const TY *Y,
                                                          const int64_t *y_r,
@@ -110,9 +131,27 @@ __global__ __launch_bounds__(BT) void block_apply_staged(BlockArgs a, const TY *
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t r0 = tile * BT;
     const int64_t base = y_r[r0] + minoff;
-    __syncthreads();  // the previous tile's reads are done
-    for (int e = tid; e < span; e += BT) sY[e] = Y[base + e];
-    __syncthreads();
+    lds_barrier();  // every thread has read the previous tile's span
+    // batches of STAGE_ILP loads per thread in flight before the first wait (a
+    // plain copy loop waits for each load before its LDS write)
+    // (scalar register arrays: an array of vector-typed elements goes to scratch;
+    // the loads are unconditional at a clamped index inside the span)
+    for (int e0 = 0; e0 < span; e0 += STAGE_ILP * BT) {
+      typename StageScalar<TY>::T br[STAGE_ILP], bi[STAGE_ILP];
+#pragma unroll
+      for (int k = 0; k < STAGE_ILP; ++k) {
+        const int e = e0 + k * BT + tid;
+        const TY v = Y[base + (e < span ? e : span - 1)];
+        br[k] = v.x;
+        bi[k] = v.y;
+      }
+#pragma unroll
+      for (int k = 0; k < STAGE_ILP; ++k) {
+        const int e = e0 + k * BT + tid;
+        if (e < span) ptn::cstore(sY + e, (double)br[k], (double)bi[k]);
+      }
+    }
+    lds_barrier();  // the span is in LDS
     const int64_t r = r0 + tid;
     const int64_t yb = (int64_t)tid * s0 - minoff;
     const int64_t ob = o_r[r];
