@@ -8,11 +8,13 @@ H = -sum_e (Jx XX + Jy YY + Jz ZZ) with J ~ U[-1, 1] per edge and axis
 QR(KEEP)-canonicalised and normalised; dt = 0.05, Krylov dim <= 10,
 krylov_tol = 1e-10.  One step = forward + backward sweep.
 
-Runs the FP64 path (complex128, 4M FP64 MFMA) and the mixed path (complex64
-environments, 3M FP32 MFMA; Krylov/QR/SVD FP64) from the same state, reports
-both and the mixed energy error against FP64 at the same time.  ``value`` is
-the FP64 sweeps/s unless the mixed path meets |dE| <= 1e-8, in which case it is
-the mixed sweeps/s.
+Runs from the same state the FP64 path (complex128, FP64 MFMA), the relaxed
+path (FP64 except the Krylov applications of H_eff whose error can change the
+local result by at most RELAX_TOL relative to its input; those run as 3M FP32
+MFMA or rocBLAS cgemm on complex64 copies of the environments) and the mixed path (complex64 environments, 3M FP32 MFMA;
+Krylov/QR/SVD FP64).  Each reduced-precision path reports its energy error
+against FP64 at the same time.  ``value`` is the sweeps/s of the fastest path
+with |dE| <= 1e-8 (FP64 meets it by definition).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--chi 256] [--no-cpu]
 """
@@ -44,6 +46,9 @@ KERNEL_NAME = {("gather", "fp64"): "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_ga
 EXECUTED_PER_CMAC = {("gather", "fp64"): 6.0, ("blas", "fp64"): 8.0,
                      ("gather", "fp32"): 6.0, ("blas", "fp32"): 8.0}
 METRIC = "TDVP sweeps/s & H_eff TFLOP/s at χ=256, mixed vs FP64 GPU at ΔE≤1e-8"
+# relax_tol of the relaxed path: bound on the change of each local Krylov result
+# (relative to its input's norm) from the applications run in 3M FP32
+RELAX_TOL = 1e-7
 
 
 def log(*a):
@@ -60,11 +65,14 @@ def make_problem(chi: int, device):
     return st, H, (ids, parent, children)
 
 
-def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool):
+def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
+             relax_tol: float = RELAX_TOL):
     import torch
     import pytreenet_amd as ptn
     from pytreenet_amd import engine as E
-    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision)
+    from pytreenet_amd import krylov as K
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
+                         relax_tol=relax_tol)
     E.PROFILER.phase = f"{precision}-init"
     t0 = time.time()
     alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.05 * (warmup + steps), [], config=cfg,
@@ -82,6 +90,7 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool):
         log(f"[{precision}] warmup step {i} {time.time() - t1:.2f}s")
     nlog = len(alg.apply_log)
     fl0, n0 = E.PROFILER.flops, E.PROFILER.launches
+    low0, app0 = K.STATS.low_applies, K.STATS.applies
     E.PROFILER.phase = f"{precision}-timed"
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -91,13 +100,14 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool):
     elapsed = time.perf_counter() - t1
     log(f"[{precision}] {steps} timed steps {elapsed:.3f}s")
     fl1, n1 = E.PROFILER.flops, E.PROFILER.launches
+    low, apps = K.STATS.low_applies - low0, K.STATS.applies - app0
     E.PROFILER.phase = f"{precision}-eval"
     energy = ptn.expectation_value(alg.state, H).real
     return {"elapsed": elapsed, "energy": energy, "alg": alg,
             "applies": alg.apply_log[nlog:],
             "gemm_flops": fl1 - fl0, "launches": n1 - n0,
-            "kernels": E.PROFILER.summary(phase=f"{precision}-timed", compute=precision)
-            if profile else []}
+            "low_fraction": low / max(apps, 1),
+            "kernels": E.PROFILER.summary(phase=f"{precision}-timed") if profile else []}
 
 
 def cpu_baseline(chi_sample: int, steps_applies, shapes_full, wshapes_full):
@@ -239,19 +249,22 @@ def main():
            "fp64_gemm_tflops": r64["gemm_flops"] / r64["elapsed"] / 1e12,
            "fp64_launches_per_step": r64["launches"] / args.steps,
            "fp64_applies_per_step": sum(m for _, _, m in r64["applies"]) / args.steps}
-    mixed_ok = False
+    # (path, run, sweeps/s) of every path that meets |dE| <= 1e-8 against FP64
+    ok = [("fp64", r64, sps64)]
     if not args.no_mixed:
-        r32 = run_tdvp(st, H, "fp32", args.warmup, args.steps, profile=True)
-        sps32 = args.steps / r32["elapsed"]
-        de = abs(r32["energy"] - r64["energy"])
-        mixed_ok = de <= 1e-8
-        res.update({"mixed_sweeps_per_s": sps32, "mixed_energy": r32["energy"],
-                    "mixed_abs_energy_error": de, "mixed_speedup": sps32 / sps64,
-                    "mixed_gemm_tflops": r32["gemm_flops"] / r32["elapsed"] / 1e12})
-    value = res["mixed_sweeps_per_s"] if mixed_ok else sps64
+        for prec, key in (("relaxed", "relaxed"), ("fp32", "mixed")):
+            r = run_tdvp(st, H, prec, args.warmup, args.steps, profile=True)
+            sps = args.steps / r["elapsed"]
+            de = abs(r["energy"] - r64["energy"])
+            res.update({f"{key}_sweeps_per_s": sps, f"{key}_energy": r["energy"],
+                        f"{key}_abs_energy_error": de, f"{key}_speedup": sps / sps64,
+                        f"{key}_gemm_tflops": r["gemm_flops"] / r["elapsed"] / 1e12,
+                        f"{key}_low_precision_applies": r["low_fraction"]
+                        if prec == "relaxed" else 1.0})
+            if de <= 1e-8:
+                ok.append((prec, r, sps))
+    path, run, value = max(ok, key=lambda t: t[2])
     # the roofline follows the path whose rate is `value`
-    prec = "fp32" if mixed_ok else "fp64"
-    run = r32 if mixed_ok else r64
     kern = run["kernels"]
     roofline = None
     if kern:
@@ -263,24 +276,25 @@ def main():
         # the MFMAs run (6 M N K for the three-product form)
         by = {}
         for k in kern:
-            by.setdefault(k["backend"], []).append(k)
+            by.setdefault((k["backend"], k["compute"]), []).append(k)
         tot = {b: sum(k["total_ms"] for k in ks) for b, ks in by.items()}
         order = sorted(by, key=lambda b: -tot[b])
 
-        def block(b):
-            ks = by[b]
+        def block(bp):
+            ks = by[bp]
+            b, prec = bp
             n = sum(k["launches"] for k in ks)
             fl = sum(k["flops_per_launch"] * k["launches"] for k in ks)
             ms = sum(k["total_ms"] for k in ks)
             ach = fl / (ms * 1e-3)
-            ex = ach * EXECUTED_PER_CMAC[(b, prec)] / 8.0
+            ex = ach * EXECUTED_PER_CMAC[bp] / 8.0
             return {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK[prec] / 1e12,
                     "unit": "TFLOP/s", "frac": ach / PEAK[prec],
                     "executed_tflops": ex / 1e12, "executed_frac": ex / PEAK[prec],
                     # HBM bytes per launch: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
                     # (scales from --calibrate-copy), averaged over the launches
-                    "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((b, prec)),
-                    "kernel": KERNEL_NAME[(b, prec)], "flops_per_launch": fl / n,
+                    "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(bp),
+                    "kernel": KERNEL_NAME[bp], "flops_per_launch": fl / n,
                     "avg_ms": ms / n, "launches": n, "timed_steps": args.steps,
                     "share_of_step": ms * 1e-3 / run["elapsed"],
                     "shapes": [{"M": k["M"], "N": k["N"], "K": k["K"],
@@ -290,13 +304,13 @@ def main():
         roofline = block(order[0])
         if len(order) > 1:
             roofline["other"] = block(order[1])
-        allk = E.PROFILER.summary(compute=prec, backend=order[0])
+        allk = E.PROFILER.summary(compute=order[0][1], backend=order[0][0])
         n_all = sum(k["launches"] for k in allk)
         roofline["all_phases"] = {"launches": n_all,
                                   "avg_ms": sum(k["total_ms"] for k in allk) / n_all,
-                                  "note": f"every LARGE {prec} launch of this backend in "
-                                          "this process (setup, warmup, timed, evaluation, "
-                                          "and the other run's launches of this precision): "
+                                  "note": f"every LARGE {order[0][1]} launch of this backend "
+                                          "in this process (setup, warmup, timed, evaluation, "
+                                          "and the other runs' launches of this precision): "
                                           "the set rocprofv3 --stats averages over"}
     E.PROFILER.stop()
     cpu = None
@@ -318,12 +332,13 @@ def main():
     line = {"metric": METRIC, "value": value, "unit": "sweeps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 / value, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "c128" if not mixed_ok else "c64-mixed",
+            "vs_baseline": None,
+            "dtype": {"fp64": "c128", "relaxed": "c128 (+c64 3M on relaxed applies)",
+                      "fp32": "c64-mixed"}[path],
             "data": "synthetic",
             "config": {"workload": "2nd-order 1-site TDVP, ternary tree N=40, chi=256, d=2, "
                                    "random XYZ TTNO D=5, dt=0.05, krylov m<=10 tol 1e-10",
-                       "chi": args.chi, "nodes": 40, "precision": "fp64" if not mixed_ok
-                       else "mixed"},
+                       "chi": args.chi, "nodes": 40, "precision": path},
             "roofline": roofline, "cpu_baseline": cpu}
     line.update(res)
     print(json.dumps(line), flush=True)

@@ -33,6 +33,9 @@ Label = Hashable
 
 FP64 = "fp64"
 FP32 = "fp32"
+# FP64 environments and Krylov applications, each application run in 3M FP32 when
+# its contribution to the Krylov result is small enough (RelaxedNetwork)
+RELAXED = "relaxed"
 _COMPUTE = {FP64: _lib.PTN_COMPUTE_F64_4M, FP32: _lib.PTN_COMPUTE_F32_3M}
 _DTYPE_CODE = {torch.complex128: _lib.PTN_C128, torch.complex64: _lib.PTN_C64}
 
@@ -1115,6 +1118,36 @@ class CompiledNetwork:
             else:
                 self.op_roles.append((role(op[1]),))
         return out
+
+
+class RelaxedNetwork:
+    """One H_eff action held at two precisions, for relaxed-precision Krylov.
+
+    ``high`` contracts complex128 inputs with FP64 GEMMs; ``low`` contracts
+    complex64 copies of the same environments and operator with 3M FP32 GEMMs
+    (FP32 accumulation).  Calling the object runs ``high``.  The Krylov loop runs
+    ``low`` for an application whose error cannot move the result by more than
+    ``relax_tol`` times the input's norm (``krylov.krylov_exp_action``): the error of the j-th
+    application enters the result weighted by the j-th expansion coefficient,
+    which the loop estimates as it goes.  This is the relaxation of
+    matrix-vector accuracy in inexact Krylov methods (Simoncini and Szyld,
+    SIAM J. Sci. Comput. 25 (2003) 454-477; for the exponential, van den Eshof
+    and Hochbruck, SIAM J. Sci. Comput. 27 (2006) 1438-1457).  ``low_eps`` is
+    the relative error of one ``low`` application.
+    """
+
+    LOW_EPS = 1e-6
+
+    def __init__(self, high: "CompiledNetwork", low: "CompiledNetwork", relax_tol: float,
+                 low_eps: Optional[float] = None):
+        if high.precision != FP64 or low.precision != FP32:
+            raise ValueError("RelaxedNetwork pairs an FP64 network with an FP32 one")
+        self.high, self.low = high, low
+        self.relax_tol = float(relax_tol)
+        self.low_eps = self.LOW_EPS if low_eps is None else float(low_eps)
+
+    def __call__(self, x: torch.Tensor) -> torch.Tensor:
+        return self.high(x)
 
 
 # --------------------------------------------------------------------------------------

@@ -127,10 +127,11 @@ class TimeEvoMode(Enum):
             raise NotImplementedError(
                 f"mode {self.value}: the device path evolves matrix-free with "
                 "TimeEvoMode.KRYLOV only")
+        relaxed = options.pop("relaxed", None)
         return krylov_exp_action(psi, lambda y: time_evo_action(0.0, y), time_difference,
                                  exp_factor, krylov_tol=options.pop("krylov_tol", 1e-10),
                                  max_krylov_dim=options.pop("max_krylov_dim", None),
-                                 hermitian=hermitian)
+                                 hermitian=hermitian, relaxed=relaxed)
 
 
 def dense_exp_action(hamiltonian: torch.Tensor, psi: torch.Tensor, factor: complex
@@ -239,6 +240,7 @@ class KrylovStats:
     def __init__(self):
         self.calls = 0
         self.applies = 0
+        self.low_applies = 0     # applications run in low precision (relaxed form)
         self.dims = []
 
     def record(self, m: int):
@@ -288,8 +290,20 @@ def _pinned(count: int) -> torch.Tensor:
 def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.Tensor],
                       time_difference: float, exp_factor: complex,
                       krylov_tol: float = 1e-10, max_krylov_dim: Optional[int] = None,
-                      hermitian: bool = True) -> torch.Tensor:
-    """exp(exp_factor * dt * A) psi with A given by ``action`` (device tensors)."""
+                      hermitian: bool = True, relaxed=None) -> torch.Tensor:
+    """exp(exp_factor * dt * A) psi with A given by ``action`` (device tensors).
+
+    ``relaxed`` (an ``engine.RelaxedNetwork`` of the same A): application j runs
+    its low-precision form when tau * c_j * low_eps * ||T|| <= relax_tol * ||psi||
+    (relax_tol bounds the change of the result relative to ||psi||), with
+    tau = |exp_factor * dt|, ||T|| the largest column sum of |T| so far (it bounds
+    ||A q_j||, hence the error of one application over low_eps) and c_j the
+    latest Saad estimate beta_0 * beta * |[exp(f dt T) e_1]_{m-1}|.  In the
+    perturbed Arnoldi relation the error of application j enters the result
+    weighted by at most tau times the coefficient of q_j, which the estimate of
+    step j-1 approximates; the estimates decrease along the recurrence, so an
+    older one (the look-ahead uses the estimate of step j-2) overstates the
+    weight.  The first two applications always run in full precision."""
     E.require_device(psi)
     shape = psi.shape
     dev = psi.device
@@ -312,13 +326,29 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     beta0 = 0.0
     w_next = None
     m = 0
+    low = None
+    if relaxed is not None and relaxed.relax_tol > 0.0:
+        low = relaxed.low
+        tau = abs(complex(exp_factor) * time_difference)
+    est = math.inf          # Saad estimate of the latest step
+    t_norm = 0.0            # largest column sum of |T| so far
+
+    def apply(k):
+        if low is not None and k >= 2 and tau * est * relaxed.low_eps * t_norm <= \
+                relaxed.relax_tol * beta0:
+            STATS.low_applies += 1
+            # complex64 input: every product of the FP32 network then has
+            # complex64 operands (strided ones run as rocBLAS cgemm)
+            return low(Q[k].view(shape).to(torch.complex64))
+        return action(Q[k].view(shape))
+
     for j in range(max_steps):
         if j + 2 > Q.shape[0]:
             grown = torch.empty((min(2 * Q.shape[0], max_steps + 1), n),
                                 dtype=torch.complex128, device=dev)
             grown[: Q.shape[0]].copy_(Q)
             Q = grown
-        w = w_next if w_next is not None else action(Q[j].view(shape))
+        w = w_next if w_next is not None else apply(j)
         w_next = None
         w = _flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
         if w.data_ptr() == Q[j].data_ptr():
@@ -332,7 +362,7 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
         host_buf[: j + 3].copy_(packed[: j + 3], non_blocking=True)
         done.record()
         if lookahead and j + 1 < max_steps:
-            w_next = action(Q[j + 1].view(shape))
+            w_next = apply(j + 1)
         done.synchronize()
         host = host_buf[: j + 3].numpy()
         if j == 0:
@@ -346,9 +376,11 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
         m = j + 1
         if beta <= max(breakdown, 1e-12 * w0_norm):
             break
-        if krylov_tol > 0.0:
+        if krylov_tol > 0.0 or low is not None:
             col = small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
-            if beta0 * beta * abs(col[m - 1]) < krylov_tol:
+            est = beta0 * beta * abs(col[m - 1])
+            t_norm = max(t_norm, float(np.abs(hess[: j + 2, j]).sum()))
+            if krylov_tol > 0.0 and est < krylov_tol:
                 break
     coeffs = beta0 * small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
     out = torch.empty(n, dtype=torch.complex128, device=dev)

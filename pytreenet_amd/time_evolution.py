@@ -16,9 +16,12 @@ All tensors stay on the device.  The transient link tensor of a one-site
 update is the R factor held as a (node side, next side) matrix rather than a
 node inserted into the tree; the two-site tensor has legs
 (outer legs of the first node, outer legs of the second, phys1, phys2).
-``TDVPConfig.precision`` selects FP64 (complex128, 4M FP64 MFMA) or the mixed
-form (complex64 environments, 3M FP32 MFMA contractions; Krylov
-re-orthogonalisation, the small propagator, QR and SVD stay FP64).
+``TDVPConfig.precision`` selects FP64 (complex128, FP64 MFMA), the mixed form
+"fp32" (complex64 environments, 3M FP32 MFMA contractions; Krylov
+re-orthogonalisation, the small propagator, QR and SVD stay FP64), or
+"relaxed": FP64 throughout except the Krylov applications of H_eff whose
+contribution to the local result is below ``relax_tol``, which run in 3M FP32
+on complex64 copies of the environments.
 """
 from __future__ import annotations
 
@@ -67,6 +70,10 @@ class TDVPConfig(TTNTimeEvolutionConfig):
     time_evo_mode: TimeEvoMode = TimeEvoMode.FASTEST
     main_path_mode: str = PathFinderMode.LeafToRoot
     precision: str = E.FP64
+    # precision="relaxed": largest change of a local Krylov result, relative to
+    # the norm of its input, that the low-precision applications may cause
+    # (engine.RelaxedNetwork)
+    relax_tol: float = 1e-12
 
 
 @dataclass
@@ -287,6 +294,10 @@ def local_time_evolution(x: torch.Tensor, action, dt: float, direction: EvoDirec
         mode = TimeEvoMode.KRYLOV
     opts = dict(TimeEvoMode.default_solver_options(mode))
     opts.update(solver_options or {})
+    relaxed = None
+    if isinstance(action, E.RelaxedNetwork):
+        # only the matrix-free Krylov loop uses the low-precision form
+        relaxed, action = action, action.high
     if mode is TimeEvoMode.KRYLOV:
         if x.numel() <= _K.SMALL_N and isinstance(action, E.CompiledNetwork):
             # short local spaces: the recurrence runs in one launch on the dense
@@ -296,7 +307,7 @@ def local_time_evolution(x: torch.Tensor, action, dt: float, direction: EvoDirec
             if y is not None:
                 return y
         return mode.time_evolve_action(x, lambda t, y: action(y), dt, forward=direction,
-                                       **opts)
+                                       relaxed=relaxed, **opts)
     H = E.dense_operator(action)
     return mode.time_evolve(x, H, dt, forward=direction,
                             **{k: v for k, v in opts.items() if k == "exp_factor"})
@@ -328,8 +339,13 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
                                       "TimeEvoMode.KRYLOV or a dense exponential mode")
         self._mode = mode
         self.precision = getattr(self.config, "precision", E.FP64)
-        if self.precision not in (E.FP64, E.FP32):
+        if self.precision not in (E.FP64, E.FP32, E.RELAXED):
             raise ValueError(f"unknown precision {self.precision!r}")
+        # relaxed: FP64 storage and environments; site applications in the Krylov
+        # loop drop to 3M FP32 where their contribution is small
+        self.relaxed = self.precision == E.RELAXED
+        if self.relaxed:
+            self.precision = E.FP64
         self.apply_log: List[tuple] = []   # (kind, node or edge, Krylov applies)
         self._last_applies = 0
         # dense TTNO tensors in the site networks (the batched replay needs them)
@@ -455,10 +471,21 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         tensors[0] = tensor
         # recorded on the first Krylov apply, replayed for the others
         action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision)
+        if self.relaxed:
+            op32 = operator_tensor(self.hamiltonian, node_id, E.FP32,
+                                   block=False if self.dense_site_operator else None)
+            t32, l32, o32 = heff_network(node, op_node, op32,
+                                         [e.to(torch.complex64) for e in envs])
+            t32[0] = tensor
+            action = self._relaxed(action, E.CompiledNetwork(t32, l32, o32,
+                                                             precision=E.FP32))
 
         self.state.tensors[node_id] = yield from self._evolve(
             tensor, action, self.time_step_size * time_step_factor, EvoDirection.FORWARD)
         self.apply_log.append(("site", node_id, self._last_applies))
+
+    def _relaxed(self, high, low):
+        return E.RelaxedNetwork(high, low, getattr(self.config, "relax_tol", 1e-12))
 
     def _move_orth_and_update_cache_for_path(self, path: List[str]):
         if len(path) == 0:
@@ -616,6 +643,12 @@ class TwoSiteTDVP(TDVPAlgorithm):
             tensors.append(cache.get_entry(nb, n2))
             labels.append([("a", k1 + i), ("w", k1 + i), ("b", k1 + i)])
         out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
+        if self.relaxed:
+            t32 = [h.tensor_in(n1, torch.complex64), h.tensor_in(n2, torch.complex64)] + \
+                [e.to(torch.complex64) for e in tensors[3:]]
+            return (lambda y: self._relaxed(
+                E.CompiledNetwork([y] + tensors[1:], labels, out, precision=self.precision),
+                E.CompiledNetwork([y] + t32, labels, out, precision=E.FP32))), k1, k2
         return (lambda y: E.CompiledNetwork([y] + tensors[1:], labels, out,
                                             precision=self.precision)), k1, k2
 
@@ -723,6 +756,9 @@ class BatchedTDVP:
                     a.krylov_options() != a0.krylov_options():
                 raise ValueError("batched instances must share the algorithm, tree, time "
                                  "step, precision and solver options")
+        if any(a.relaxed for a in algs):
+            raise NotImplementedError("precision='relaxed' runs single instances; batched "
+                                      "instances use 'fp64' or 'fp32'")
         for a in algs:
             a.dense_site_operator = True
         self.algorithms = algs

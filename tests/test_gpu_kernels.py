@@ -481,3 +481,40 @@ def test_direct_gemm_m_contiguous_output(gpu, case, conj):
                accumulate=True, alpha=-0.5 + 0.25j)
     np.testing.assert_allclose(out.cpu().numpy(), base + (-0.5 + 0.25j) * ref, rtol=0,
                                atol=1e-14 * np.sqrt(k) * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("out_dt", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
+@pytest.mark.parametrize("case", [
+    ((4000, 37), "mk", (37, 2100), "kn", "mn"),               # n-contiguous C
+    ((4000, 37), "mk", (37, 2100), "kn", "nm"),               # m-contiguous C
+    ((40, 100, 3, 13), "abkc", (13, 3, 2100), "ckn", "nab"),  # gathered m and k
+])
+def test_gemm_fp32_large_c64(gpu, case, conj, out_dt):
+    """LARGE FP32 products of complex64 operands (tile grid >= 2048, ragged edge
+    tiles, odd K): rocBLAS cgemm for the strided layouts, the gather kernel for
+    the gathered one; plain and accumulating with alpha, against einsum."""
+    sa, la, sb, lb, lo = case
+    _say(f"large gemm fp32 c64 {la},{lb}->{lo} conj={conj} out={out_dt}")
+    rng = np.random.default_rng(sum(sa) + 3 * conj[0] + 5 * conj[1])
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    a32, b32 = a.astype(np.complex64), b.astype(np.complex64)
+    ref = np.einsum(f"{la},{lb}->{lo}", a32.conj() if conj[0] else a32,
+                    b32.conj() if conj[1] else b32).astype(np.complex128)
+    M = int(np.prod([d for d, l in zip(sa, la) if l not in lb]))
+    N = int(np.prod([d for d, l in zip(sb, lb) if l not in la]))
+    K = int(np.prod([d for d, l in zip(sa, la) if l in lb]))
+    assert E.gemm_kernel_class(M, N, K, E.FP32) == "large"
+    ta, tb = _dev(a, torch.complex64), _dev(b, torch.complex64)
+    got = E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.FP32,
+                     out_dtype=out_dt).cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-6, err
+    base = _crand(rng, ref.shape)
+    out = _dev(base, out_dt)
+    base = out.cpu().numpy().astype(np.complex128)
+    E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.FP32, out=out,
+               accumulate=True, alpha=-0.5 + 0.25j)
+    want = base + (-0.5 + 0.25j) * ref
+    err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
+    assert err < 2e-6, err

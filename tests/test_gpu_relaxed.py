@@ -1,0 +1,161 @@
+"""Relaxed-precision Krylov (TDVPConfig(precision="relaxed")) against the oracle.
+
+The relaxed form keeps every environment, split and Krylov vector in FP64 and
+runs an application of H_eff in 3M FP32 only when its weight in the local
+result (tau * coefficient estimate * ||T|| * low_eps) is below relax_tol.
+Bars: with relax_tol = 0 no application drops precision and the run is the FP64
+path (1e-12 relative to the oracle); with relax_tol = 1e-12 some applications
+do drop, and energies and <Z_i> stay within 1e-9 relative of the oracle, while
+the plain FP32 form on the same inputs is off by more than that.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pytreenet_amd as ptn
+from oracle.tdvp import OracleState, OracleTDVP
+from pytreenet_amd import krylov as K
+from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
+from pytreenet_amd.ttno import PAULI_Z
+from tests.helpers import host_tensors, topology_from_ttn
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _pair(kind, precision, relax_tol, chi=16, seed=5, steps=3, svd=None):
+    ids, parent, children = complete_tree(2, 3)
+    tensors = random_state_tensors(ids, parent, children, chi, seed)
+    st = build_state(ids, parent, children, tensors, device="cuda")
+    H = ptn.random_xyz_model(st, seed)
+    ops = {nid: ptn.TensorProduct({nid: PAULI_Z}) for nid in list(st.nodes)[:4]}
+    opts = {"krylov_tol": 1e-10, "max_krylov_dim": 10}
+    if kind == "second2":
+        cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV,
+                                    precision=precision, relax_tol=relax_tol, **(svd or {}))
+        alg = ptn.SecondOrderTwoSiteTDVP(st, H, 0.05, 0.05 * steps, ops, config=cfg,
+                                         solver_options=opts)
+    else:
+        cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
+                             relax_tol=relax_tol)
+        alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.05 * steps, ops, config=cfg,
+                                         solver_options=opts)
+    otd = OracleTDVP(OracleState(topology_from_ttn(st), tensors), host_tensors(H), 0.05,
+                     kind=kind, krylov_tol=1e-10, max_krylov_dim=10, svd_params=svd)
+    return alg, otd, ops
+
+
+def _run(alg, otd, steps):
+    errs = []
+    for _ in range(steps):
+        alg.run_one_time_step()
+        otd.run_one_time_step()
+        e = ptn.expectation_value(alg.state, alg.hamiltonian)
+        e_ref = otd.energy()
+        errs.append(abs(e - e_ref) / abs(e_ref))
+    return errs
+
+
+def _z_err(alg, otd, ops):
+    """Largest |<Z_i> - <Z_i>_oracle| relative to <psi|psi> (the states are not
+    normalised)."""
+    worst = 0.0
+    for nid in ops:
+        z = alg.state.single_site_operator_expectation_value(nid, PAULI_Z)
+        z_ref = otd.expect_site(nid, PAULI_Z)
+        worst = max(worst, abs(z - z_ref))
+    return worst / otd.norm_sq()
+
+
+def test_relaxed_zero_tol_is_fp64(gpu):
+    _say("relaxed precision, relax_tol=0: FP64 parity 1e-12")
+    alg, otd, _ = _pair("second1", "relaxed", 0.0)
+    low0 = K.STATS.low_applies
+    errs = _run(alg, otd, 2)
+    assert K.STATS.low_applies == low0
+    assert max(errs) < 1e-12, errs
+
+
+def test_relaxed_one_site_error_bound(gpu):
+    _say("relaxed precision one-site, relax_tol=1e-12 vs oracle and vs plain fp32")
+    alg, otd, ops = _pair("second1", "relaxed", 1e-12)
+    low0, app0 = K.STATS.low_applies, K.STATS.applies
+    errs = _run(alg, otd, 3)
+    low = K.STATS.low_applies - low0
+    _say(f"  low applications {low} of {K.STATS.applies - app0}; rel dE {max(errs):.2e}")
+    assert low > 0
+    assert max(errs) < 1e-9, errs
+    assert _z_err(alg, otd, ops) < 1e-9
+    # the plain FP32 form on the same inputs is far less accurate
+    a32, o32, _ = _pair("second1", "fp32", 0.0)
+    errs32 = _run(a32, o32, 3)
+    assert max(errs32) > 10 * max(errs), (errs32, errs)
+
+
+def test_relaxed_more_low_with_looser_tol(gpu):
+    _say("relaxed precision: looser relax_tol drops more applications")
+    counts = []
+    for tol in (1e-13, 1e-8):
+        alg, otd, _ = _pair("second1", "relaxed", tol, steps=1)
+        low0 = K.STATS.low_applies
+        errs = _run(alg, otd, 1)
+        counts.append(K.STATS.low_applies - low0)
+        assert max(errs) < 1e-6, (tol, errs)
+    assert counts[1] > counts[0], counts
+
+
+def test_relaxed_two_site_kept_dims(gpu):
+    _say("relaxed precision two-site with truncation: kept dims and energy")
+    svd = {"max_bond_dim": 12, "rel_tol": 1e-10, "total_tol": 1e-10}
+    alg, otd, _ = _pair("second2", "relaxed", 1e-12, chi=8, steps=2, svd=svd)
+    low0 = K.STATS.low_applies
+    errs = _run(alg, otd, 2)
+    assert K.STATS.low_applies > low0
+    for nid, t in alg.state.tensors.items():
+        assert tuple(t.shape) == tuple(otd.s.A[nid].shape), nid
+    assert max(errs) < 1e-9, errs
+
+
+def test_relaxed_rejected_in_batched(gpu):
+    alg, _, _ = _pair("second1", "relaxed", 1e-12, steps=1)
+    with pytest.raises(NotImplementedError):
+        ptn.BatchedTDVP([alg])
+
+
+def test_config3_relaxed_20_steps_energy_error(gpu):
+    """The chi=256 configuration (ternary tree N=40, random XYZ, dt=0.05) over its
+    20 steps: the relaxed path at the benchmark's relax_tol stays within
+    |dE| <= 1e-8 of the FP64 path, and so do <Z_i> on four sites (root, level 1,
+    level 2, leaf), while part of the applications ran in FP32."""
+    _say("config C3 full size, 20 steps, fp64 vs relaxed")
+    import bench
+    from pytreenet_amd import engine as E
+    res = {}
+    for prec in ("fp64", "relaxed"):
+        st, H, _ = bench.make_problem(256, torch.device("cuda"))
+        cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=prec,
+                             relax_tol=bench.RELAX_TOL)
+        alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 1.0, [], config=cfg,
+                                         solver_options={"max_krylov_dim": 10,
+                                                         "krylov_tol": 1e-10})
+        c = alg.state.orthogonality_center_id
+        alg.state.tensors[c] = alg.state.tensors[c] / alg.state.norm()
+        assert alg.num_time_steps == 20
+        low0 = K.STATS.low_applies
+        for step in range(20):
+            alg.run_one_time_step()
+        sites = ["site0", "site1", "site4", "site13"]
+        res[prec] = (ptn.expectation_value(alg.state, H).real,
+                     [alg.state.single_site_operator_expectation_value(n, PAULI_Z).real
+                      for n in sites], K.STATS.low_applies - low0)
+        _say(f"  {prec}: E={res[prec][0]:.15f} low applications {res[prec][2]}")
+        del alg, st, H
+        E.clear_caches()
+        torch.cuda.empty_cache()
+    (e64, z64, _), (er, zr, low) = res["fp64"], res["relaxed"]
+    assert low > 0
+    assert abs(er - e64) <= 1e-8, (er, e64)
+    assert max(abs(a - b) for a, b in zip(zr, z64)) <= 1e-8, (zr, z64)
