@@ -227,6 +227,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-chi", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true",
+                    help="launch the non-LARGE GEMMs on autotuned configurations")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 8 ptn_permute copies of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -242,6 +244,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     dev = torch.device("cuda:0")
     from pytreenet_amd import engine as E
+    E.AUTOTUNE = args.autotune
     # HIP events around every LARGE gather-GEMM launch of the process, tagged by phase
     E.PROFILER.start("setup")
     st, H, _ = make_problem(args.chi, dev)
@@ -343,6 +346,7 @@ def main():
                        "chi": args.chi, "nodes": 40, "precision": path},
             "roofline": roofline, "cpu_baseline": cpu}
     line.update(res)
+    line["autotune"] = {"on": E.AUTOTUNE, "tuned_signatures": len(E.TUNE_LOG)}
     print(json.dumps(line), flush=True)
 
 

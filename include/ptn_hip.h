@@ -78,6 +78,22 @@ typedef struct ptn_gemm_desc {
 
 int ptn_gemm_gather(void *stream, const ptn_gemm_desc *desc);
 
+/* The same contraction on a kernel configuration chosen by the caller (the
+ * project's autotuner times the candidates once per shape and replays the
+ * fastest).  No counterpart in the reference, whose np.tensordot has no
+ * launch configuration.  kind:
+ *   PTN_GEMM_RULE  the library's own choice (exactly ptn_gemm_gather);
+ *   PTN_GEMM_WAVE  the wave kernel with (bmb, bnb) 16-blocks per wave, one of
+ *                  1x1 2x1 1x2 2x2 4x1 1x4 1x3 3x1 1x5 (K split over waves by
+ *                  the library's rule when desc->ws allows);
+ *   PTN_GEMM_TILE  the 64 x 64 workgroup kernel with K cut into `ksplit`
+ *                  ranges (1..1024; > 1 needs ksplit*M*N*16 workspace bytes).
+ * Every configuration computes the same result up to rounding; PTN_EINVAL for
+ * a configuration that does not exist or does not fit the workspace.       */
+enum { PTN_GEMM_RULE = 0, PTN_GEMM_WAVE = 1, PTN_GEMM_TILE = 2 };
+int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *desc, int32_t kind,
+                           int32_t bmb, int32_t bnb, int32_t ksplit);
+
 /* The same contraction when every offset table is an arithmetic progression
  * (the legs of each index space are already grouped in memory):
  *

@@ -14,9 +14,10 @@ LIB_PATH = _PKG / "libptnhip.so"
 
 PTN_C128, PTN_C64 = 0, 1
 PTN_COMPUTE_F64_4M, PTN_COMPUTE_F32_3M = 0, 1
+PTN_GEMM_RULE, PTN_GEMM_WAVE, PTN_GEMM_TILE = 0, 1, 2
 
 EXPORTED = (
-    "ptn_gemm_gather", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
+    "ptn_gemm_gather", "ptn_gemm_gather_config", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
     "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize", "ptn_normalize",
     "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc", "ptn_krylov_small_work_bytes",
     "ptn_krylov_expm_small",
@@ -80,6 +81,8 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
     sig = {
         "ptn_gemm_gather": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc)]),
+        "ptn_gemm_gather_config": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc), i32, i32, i32,
+                                                  i32]),
         "ptn_gemm_strided": (ctypes.c_int, [vp, ctypes.POINTER(StridedDesc)]),
         "ptn_gemm_strided_ok": (ctypes.c_int, [ctypes.POINTER(StridedDesc)]),
         "ptn_permute": (ctypes.c_int, [vp, i32, vp, vp, vp, i32, vp, vp, i32, i32]),

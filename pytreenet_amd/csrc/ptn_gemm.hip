@@ -1098,6 +1098,57 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
 }
 
+// the (BMB, BNB) block shapes the wave kernel is instantiated for
+static bool wave_block_ok(int bmb, int bnb) {
+  static const int ok[][2] = {{1, 1}, {2, 1}, {1, 2}, {2, 2}, {4, 1}, {1, 4}, {1, 3}, {3, 1}, {1, 5}};
+  for (const auto &b : ok)
+    if (b[0] == bmb && b[1] == bnb) return true;
+  return false;
+}
+
+extern "C" int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *d, int32_t kind,
+                                      int32_t bmb, int32_t bnb, int32_t ksplit) {
+  if (kind == PTN_GEMM_RULE) return ptn_gemm_gather(stream, d);
+  KArgs a;
+  int64_t tiles = 0;
+  const int rc0 = to_kargs(d, a, tiles);
+  if (rc0 == 1) return PTN_OK;
+  if (rc0) return rc0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (kind == PTN_GEMM_WAVE) {
+    const int code = d->a_dtype * 4 + d->b_dtype * 2 + d->c_dtype;
+    const bool wave_types = code == 0 || code == 7 || code == 6 || code == 3 || code == 5;
+    if (!wave_types || !wave_block_ok(bmb, bnb)) {
+      ptn::set_error("ptn_gemm_gather_config: no wave kernel for blocks %dx%d (dtypes %d)", bmb,
+                     bnb, code);
+      return PTN_EINVAL;
+    }
+    a.ws = static_cast<double2 *>(d->ws);
+    return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0, d->a_dtype,
+                       d->b_dtype, d->c_dtype);
+  }
+  if (kind != PTN_GEMM_TILE || ksplit < 1 || ksplit > 1024) {
+    ptn::set_error("ptn_gemm_gather_config: bad kind %d / ksplit %d", kind, ksplit);
+    return PTN_EINVAL;
+  }
+  const int64_t kc = (d->K + ksplit - 1) / ksplit;
+  a.kchunk = d->K > 0 ? ((kc + BK - 1) / BK) * BK : BK;
+  int ks = d->K > 0 ? (int)((d->K + a.kchunk - 1) / a.kchunk) : 1;
+  if (ks < 1) ks = 1;
+  if (ks > 1 && (!d->ws || d->ws_bytes < (int64_t)sizeof(double2) * ks * d->M * d->N)) {
+    ptn::set_error("ptn_gemm_gather_config: %d K splits need %lld workspace bytes", ks,
+                   (long long)((int64_t)sizeof(double2) * ks * d->M * d->N));
+    return PTN_EINVAL;
+  }
+  a.ws = ks > 1 ? static_cast<double2 *>(d->ws) : nullptr;
+  a.remap = (tiles % 8 == 0) ? 1 : 0;
+  if (d->compute == PTN_COMPUTE_F64_4M)
+    return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles, ks);
+  if (d->a_dtype == PTN_C128)
+    return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
+  return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
+}
+
 extern "C" int64_t ptn_gemm_grouped_work_bytes(int32_t count) {
   if (count < 1) return 0;
   return ((((int64_t)count * sizeof(KArgs) + 255) / 256) * 256) + ((int64_t)count + 1) * 8;

@@ -518,3 +518,62 @@ def test_gemm_fp32_large_c64(gpu, case, conj, out_dt):
     want = base + (-0.5 + 0.25j) * ref
     err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
     assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("case", [
+    ((3000, 40), "mk", (40, 24), "kn", "mn", torch.complex128, E.FP64),    # thin N
+    ((24, 700), "mk", (700, 900), "kn", "nm", torch.complex128, E.FP64),   # thin M, m-contiguous C
+    ((300, 260, 2), "mkp", (260, 2, 200), "kpn", "mn", torch.complex128, E.FP64),  # mid, long K
+    ((70, 33), "mk", (33, 50), "kn", "mn", torch.complex128, E.FP64),      # small, ragged
+    ((500, 300), "mk", (300, 40), "kn", "nm", torch.complex64, E.FP32),    # mixed form
+])
+def test_gemm_configurations_and_autotuner(gpu, case):
+    """Every launch configuration the autotuner times (library rule, wave kernel
+    block shapes, 64 x 64 kernel with and without K splits) gives the product;
+    the tuned launch replays the fastest one."""
+    sa, la, sb, lb, lo, dt, prec = case
+    _say(f"gemm configurations {la},{lb}->{lo} {prec}")
+    rng = np.random.default_rng(sum(sa) + sum(sb))
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    ta = _dev(a, torch.complex128)               # complex128 A: the mixed form's Krylov side
+    tb = _dev(b, dt)
+    bb = b.astype(np.complex64) if dt == torch.complex64 else b
+    ref = np.einsum(f"{la},{lb}->{lo}", a, bb)
+    k = int(np.prod([d for d, l in zip(sa, la) if l in lb]))
+    tol = (1e-14 * np.sqrt(k) if prec == E.FP64 else 1e-5) * np.abs(ref).max()
+    E.AUTOTUNE = True
+    try:
+        n0 = len(E.TUNE_LOG)
+        got = E.contract(ta, la, tb, lb, lo, precision=prec).cpu().numpy()
+        np.testing.assert_allclose(got, ref, rtol=0, atol=tol)
+        assert len(E.TUNE_LOG) == n0 + 1
+        key, chosen, times = E.TUNE_LOG[-1]
+        assert chosen == min(times, key=times.get) and len(times) >= 3
+        for cfg in times:
+            E._TUNED[key] = cfg
+            got = E.contract(ta, la, tb, lb, lo, precision=prec).cpu().numpy()
+            np.testing.assert_allclose(got, ref, rtol=0, atol=tol, err_msg=str(cfg))
+        E._TUNED[key] = chosen
+    finally:
+        E.AUTOTUNE = False
+
+
+def test_gemm_config_rejects_unknown(gpu):
+    """A wave block shape without a kernel instantiation, and K splits beyond the
+    workspace, are errors, not launches."""
+    import ctypes
+    from pytreenet_amd import _lib
+    rng = np.random.default_rng(3)
+    a, b = _dev(_crand(rng, (40, 300))), _dev(_crand(rng, (300, 30)))
+    E.AUTOTUNE = True
+    try:
+        E.contract(a, "mk", b, "kn", "mn")
+        key, chosen, _ = E.TUNE_LOG[-1]
+        for bad in ((_lib.PTN_GEMM_WAVE, 3, 3, 0), (_lib.PTN_GEMM_TILE, 0, 0, 0),
+                    (_lib.PTN_GEMM_TILE, 0, 0, 1 << 20), (7, 0, 0, 0)):
+            E._TUNED[key] = bad
+            with pytest.raises(Exception):
+                E.contract(a, "mk", b, "kn", "mn")
+        E._TUNED[key] = chosen
+    finally:
+        E.AUTOTUNE = False

@@ -373,3 +373,21 @@ def test_config3_full_size_conserves_norm_and_energy(gpu):
     del alg, st, H
     E.clear_caches()
     torch.cuda.empty_cache()
+
+
+def test_tdvp_autotuned_launches_match_oracle(gpu):
+    """A second-order one-site run with every non-LARGE gather-GEMM launched on its
+    autotuned configuration stays at the oracle's energies (1e-12)."""
+    _say("tdvp second1 with autotuned GEMM configurations")
+    alg, otd, _ = _run_pair("second1", 2, 3, 16, 21, "xxz", 0.05, 2)
+    E.AUTOTUNE = True
+    n0 = len(E.TUNE_LOG)
+    try:
+        for step in range(2):
+            alg.run_one_time_step()
+            otd.run_one_time_step()
+            e, e_ref = ptn.expectation_value(alg.state, alg.hamiltonian), otd.energy()
+            assert _rel(e, e_ref) < REL, (step, e, e_ref)
+    finally:
+        E.AUTOTUNE = False
+    assert len(E.TUNE_LOG) > n0
