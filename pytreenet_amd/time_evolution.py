@@ -74,6 +74,10 @@ class TDVPConfig(TTNTimeEvolutionConfig):
     # the norm of its input, that the low-precision applications may cause
     # (engine.RelaxedNetwork)
     relax_tol: float = 1e-12
+    # smallest local space (elements) whose applications may drop to FP32: below
+    # it the H_eff GEMMs are bound by memory and launches, where the second
+    # network costs more than the FP32 arithmetic saves
+    relax_min_size: int = 1 << 20
 
 
 @dataclass
@@ -471,7 +475,7 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         tensors[0] = tensor
         # recorded on the first Krylov apply, replayed for the others
         action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision)
-        if self.relaxed:
+        if self.relaxed and tensor.numel() >= getattr(self.config, "relax_min_size", 0):
             op32 = operator_tensor(self.hamiltonian, node_id, E.FP32,
                                    block=False if self.dense_site_operator else None)
             t32, l32, o32 = heff_network(node, op_node, op32,
@@ -644,11 +648,18 @@ class TwoSiteTDVP(TDVPAlgorithm):
             labels.append([("a", k1 + i), ("w", k1 + i), ("b", k1 + i)])
         out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
         if self.relaxed:
-            t32 = [h.tensor_in(n1, torch.complex64), h.tensor_in(n2, torch.complex64)] + \
-                [e.to(torch.complex64) for e in tensors[3:]]
-            return (lambda y: self._relaxed(
-                E.CompiledNetwork([y] + tensors[1:], labels, out, precision=self.precision),
-                E.CompiledNetwork([y] + t32, labels, out, precision=E.FP32))), k1, k2
+            min_size = getattr(self.config, "relax_min_size", 0)
+
+            def make(y):
+                high = E.CompiledNetwork([y] + tensors[1:], labels, out,
+                                         precision=self.precision)
+                if y.numel() < min_size:
+                    return high
+                t32 = [h.tensor_in(n1, torch.complex64), h.tensor_in(n2, torch.complex64)] + \
+                    [e.to(torch.complex64) for e in tensors[3:]]
+                return self._relaxed(high, E.CompiledNetwork([y] + t32, labels, out,
+                                                             precision=E.FP32))
+            return make, k1, k2
         return (lambda y: E.CompiledNetwork([y] + tensors[1:], labels, out,
                                             precision=self.precision)), k1, k2
 

@@ -3,6 +3,7 @@
 The relaxed form keeps every environment, split and Krylov vector in FP64 and
 runs an application of H_eff in 3M FP32 only when its weight in the local
 result (tau * coefficient estimate * ||T|| * low_eps) is below relax_tol.
+The small trees here set relax_min_size = 0 (every local space may drop).
 Bars: with relax_tol = 0 no application drops precision and the run is the FP64
 path (1e-12 relative to the oracle); with relax_tol = 1e-12 some applications
 do drop, and energies and <Z_i> stay within 1e-9 relative of the oracle, while
@@ -35,12 +36,13 @@ def _pair(kind, precision, relax_tol, chi=16, seed=5, steps=3, svd=None):
     opts = {"krylov_tol": 1e-10, "max_krylov_dim": 10}
     if kind == "second2":
         cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV,
-                                    precision=precision, relax_tol=relax_tol, **(svd or {}))
+                                    precision=precision, relax_tol=relax_tol,
+                                    relax_min_size=0, **(svd or {}))
         alg = ptn.SecondOrderTwoSiteTDVP(st, H, 0.05, 0.05 * steps, ops, config=cfg,
                                          solver_options=opts)
     else:
         cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
-                             relax_tol=relax_tol)
+                             relax_tol=relax_tol, relax_min_size=0)
         alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.05 * steps, ops, config=cfg,
                                          solver_options=opts)
     otd = OracleTDVP(OracleState(topology_from_ttn(st), tensors), host_tensors(H), 0.05,
@@ -159,3 +161,23 @@ def test_config3_relaxed_20_steps_energy_error(gpu):
     assert low > 0
     assert abs(er - e64) <= 1e-8, (er, e64)
     assert max(abs(a - b) for a, b in zip(zr, z64)) <= 1e-8, (zr, z64)
+
+
+def test_relaxed_min_size_keeps_small_spaces_fp64(gpu):
+    """Below relax_min_size every application runs in FP64: the run is the FP64
+    path (1e-12 against the oracle) and no application dropped."""
+    _say("relaxed precision with the default relax_min_size on a small tree")
+    ids, parent, children = complete_tree(2, 3)
+    tensors = random_state_tensors(ids, parent, children, 16, 5)
+    st = build_state(ids, parent, children, tensors, device="cuda")
+    H = ptn.random_xyz_model(st, 5)
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision="relaxed",
+                         relax_tol=1e-6)
+    alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.1, [], config=cfg,
+                                     solver_options={"krylov_tol": 1e-10, "max_krylov_dim": 10})
+    otd = OracleTDVP(OracleState(topology_from_ttn(st), tensors), host_tensors(H), 0.05,
+                     kind="second1", krylov_tol=1e-10, max_krylov_dim=10)
+    low0 = K.STATS.low_applies
+    errs = _run(alg, otd, 2)
+    assert K.STATS.low_applies == low0
+    assert max(errs) < 1e-12, errs
