@@ -475,14 +475,18 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         tensors[0] = tensor
         # recorded on the first Krylov apply, replayed for the others
         action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision)
-        if self.relaxed and tensor.numel() >= getattr(self.config, "relax_min_size", 0):
+        if getattr(self, "relaxed", False) and \
+                tensor.numel() >= getattr(self.config, "relax_min_size", 0):
             op32 = operator_tensor(self.hamiltonian, node_id, E.FP32,
                                    block=False if self.dense_site_operator else None)
             t32, l32, o32 = heff_network(node, op_node, op32,
                                          [e.to(torch.complex64) for e in envs])
             t32[0] = tensor
+            # complex64 output: the last product then has complex64 operands
+            # and result, like the others (rocBLAS cgemm where it is strided)
             action = self._relaxed(action, E.CompiledNetwork(t32, l32, o32,
-                                                             precision=E.FP32))
+                                                             precision=E.FP32,
+                                                             out_dtype=torch.complex64))
 
         self.state.tensors[node_id] = yield from self._evolve(
             tensor, action, self.time_step_size * time_step_factor, EvoDirection.FORWARD)
@@ -647,7 +651,8 @@ class TwoSiteTDVP(TDVPAlgorithm):
             tensors.append(cache.get_entry(nb, n2))
             labels.append([("a", k1 + i), ("w", k1 + i), ("b", k1 + i)])
         out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
-        if self.relaxed:
+        # (DMRG borrows this method: its objects have no relaxed form)
+        if getattr(self, "relaxed", False):
             min_size = getattr(self.config, "relax_min_size", 0)
 
             def make(y):
@@ -658,7 +663,8 @@ class TwoSiteTDVP(TDVPAlgorithm):
                 t32 = [h.tensor_in(n1, torch.complex64), h.tensor_in(n2, torch.complex64)] + \
                     [e.to(torch.complex64) for e in tensors[3:]]
                 return self._relaxed(high, E.CompiledNetwork([y] + t32, labels, out,
-                                                             precision=E.FP32))
+                                                             precision=E.FP32,
+                                                             out_dtype=torch.complex64))
             return make, k1, k2
         return (lambda y: E.CompiledNetwork([y] + tensors[1:], labels, out,
                                             precision=self.precision)), k1, k2
