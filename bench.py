@@ -32,21 +32,23 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 # measured HBM bytes per LARGE launch of each (backend, precision) (rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE on this command, scaled by the copy calibration;
 # the summaries are under profiles/); None where not measured
 TRAFFIC_BYTES_PER_LAUNCH = {("gather", "fp64"): 2.75e9, ("blas", "fp64"): None,
                             # mean over the relaxed run's timed LARGE FP32 launches
                             # (112 at 0.88 GB, 56 at 3.04 GB)
-                            ("gather", "fp32"): 1.60e9, ("blas", "fp32"): None}
+                            ("gather", "fp32"): 1.60e9, ("blas", "fp32"): None,
+                            ("gather", "bf16"): None}
 KERNEL_NAME = {("gather", "fp64"): "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, FP64 3M)",
                ("blas", "fp64"): "rocblas_zgemm (ptn_gemm_strided)",
                ("gather", "fp32"): "gemm_gather_kernel<F32_3M, ..., LARGE=1> (ptn_gemm_gather)",
-               ("blas", "fp32"): "rocblas_cgemm (ptn_gemm_strided)"}
+               ("blas", "fp32"): "rocblas_cgemm (ptn_gemm_strided)",
+               ("gather", "bf16"): "gemm_bf16_kernel<..., LARGE=1> (ptn_gemm_gather, BF16 3M)"}
 # real flops executed per complex multiply-add (8 = four-product form)
 EXECUTED_PER_CMAC = {("gather", "fp64"): 6.0, ("blas", "fp64"): 8.0,
-                     ("gather", "fp32"): 6.0, ("blas", "fp32"): 8.0}
+                     ("gather", "fp32"): 6.0, ("blas", "fp32"): 8.0, ("gather", "bf16"): 6.0}
 METRIC = "TDVP sweeps/s & H_eff TFLOP/s at χ=256, mixed vs FP64 GPU at ΔE≤1e-8"
 # relax_tol of the relaxed path: bound on the change of each local Krylov result
 # (relative to its input's norm) from the applications run in 3M FP32
@@ -68,14 +70,15 @@ def make_problem(chi: int, device):
 
 
 def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
-             relax_tol: float = RELAX_TOL):
+             relax_tol: float = RELAX_TOL, relax_bf16: bool = False):
     import torch
     import pytreenet_amd as ptn
     from pytreenet_amd import engine as E
     from pytreenet_amd import krylov as K
     cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
-                         relax_tol=relax_tol)
-    E.PROFILER.phase = f"{precision}-init"
+                         relax_tol=relax_tol, relax_bf16=relax_bf16)
+    tag = precision + ("-bf16" if relax_bf16 else "")
+    E.PROFILER.phase = f"{tag}-init"
     t0 = time.time()
     alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.05 * (warmup + steps), [], config=cfg,
                                      solver_options={"max_krylov_dim": 10,
@@ -83,33 +86,34 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
     c = alg.state.orthogonality_center_id
     alg.state.tensors[c] = alg.state.tensors[c] / alg.state.norm()
     torch.cuda.synchronize()
-    log(f"[{precision}] init {time.time() - t0:.2f}s")
-    E.PROFILER.phase = f"{precision}-warmup"
+    log(f"[{tag}] init {time.time() - t0:.2f}s")
+    E.PROFILER.phase = f"{tag}-warmup"
     for i in range(warmup):
         t1 = time.time()
         alg.run_one_time_step()
         torch.cuda.synchronize()
-        log(f"[{precision}] warmup step {i} {time.time() - t1:.2f}s")
+        log(f"[{tag}] warmup step {i} {time.time() - t1:.2f}s")
     nlog = len(alg.apply_log)
     fl0, n0 = E.PROFILER.flops, E.PROFILER.launches
-    low0, app0 = K.STATS.low_applies, K.STATS.applies
-    E.PROFILER.phase = f"{precision}-timed"
+    low0, app0, lower0 = K.STATS.low_applies, K.STATS.applies, K.STATS.lower_applies
+    E.PROFILER.phase = f"{tag}-timed"
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for i in range(steps):
         alg.run_one_time_step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
-    log(f"[{precision}] {steps} timed steps {elapsed:.3f}s")
+    log(f"[{tag}] {steps} timed steps {elapsed:.3f}s")
     fl1, n1 = E.PROFILER.flops, E.PROFILER.launches
     low, apps = K.STATS.low_applies - low0, K.STATS.applies - app0
-    E.PROFILER.phase = f"{precision}-eval"
+    E.PROFILER.phase = f"{tag}-eval"
     energy = ptn.expectation_value(alg.state, H).real
     return {"elapsed": elapsed, "energy": energy, "alg": alg,
             "applies": alg.apply_log[nlog:],
             "gemm_flops": fl1 - fl0, "launches": n1 - n0,
             "low_fraction": low / max(apps, 1),
-            "kernels": E.PROFILER.summary(phase=f"{precision}-timed") if profile else []}
+            "lower_fraction": (K.STATS.lower_applies - lower0) / max(apps, 1),
+            "kernels": E.PROFILER.summary(phase=f"{tag}-timed") if profile else []}
 
 
 def cpu_baseline(chi_sample: int, steps_applies, shapes_full, wshapes_full):
@@ -257,8 +261,9 @@ def main():
     # (path, run, sweeps/s) of every path that meets |dE| <= 1e-8 against FP64
     ok = [("fp64", r64, sps64)]
     if not args.no_mixed:
-        for prec, key in (("relaxed", "relaxed"), ("fp32", "mixed")):
-            r = run_tdvp(st, H, prec, args.warmup, args.steps, profile=True)
+        for prec, key, bf16 in (("relaxed", "relaxed", False),
+                                ("relaxed", "relaxed_bf16", True), ("fp32", "mixed", False)):
+            r = run_tdvp(st, H, prec, args.warmup, args.steps, profile=True, relax_bf16=bf16)
             sps = args.steps / r["elapsed"]
             de = abs(r["energy"] - r64["energy"])
             res.update({f"{key}_sweeps_per_s": sps, f"{key}_energy": r["energy"],
@@ -266,8 +271,10 @@ def main():
                         f"{key}_gemm_tflops": r["gemm_flops"] / r["elapsed"] / 1e12,
                         f"{key}_low_precision_applies": r["low_fraction"]
                         if prec == "relaxed" else 1.0})
+            if bf16:
+                res[f"{key}_bf16_applies"] = r["lower_fraction"]
             if de <= 1e-8:
-                ok.append((prec, r, sps))
+                ok.append((key if prec == "relaxed" else prec, r, sps))
     path, run, value = max(ok, key=lambda t: t[2])
     # the roofline follows the path whose rate is `value`
     kern = run["kernels"]
@@ -339,6 +346,7 @@ def main():
             "ms_per_step": 1e3 / value, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None,
             "dtype": {"fp64": "c128", "relaxed": "c128 (+c64 3M on relaxed applies)",
+                      "relaxed_bf16": "c128 (+c64 3M FP32 / bf16 on relaxed applies)",
                       "fp32": "c64-mixed"}[path],
             "data": "synthetic",
             "config": {"workload": "2nd-order 1-site TDVP, ternary tree N=40, chi=256, d=2, "

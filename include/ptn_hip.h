@@ -28,7 +28,7 @@ extern "C" {
 #endif
 
 enum { PTN_C128 = 0, PTN_C64 = 1 };
-enum { PTN_COMPUTE_F64_4M = 0, PTN_COMPUTE_F32_3M = 1 };
+enum { PTN_COMPUTE_F64_4M = 0, PTN_COMPUTE_F32_3M = 1, PTN_COMPUTE_BF16_3M = 2 };
 enum {
   PTN_OK = 0,
   PTN_EINVAL = -1,     /* bad argument / shape mismatch                       */

@@ -318,6 +318,199 @@ __device__ __forceinline__ void gather_tile(const KArgs &p, int64_t bid, int64_t
   }
 }
 
+// ---- BF16 (3M) register-staged kernel ---------------------------------------------
+// The lowest precision of the relaxed Krylov form.  Operands (complex64 or
+// complex128) are read through the offset tables as in gather_tile, rounded to
+// bf16 (round to nearest even) while staging, and multiplied on
+// v_mfma_f32_16x16x16_bf16 with FP32 accumulation in the three-product form.  The
+// sums ar + ai and br + bi of the third product are formed in FP32 before
+// rounding.  bf16 keeps FP32's exponent range, so unit-norm Krylov vectors and
+// environments need no scaling.  LDS planes are [m][k] with k contiguous: lane l
+// reads its four k values of a 16x16x16 operand (rows l%16, k = 4 (l/16) + 0..3)
+// as one 8-byte load.  K step 16 (one MFMA slab per stage; a K step of 32
+// spills the per-element offsets), two stages.
+constexpr int HBK = 16, HKP = HBK + 4, HLA = BM * HBK / NT;
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+template <typename TA, typename TB, typename TC>
+__device__ __forceinline__ void gather_tile_bf16(const KArgs &p, int64_t bid, int64_t split,
+                                                 bool splitk) {
+  // planes: 0 re, 1 im, 2 re + im
+  __shared__ __attribute__((aligned(16))) unsigned short sA[2][3][BM][HKP];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[2][3][BN][HKP];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const TA *A = static_cast<const TA *>(p.A);
+  const TB *B = static_cast<const TB *>(p.B);
+
+  int64_t a_moff[HLA], b_noff[HLA];
+  bool a_mval[HLA], b_nval[HLA];
+  int a_kl[HLA], a_ml[HLA], b_kl[HLA], b_nl[HLA];
+#pragma unroll
+  for (int i = 0; i < HLA; ++i) {
+    if (p.akmaj) {
+      a_kl[i] = tid % HBK;
+      a_ml[i] = tid / HBK + (NT / HBK) * i;
+    } else {
+      a_kl[i] = (tid >> 6) + 4 * i;
+      a_ml[i] = tid & 63;
+    }
+    const int64_t gm = m0 + a_ml[i];
+    a_mval[i] = gm < p.M;
+    a_moff[i] = a_mval[i] ? p.aom[gm] : 0;
+    if (p.bkmaj) {
+      b_kl[i] = tid % HBK;
+      b_nl[i] = tid / HBK + (NT / HBK) * i;
+    } else {
+      b_kl[i] = (tid >> 6) + 4 * i;
+      b_nl[i] = tid & 63;
+    }
+    const int64_t gn = n0 + b_nl[i];
+    b_nval[i] = gn < p.N;
+    b_noff[i] = b_nval[i] ? p.bon[gn] : 0;
+  }
+
+  float ra_r[HLA], ra_i[HLA], rb_r[HLA], rb_i[HLA];
+  int64_t a_koff[HLA], b_koff[HLA];
+  const float asg = p.aconj ? -1.0f : 1.0f, bsg = p.bconj ? -1.0f : 1.0f;
+  auto kload = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < HLA; ++i) {
+      const int64_t ga = k0 + a_kl[i];
+      a_koff[i] = (ga < kend) ? p.aok[ga] : -1;
+      const int64_t gb = k0 + b_kl[i];
+      b_koff[i] = (gb < kend) ? p.bok[gb] : -1;
+    }
+  };
+  auto gload = [&]() {
+#pragma unroll
+    for (int i = 0; i < HLA; ++i) {
+      const bool ok = a_mval[i] && a_koff[i] >= 0;
+      float re, im;
+      ptn::cload(A + (ok ? a_moff[i] + a_koff[i] : 0), re, im);
+      ra_r[i] = ok ? re : 0.0f;
+      ra_i[i] = ok ? im : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < HLA; ++i) {
+      const bool ok = b_nval[i] && b_koff[i] >= 0;
+      float re, im;
+      ptn::cload(B + (ok ? b_koff[i] + b_noff[i] : 0), re, im);
+      rb_r[i] = ok ? re : 0.0f;
+      rb_i[i] = ok ? im : 0.0f;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < HLA; ++i) {
+      const float ai = asg * ra_i[i], bi = bsg * rb_i[i];
+      sA[buf][0][a_ml[i]][a_kl[i]] = bf16_rne(ra_r[i]);
+      sA[buf][1][a_ml[i]][a_kl[i]] = bf16_rne(ai);
+      sA[buf][2][a_ml[i]][a_kl[i]] = bf16_rne(ra_r[i] + ai);
+      sB[buf][0][b_nl[i]][b_kl[i]] = bf16_rne(rb_r[i]);
+      sB[buf][1][b_nl[i]][b_kl[i]] = bf16_rne(bi);
+      sB[buf][2][b_nl[i]][b_kl[i]] = bf16_rne(rb_r[i] + bi);
+    }
+  };
+
+  v4f c0[2][2], c1[2][2], c2[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      c0[i][j] = v4f{0, 0, 0, 0};
+      c1[i][j] = v4f{0, 0, 0, 0};
+      c2[i][j] = v4f{0, 0, 0, 0};
+    }
+
+  const int64_t KT = kend > kbeg ? (kend - kbeg + HBK - 1) / HBK : 0;
+  if (KT > 0) {
+    kload(kbeg);
+    gload();
+    sstore(0);
+    if (KT > 1) kload(kbeg + HBK);
+  }
+  __syncthreads();
+  const int fr = lane & 15, fk = lane >> 4;
+  for (int64_t kt = 0; kt < KT; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < KT) {
+      gload();
+      if (kt + 2 < KT) kload(kbeg + (kt + 2) * HBK);
+    }
+#pragma unroll
+    for (int sl = 0; sl < HBK / 16; ++sl) {
+      v4s a[3][2], b[3][2];
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          a[q][i] = *reinterpret_cast<const v4s *>(
+              &sA[buf][q][wm * 32 + i * 16 + fr][sl * 16 + fk * 4]);
+          b[q][i] = *reinterpret_cast<const v4s *>(
+              &sB[buf][q][wn * 32 + i * 16 + fr][sl * 16 + fk * 4]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          c0[i][j] =
+              __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0][i], b[0][j], c0[i][j], 0, 0, 0);
+          c1[i][j] =
+              __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1][i], b[1][j], c1[i][j], 0, 0, 0);
+          c2[i][j] =
+              __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[2][i], b[2][j], c2[i][j], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < KT) sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // ---- epilogue (accumulator row of (lane, r) = 4*(lane/16) + r) -------------
+  TC *C = static_cast<TC *>(p.C);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t gn = n0 + wn * 32 + j * 16 + fr;
+    if (gn >= p.N) continue;
+    const int64_t noff = p.con[gn];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm * 32 + i * 16 + fk * 4 + r;
+        if (gm >= p.M) continue;
+        const float t1 = c0[i][j][r], t2 = c1[i][j][r], t3 = c2[i][j][r];
+        const double re = (double)(t1 - t2), im = (double)(t3 - t1 - t2);
+        if (splitk) {
+          p.ws[(split * p.M + gm) * p.N + gn] = make_double2(re, im);
+          continue;
+        }
+        double ore = p.are * re - p.aim * im;
+        double oim = p.are * im + p.aim * re;
+        TC *dst = C + p.com[gm] + noff;
+        if (p.beta_one) {
+          double xr, xi;
+          ptn::cload(dst, xr, xi);
+          ore += xr;
+          oim += xi;
+        }
+        ptn::cstore(dst, ore, oim);
+      }
+    }
+  }
+}
+
 // tile index of a launch with the XCD-aware remap applied
 __device__ __forceinline__ int64_t remapped_tile(const KArgs &p) {
   int64_t bid = blockIdx.x;
@@ -331,6 +524,11 @@ __device__ __forceinline__ int64_t remapped_tile(const KArgs &p) {
 template <int MODE, typename TA, typename TB, typename TC, int LARGE>
 __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) {
   gather_tile<MODE, TA, TB, TC>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
+}
+
+template <typename TA, typename TB, typename TC, int LARGE>
+__global__ __launch_bounds__(NT, 2) void gemm_bf16_kernel(KArgs p) {
+  gather_tile_bf16<TA, TB, TC>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
 }
 
 // ---- FP64 kernel with direct global->LDS tile loads ---------------------------
@@ -955,7 +1153,14 @@ int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
       hipLaunchKernelGGL((gemm_direct_f64_kernel<0, false>), dim3((unsigned)tiles, ksplit),
                          dim3(NT), 0, s, a, a.aok, a.bok);
   } else {
-    if (large)
+    if constexpr (MODE == PTN_COMPUTE_BF16_3M) {
+      if (large)
+        hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
+                           dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC, 0>), dim3((unsigned)tiles, ksplit),
+                           dim3(NT), 0, s, a);
+    } else if (large)
       hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
                          dim3(NT), 0, s, a);
     else
@@ -1001,7 +1206,8 @@ int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
   if (d->M == 0 || d->N == 0) return 1;
   const bool dt_ok = (unsigned)d->a_dtype <= 1 && (unsigned)d->b_dtype <= 1 &&
                      (unsigned)d->c_dtype <= 1;
-  if (!dt_ok || (d->compute != PTN_COMPUTE_F64_4M && d->compute != PTN_COMPUTE_F32_3M)) {
+  if (!dt_ok || (d->compute != PTN_COMPUTE_F64_4M && d->compute != PTN_COMPUTE_F32_3M &&
+                 d->compute != PTN_COMPUTE_BF16_3M)) {
     ptn::set_error("ptn_gemm_gather: bad dtype/compute code");
     return PTN_EINVAL;
   }
@@ -1093,6 +1299,12 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   if (d->K == 0) a.kchunk = BK;
   if (d->compute == PTN_COMPUTE_F64_4M)
     return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles, ksplit);
+  if (d->compute == PTN_COMPUTE_BF16_3M) {
+    if (d->a_dtype == PTN_C128)
+      return dispatch_b<PTN_COMPUTE_BF16_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype,
+                                                      ksplit);
+    return dispatch_b<PTN_COMPUTE_BF16_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
+  }
   if (d->a_dtype == PTN_C128)
     return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
@@ -1144,6 +1356,11 @@ extern "C" int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *d, int3
   a.remap = (tiles % 8 == 0) ? 1 : 0;
   if (d->compute == PTN_COMPUTE_F64_4M)
     return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles, ks);
+  if (d->compute == PTN_COMPUTE_BF16_3M) {
+    if (d->a_dtype == PTN_C128)
+      return dispatch_b<PTN_COMPUTE_BF16_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
+    return dispatch_b<PTN_COMPUTE_BF16_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
+  }
   if (d->a_dtype == PTN_C128)
     return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
@@ -1164,6 +1381,10 @@ extern "C" int ptn_gemm_gather_grouped(void *stream, const ptn_gemm_desc *descs,
   std::vector<int64_t> prefix(1, 0);
   ka.reserve(count);
   const ptn_gemm_desc &d0 = descs[0];
+  if (d0.compute == PTN_COMPUTE_BF16_3M) {
+    ptn::set_error("ptn_gemm_gather_grouped: no grouped form of the BF16 compute mode");
+    return PTN_EINVAL;
+  }
   for (int i = 0; i < count; ++i) {
     const ptn_gemm_desc &d = descs[i];
     if (d.compute != d0.compute || d.a_dtype != d0.a_dtype || d.b_dtype != d0.b_dtype ||

@@ -33,10 +33,13 @@ Label = Hashable
 
 FP64 = "fp64"
 FP32 = "fp32"
+# 3M bf16 products with FP32 accumulation (the lowest level of RelaxedNetwork)
+BF16 = "bf16"
 # FP64 environments and Krylov applications, each application run in 3M FP32 when
 # its contribution to the Krylov result is small enough (RelaxedNetwork)
 RELAXED = "relaxed"
-_COMPUTE = {FP64: _lib.PTN_COMPUTE_F64_4M, FP32: _lib.PTN_COMPUTE_F32_3M}
+_COMPUTE = {FP64: _lib.PTN_COMPUTE_F64_4M, FP32: _lib.PTN_COMPUTE_F32_3M,
+            BF16: _lib.PTN_COMPUTE_BF16_3M}
 _DTYPE_CODE = {torch.complex128: _lib.PTN_C128, torch.complex64: _lib.PTN_C64}
 
 
@@ -1198,7 +1201,9 @@ class RelaxedNetwork:
     complex64 copies of the same environments and operator with 3M FP32 GEMMs
     (FP32 accumulation).  Calling the object runs ``high``.  The Krylov loop runs
     ``low`` for an application whose error cannot move the result by more than
-    ``relax_tol`` times the input's norm (``krylov.krylov_exp_action``): the error of the j-th
+    ``relax_tol`` times the input's norm (``krylov.krylov_exp_action``), and the
+    optional ``lower`` network (3M bf16 products, FP32 accumulation, relative error
+    ``lower_eps``) where even that error is small enough: the error of the j-th
     application enters the result weighted by the j-th expansion coefficient,
     which the loop estimates as it goes.  This is the relaxation of
     matrix-vector accuracy in inexact Krylov methods (Simoncini and Szyld,
@@ -1208,14 +1213,19 @@ class RelaxedNetwork:
     """
 
     LOW_EPS = 1e-6
+    LOWER_EPS = 3e-2
 
     def __init__(self, high: "CompiledNetwork", low: "CompiledNetwork", relax_tol: float,
-                 low_eps: Optional[float] = None):
-        if high.precision != FP64 or low.precision != FP32:
-            raise ValueError("RelaxedNetwork pairs an FP64 network with an FP32 one")
-        self.high, self.low = high, low
+                 low_eps: Optional[float] = None, lower: Optional["CompiledNetwork"] = None,
+                 lower_eps: Optional[float] = None):
+        if high.precision != FP64 or low.precision != FP32 or \
+                (lower is not None and lower.precision != BF16):
+            raise ValueError("RelaxedNetwork pairs an FP64 network with an FP32 one "
+                             "(and optionally a BF16 one)")
+        self.high, self.low, self.lower = high, low, lower
         self.relax_tol = float(relax_tol)
         self.low_eps = self.LOW_EPS if low_eps is None else float(low_eps)
+        self.lower_eps = self.LOWER_EPS if lower_eps is None else float(lower_eps)
 
     def __call__(self, x: torch.Tensor) -> torch.Tensor:
         return self.high(x)

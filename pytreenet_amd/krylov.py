@@ -240,7 +240,8 @@ class KrylovStats:
     def __init__(self):
         self.calls = 0
         self.applies = 0
-        self.low_applies = 0     # applications run in low precision (relaxed form)
+        self.low_applies = 0     # applications run in reduced precision (relaxed form)
+        self.lower_applies = 0   # of them, in bf16
         self.dims = []
 
     def record(self, m: int):
@@ -303,7 +304,9 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     weighted by at most tau times the coefficient of q_j, which the estimate of
     step j-1 approximates; the estimates decrease along the recurrence, so an
     older one (the look-ahead uses the estimate of step j-2) overstates the
-    weight.  The first two applications always run in full precision."""
+    weight.  The first two applications always run in full precision.  When the
+    relaxed network also holds a bf16 form, an application whose weight allows
+    lower_eps instead of low_eps runs in bf16."""
     E.require_device(psi)
     shape = psi.shape
     dev = psi.device
@@ -334,12 +337,20 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     t_norm = 0.0            # largest column sum of |T| so far
 
     def apply(k):
-        if low is not None and k >= 2 and tau * est * relaxed.low_eps * t_norm <= \
-                relaxed.relax_tol * beta0:
-            STATS.low_applies += 1
-            # complex64 input: every product of the FP32 network then has
-            # complex64 operands (strided ones run as rocBLAS cgemm)
-            return low(Q[k].view(shape).to(torch.complex64))
+        if low is not None and k >= 2:
+            weight = tau * est * t_norm
+            net = None
+            if relaxed.lower is not None and weight * relaxed.lower_eps <= \
+                    relaxed.relax_tol * beta0:
+                net = relaxed.lower
+                STATS.lower_applies += 1
+            elif weight * relaxed.low_eps <= relaxed.relax_tol * beta0:
+                net = low
+            if net is not None:
+                STATS.low_applies += 1
+                # complex64 input: every product of the reduced-precision network
+                # then has complex64 operands (strided FP32 ones run as rocBLAS cgemm)
+                return net(Q[k].view(shape).to(torch.complex64))
         return action(Q[k].view(shape))
 
     for j in range(max_steps):

@@ -78,6 +78,9 @@ class TDVPConfig(TTNTimeEvolutionConfig):
     # it the H_eff GEMMs are bound by memory and launches, where the second
     # network costs more than the FP32 arithmetic saves
     relax_min_size: int = 1 << 20
+    # precision="relaxed": also run applications of the smallest weights as 3M
+    # bf16 products (FP32 accumulation)
+    relax_bf16: bool = False
 
 
 @dataclass
@@ -486,14 +489,19 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             # and result, like the others (rocBLAS cgemm where it is strided)
             action = self._relaxed(action, E.CompiledNetwork(t32, l32, o32,
                                                              precision=E.FP32,
-                                                             out_dtype=torch.complex64))
+                                                             out_dtype=torch.complex64),
+                                   (t32, l32, o32))
 
         self.state.tensors[node_id] = yield from self._evolve(
             tensor, action, self.time_step_size * time_step_factor, EvoDirection.FORWARD)
         self.apply_log.append(("site", node_id, self._last_applies))
 
-    def _relaxed(self, high, low):
-        return E.RelaxedNetwork(high, low, getattr(self.config, "relax_tol", 1e-12))
+    def _relaxed(self, high, low, net32):
+        lower = None
+        if getattr(self.config, "relax_bf16", False):
+            lower = E.CompiledNetwork(*net32, precision=E.BF16, out_dtype=torch.complex64)
+        return E.RelaxedNetwork(high, low, getattr(self.config, "relax_tol", 1e-12),
+                                lower=lower)
 
     def _move_orth_and_update_cache_for_path(self, path: List[str]):
         if len(path) == 0:
@@ -664,7 +672,8 @@ class TwoSiteTDVP(TDVPAlgorithm):
                     [e.to(torch.complex64) for e in tensors[3:]]
                 return self._relaxed(high, E.CompiledNetwork([y] + t32, labels, out,
                                                              precision=E.FP32,
-                                                             out_dtype=torch.complex64))
+                                                             out_dtype=torch.complex64),
+                                     ([y] + t32, labels, out))
             return make, k1, k2
         return (lambda y: E.CompiledNetwork([y] + tensors[1:], labels, out,
                                             precision=self.precision)), k1, k2

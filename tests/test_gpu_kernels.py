@@ -577,3 +577,37 @@ def test_gemm_config_rejects_unknown(gpu):
         E._TUNED[key] = chosen
     finally:
         E.AUTOTUNE = False
+
+
+@pytest.mark.parametrize("case", [
+    ((4000, 37), "mk", (37, 2100), "kn", "mn", torch.complex64, (False, False)),  # LARGE grid
+    ((40, 100, 3, 13), "abkc", (13, 3, 2100), "ckn", "nab", torch.complex64, (True, False)),
+    ((100, 3000), "mk", (3000, 90), "kn", "nm", torch.complex128, (False, True)),  # split K
+    ((70, 33), "mk", (33, 50), "kn", "mn", torch.complex64, (False, False)),       # ragged
+])
+def test_gather_gemm_bf16_3m(gpu, case):
+    """The BF16 compute mode (3M bf16 products, FP32 accumulation) against einsum:
+    within bf16 accuracy, and measurably less accurate than FP32 (so the bf16
+    kernel, not an FP32 one, ran); accumulation with alpha."""
+    sa, la, sb, lb, lo, adt, conj = case
+    _say(f"gather bf16 3M {la},{lb}->{lo} A={adt} conj={conj}")
+    rng = np.random.default_rng(sum(sa) + 7)
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    ref = np.einsum(f"{la},{lb}->{lo}", a.conj() if conj[0] else a, b.conj() if conj[1] else b)
+    ta, tb = _dev(a, adt), _dev(b, torch.complex64)
+    M = int(np.prod([d for d, l in zip(sa, la) if l not in lb]))
+    N = int(np.prod([d for d, l in zip(sb, lb) if l not in la]))
+    K = int(np.prod([d for d, l in zip(sa, la) if l in lb]))
+    assert E.gemm_kernel_class(M, N, K, E.BF16) != "wave"
+    got = E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.BF16,
+                     out_dtype=torch.complex64).cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert 1e-4 < err < 3e-2, err
+    base = _crand(rng, ref.shape)
+    out = _dev(base, torch.complex64)
+    base = out.cpu().numpy().astype(np.complex128)
+    E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.BF16, out=out,
+               accumulate=True, alpha=-0.5 + 0.25j)
+    want = base + (-0.5 + 0.25j) * ref
+    err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
+    assert err < 3e-2, err

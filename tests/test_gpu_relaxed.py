@@ -129,17 +129,19 @@ def test_relaxed_rejected_in_batched(gpu):
 
 def test_config3_relaxed_20_steps_energy_error(gpu):
     """The chi=256 configuration (ternary tree N=40, random XYZ, dt=0.05) over its
-    20 steps: the relaxed path at the benchmark's relax_tol stays within
-    |dE| <= 1e-8 of the FP64 path, and so do <Z_i> on four sites (root, level 1,
-    level 2, leaf), while part of the applications ran in FP32."""
+    20 steps: the relaxed path at the benchmark's relax_tol, with and without the
+    bf16 level, stays within |dE| <= 1e-8 of the FP64 path, and so do <Z_i> on
+    four sites (root, level 1, level 2, leaf), while part of the applications ran
+    in reduced precision."""
     _say("config C3 full size, 20 steps, fp64 vs relaxed")
     import bench
     from pytreenet_amd import engine as E
     res = {}
-    for prec in ("fp64", "relaxed"):
+    for run in ("fp64", "relaxed", "relaxed_bf16"):
+        prec = "fp64" if run == "fp64" else "relaxed"
         st, H, _ = bench.make_problem(256, torch.device("cuda"))
         cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=prec,
-                             relax_tol=bench.RELAX_TOL)
+                             relax_tol=bench.RELAX_TOL, relax_bf16=run == "relaxed_bf16")
         alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 1.0, [], config=cfg,
                                          solver_options={"max_krylov_dim": 10,
                                                          "krylov_tol": 1e-10})
@@ -150,17 +152,19 @@ def test_config3_relaxed_20_steps_energy_error(gpu):
         for step in range(20):
             alg.run_one_time_step()
         sites = ["site0", "site1", "site4", "site13"]
-        res[prec] = (ptn.expectation_value(alg.state, H).real,
-                     [alg.state.single_site_operator_expectation_value(n, PAULI_Z).real
-                      for n in sites], K.STATS.low_applies - low0)
-        _say(f"  {prec}: E={res[prec][0]:.15f} low applications {res[prec][2]}")
+        res[run] = (ptn.expectation_value(alg.state, H).real,
+                    [alg.state.single_site_operator_expectation_value(n, PAULI_Z).real
+                     for n in sites], K.STATS.low_applies - low0)
+        _say(f"  {run}: E={res[run][0]:.15f} reduced-precision applications {res[run][2]}")
         del alg, st, H
         E.clear_caches()
         torch.cuda.empty_cache()
-    (e64, z64, _), (er, zr, low) = res["fp64"], res["relaxed"]
-    assert low > 0
-    assert abs(er - e64) <= 1e-8, (er, e64)
-    assert max(abs(a - b) for a, b in zip(zr, z64)) <= 1e-8, (zr, z64)
+    e64, z64, _ = res["fp64"]
+    for run in ("relaxed", "relaxed_bf16"):
+        er, zr, low = res[run]
+        assert low > 0, run
+        assert abs(er - e64) <= 1e-8, (run, er, e64)
+        assert max(abs(a - b) for a, b in zip(zr, z64)) <= 1e-8, (run, zr, z64)
 
 
 def test_relaxed_min_size_keeps_small_spaces_fp64(gpu):
@@ -181,3 +185,25 @@ def test_relaxed_min_size_keeps_small_spaces_fp64(gpu):
     errs = _run(alg, otd, 2)
     assert K.STATS.low_applies == low0
     assert max(errs) < 1e-12, errs
+
+
+def test_relaxed_bf16_level(gpu):
+    """With relax_bf16 the applications of the smallest weights run as bf16
+    products; the run stays within 1e-9 of the oracle (relax_tol 1e-10)."""
+    _say("relaxed precision with the bf16 level")
+    ids, parent, children = complete_tree(2, 3)
+    tensors = random_state_tensors(ids, parent, children, 16, 5)
+    st = build_state(ids, parent, children, tensors, device="cuda")
+    H = ptn.random_xyz_model(st, 5)
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision="relaxed",
+                         relax_tol=1e-10, relax_min_size=0, relax_bf16=True)
+    alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.15, [], config=cfg,
+                                     solver_options={"krylov_tol": 1e-10, "max_krylov_dim": 10})
+    otd = OracleTDVP(OracleState(topology_from_ttn(st), tensors), host_tensors(H), 0.05,
+                     kind="second1", krylov_tol=1e-10, max_krylov_dim=10)
+    b0, l0 = K.STATS.lower_applies, K.STATS.low_applies
+    errs = _run(alg, otd, 3)
+    _say(f"  bf16 applications {K.STATS.lower_applies - b0}, reduced {K.STATS.low_applies - l0};"
+         f" rel dE {max(errs):.2e}")
+    assert K.STATS.lower_applies > b0
+    assert max(errs) < 1e-9, errs
