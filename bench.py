@@ -14,7 +14,8 @@ local result by at most RELAX_TOL relative to its input; those run as 3M FP32
 MFMA or rocBLAS cgemm on complex64 copies of the environments) and the mixed path (complex64 environments, 3M FP32 MFMA;
 Krylov/QR/SVD FP64).  Each reduced-precision path reports its energy error
 against FP64 at the same time.  ``value`` is the sweeps/s of the fastest path
-with |dE| <= 1e-8 (FP64 meets it by definition).
+with |dE| <= 1e-8 (FP64 meets it by definition).  The relaxed path with the
+bf16 level is run and reported beside them.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--chi 256] [--no-cpu]
 """
@@ -273,7 +274,9 @@ def main():
                         if prec == "relaxed" else 1.0})
             if bf16:
                 res[f"{key}_bf16_applies"] = r["lower_fraction"]
-            if de <= 1e-8:
+            # the bf16 level meets the energy bar but lets <Z_i> drift to 1.6e-8
+            # over the configuration's 20 steps: reported, not taken as `value`
+            if de <= 1e-8 and not bf16:
                 ok.append((key if prec == "relaxed" else prec, r, sps))
     path, run, value = max(ok, key=lambda t: t[2])
     # the roofline follows the path whose rate is `value`
