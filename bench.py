@@ -39,8 +39,8 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 # the summaries are under profiles/); None where not measured
 TRAFFIC_BYTES_PER_LAUNCH = {("gather", "fp64"): 2.75e9, ("blas", "fp64"): None,
                             # mean over the relaxed run's timed LARGE FP32 launches
-                            # (112 at 0.88 GB, 56 at 3.04 GB)
-                            ("gather", "fp32"): 1.60e9, ("blas", "fp32"): None,
+                            # (112 with K=256 at 0.54 GB, 56 with K=1280 at 2.23 GB)
+                            ("gather", "fp32"): 1.11e9, ("blas", "fp32"): None,
                             ("gather", "bf16"): None}
 KERNEL_NAME = {("gather", "fp64"): "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, FP64 3M)",
                ("blas", "fp64"): "rocblas_zgemm (ptn_gemm_strided)",
