@@ -1151,6 +1151,14 @@ class CompiledNetwork:
                 fill_(T, op[2])
         return out
 
+    def release(self) -> None:
+        """Drop the recording and the intermediates it keeps (a later call records
+        again).  For callers that hold many networks alive at once."""
+        self.ops = None
+        self.sig = None
+        self.rec_inputs = self.rec_out = None
+        self.op_roles = None
+
     def _record(self, var, sig) -> torch.Tensor:
         global _RECORDER
         ts = list(self.tensors)
@@ -1229,6 +1237,11 @@ class RelaxedNetwork:
 
     def __call__(self, x: torch.Tensor) -> torch.Tensor:
         return self.high(x)
+
+    def release(self) -> None:
+        for net in (self.high, self.low, self.lower):
+            if net is not None:
+                net.release()
 
 
 # --------------------------------------------------------------------------------------
