@@ -21,6 +21,8 @@ with the upward pass only), ``backward`` recomputes both passes.
 """
 from __future__ import annotations
 
+import math
+
 from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -112,6 +114,12 @@ def rayleigh_quotient_gradients(state: TreeTensorNetworkState, ttno: TreeTensorN
     from .krylov import dotc
     num = dotc(A, hy).real
     den = dotc(A, norm_apply(state, r, A, ncache)).real
+    if not (math.isfinite(num) and math.isfinite(den)) or den <= 0.0:
+        # complex64 environments of an unnormalised state overflow (the subtree
+        # norms of random tensors grow geometrically with the subtree size)
+        raise FloatingPointError(
+            f"<psi|H|psi> = {num}, <psi|psi> = {den} in precision {precision!r}: "
+            "bring the state to canonical form (environments of order one) or use fp64")
     energy = num / den
     gsite: Dict[str, torch.Tensor] = {}
     gop: Optional[Dict[str, torch.Tensor]] = {} if with_operator else None

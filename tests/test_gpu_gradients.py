@@ -7,6 +7,7 @@ import pytreenet_amd as ptn
 from oracle import contract as OC
 from oracle import dense as OD
 from oracle.tdvp import OracleState
+from pytreenet_amd import engine as E
 from pytreenet_amd import gradients as G
 from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
 from pytreenet_amd.ttno import PAULI_X, build_edge_ttno_tensors
@@ -169,3 +170,30 @@ def test_config5_full_size_gradients(gpu):
             st.tensors[nid] = t0
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - expect) < 1e-6 * max(1.0, abs(expect)), (nid, idx, fd, expect)
+
+
+def test_config5_descent_200_steps_fp64_and_fp32(gpu):
+    """The differentiation configuration's optimisation: 200 gradient-descent steps
+    of the Rayleigh quotient on the 63-node binary tree at chi=128 (TFIM, per-edge
+    couplings), from the canonical form, in complex128 and in complex64 storage
+    (3M FP32 contractions).  The energy falls by more than 50 (measured: -1.70 ->
+    -60.62) and the two precisions end within 1e-6 relative of each other; an
+    unnormalised state in complex64 raises instead of returning NaN."""
+    _say("config C5 full size, 200 descent steps, fp64 and fp32")
+    res = {}
+    for prec in (E.FP64, E.FP32):
+        st, model, J, _ = _problem(seed=11, chi=128, depth=5)
+        H = model.ttno(J)
+        if prec == E.FP32:
+            with pytest.raises(FloatingPointError):
+                G.rayleigh_quotient_gradients(st, H, E.FP32, with_operator=False)
+        st.canonical_form(st.root_id, mode=ptn.SplitMode.KEEP)
+        nrm = ptn.scalar_product(st, st).real
+        st.tensors[st.root_id] = st.tensors[st.root_id] / nrm ** 0.5
+        hist = G.ground_state_descent(st, H, steps=200, lr=0.05, precision=prec)
+        assert all(np.isfinite(hist))
+        res[prec] = hist
+        _say(f"  {prec}: E {hist[0]:.6f} -> {hist[-1]:.6f}")
+    h64, h32 = res[E.FP64], res[E.FP32]
+    assert h64[-1] < h64[0] - 50
+    assert abs(h32[-1] - h64[-1]) < 1e-6 * abs(h64[-1])
