@@ -93,9 +93,13 @@ __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
 // instantiated separately for launches with a large tile grid (>= LARGE_TILES
 // tiles, no split-K), so profilers report the dominant H_eff / environment
 // GEMMs apart from the many small ones.
-template <int MODE, typename TA, typename TB, typename TC>
+// TR (FP32 only): the transposed block product (MFMA operands swapped), so the
+// accumulators run along m and the epilogue stores are contiguous when C is
+// m-contiguous.
+template <int MODE, typename TA, typename TB, typename TC, bool TR = false>
 __device__ __forceinline__ void gather_tile(const KArgs &p, int64_t bid, int64_t split,
                                             bool splitk) {
+  static_assert(!TR || MODE == PTN_COMPUTE_F32_3M, "transposed stores: FP32 form only");
   typedef typename Acc<MODE>::T T;
   typedef typename Acc<MODE>::V V;
   __shared__ T sAr[2][BK][BM + PTN_GEMM_PAD];
@@ -263,9 +267,15 @@ __device__ __forceinline__ void gather_tile(const KArgs &p, int64_t bid, int64_t
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
-            c0[i][j] = mfma(ar[kk][i], br[kk][j], c0[i][j]);
-            c1[i][j] = mfma(ai[kk][i], bi[kk][j], c1[i][j]);
-            c2[i][j] = mfma(as[i], bs[j], c2[i][j]);
+            if constexpr (TR) {
+              c0[i][j] = mfma(br[kk][j], ar[kk][i], c0[i][j]);
+              c1[i][j] = mfma(bi[kk][j], ai[kk][i], c1[i][j]);
+              c2[i][j] = mfma(bs[j], as[i], c2[i][j]);
+            } else {
+              c0[i][j] = mfma(ar[kk][i], br[kk][j], c0[i][j]);
+              c1[i][j] = mfma(ai[kk][i], bi[kk][j], c1[i][j]);
+              c2[i][j] = mfma(as[i], bs[j], c2[i][j]);
+            }
           }
       }
     }
@@ -277,18 +287,21 @@ __device__ __forceinline__ void gather_tile(const KArgs &p, int64_t bid, int64_t
   TC *C = static_cast<TC *>(p.C);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int64_t gn = n0 + wn * 32 + j * 16 + fr;
-    if (gn >= p.N) continue;
-    const int64_t noff = p.con[gn];
+    const int64_t gn0 = n0 + wn * 32 + j * 16 + fr;
+    if (!TR && gn0 >= p.N) continue;
+    const int64_t noff0 = (!TR) ? p.con[gn0] : 0;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         // accumulator row of (lane, r): f64 16x16x4 -> 4*r + lane/16,
-        // f32 16x16x4 -> 4*(lane/16) + r
+        // f32 16x16x4 -> 4*(lane/16) + r; with TR the block is C^T: that row is
+        // an n index and the column lane%16 an m index
         const int rr = (MODE == PTN_COMPUTE_F64_4M) ? (r * 4 + fk) : (fk * 4 + r);
-        const int64_t gm = m0 + wm * 32 + i * 16 + rr;
-        if (gm >= p.M) continue;
+        const int64_t gm = TR ? m0 + wm * 32 + i * 16 + fr : m0 + wm * 32 + i * 16 + rr;
+        const int64_t gn = TR ? n0 + wn * 32 + j * 16 + rr : gn0;
+        if (gm >= p.M || gn >= p.N) continue;
+        const int64_t noff = TR ? p.con[gn] : noff0;
         double re, im;
         if constexpr (MODE == PTN_COMPUTE_F64_4M) {
           re = c0[i][j][r];
@@ -521,9 +534,9 @@ __device__ __forceinline__ int64_t remapped_tile(const KArgs &p) {
   return bid;
 }
 
-template <int MODE, typename TA, typename TB, typename TC, int LARGE>
+template <int MODE, typename TA, typename TB, typename TC, int LARGE, bool TR = false>
 __global__ __launch_bounds__(NT, PTN_GEMM_OCC) void gemm_gather_kernel(KArgs p) {
-  gather_tile<MODE, TA, TB, TC>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
+  gather_tile<MODE, TA, TB, TC, TR>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
 }
 
 template <typename TA, typename TB, typename TC, int LARGE>
@@ -1160,9 +1173,15 @@ int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
       else
         hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC, 0>), dim3((unsigned)tiles, ksplit),
                            dim3(NT), 0, s, a);
-    } else if (large)
+    } else if (large && a.cmaj)
+      hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1, true>),
+                         dim3((unsigned)tiles, 1), dim3(NT), 0, s, a);
+    else if (large)
       hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
                          dim3(NT), 0, s, a);
+    else if (a.cmaj && ksplit == 1)
+      hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 0, true>),
+                         dim3((unsigned)tiles, 1), dim3(NT), 0, s, a);
     else
       hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 0>),
                          dim3((unsigned)tiles, ksplit), dim3(NT), 0, s, a);
