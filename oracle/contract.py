@@ -15,9 +15,19 @@ makes, so the CPU timing reflects the reference path:
 
 * ``env_update``       -- ``contract_any`` ket-first branch
   (``state_operator_contraction.py:407-461,533-694``; ``contraction_util.py:176-200``).
-  The operator-first branch (``compare_contr_orders``, ``:142-162``) is chosen
-  only when (d-1)*prod(D)*prod(chi) < d*(prod(D)-prod(chi)), which never holds
-  for d = 2 and chi >= 1, so it is not restated.
+  ``compare_contr_orders`` (``:142-162``) picks the operator-first branch when
+  (d-1)*prod(D)*prod(chi) < d*(prod(D)-prod(chi)).  For d = 2 that happens only
+  when the ket node's virtual dimensions multiply to 1 and prod(D) >= 3 (any
+  virtual product p >= 2 gives prod(D)*p >= 2 prod(D) > 2(prod(D) - p)), i.e.
+  for product-state tensors with all bonds of dimension 1.  The branch changes
+  only the summation order (rounding-level differences), so it is not restated;
+  every tensor of the benchmark and test states has bonds of dimension >= 2.
+* Operator legs: the TTNO's children order may differ from the state's.  The
+  reference finds the operator leg of each ket neighbour by identifier
+  (``contraction_util.py:40-80``, ``get_equivalent_legs``); here every function
+  takes an optional ``op_topo`` (the operator's own tree, same nodes and
+  parents, its own children order; default: the state's) and looks the W leg
+  of a ket neighbour up in it (``_wleg``).
 * ``heff_apply``       -- ``contract_ket_ham_with_envs`` (``:227-273``) via
   ``contract_all_neighbour_blocks_to_ket`` (``contraction_util.py:202-290``).
 * ``bond_apply``       -- ``contract_bond_tensor`` (``:984-1040``).
@@ -39,6 +49,11 @@ def _phys_leg(tensor: np.ndarray) -> int:
     return tensor.ndim - 1
 
 
+def _wleg(topo: Topology, op_topo, nid: str, nb: str) -> int:
+    """Leg of the operator tensor at ``nid`` that meets neighbour ``nb``."""
+    return (op_topo or topo).neighbour_index(nid, nb)
+
+
 def env_leaf(A: np.ndarray, W: np.ndarray, B: np.ndarray) -> np.ndarray:
     """contract_leaf: A (p, phys), W (p, out, in), B = conj(A) (p, phys) -> (ket, ham, bra)."""
     bra_ham = np.tensordot(W, B, axes=([1], [1]))          # (w, in, b)
@@ -46,7 +61,7 @@ def env_leaf(A: np.ndarray, W: np.ndarray, B: np.ndarray) -> np.ndarray:
 
 
 def env_update(topo: Topology, nid: str, to: str, A: np.ndarray, W: np.ndarray,
-               envs: Env, B: np.ndarray = None) -> np.ndarray:
+               envs: Env, B: np.ndarray = None, op_topo: Topology = None) -> np.ndarray:
     """Directed environment E[(nid, to)] (ket-first reference order)."""
     if B is None:
         B = A.conj()
@@ -66,7 +81,7 @@ def env_update(topo: Topology, nid: str, to: str, A: np.ndarray, W: np.ndarray,
     # contract_operator_tensor_ignoring_one_leg: W legs (nbs..., out, in)
     nrest = len(nbs) - 1
     t_legs = [2 + 2 * i for i in range(nrest)] + [1]
-    w_legs = [i for i in range(len(nbs)) if i != ignore] + [len(nbs) + 1]
+    w_legs = [_wleg(topo, op_topo, nid, nb) for nb in nbs if nb != to] + [len(nbs) + 1]
     t = np.tensordot(t, W, axes=(t_legs, w_legs))
     # t legs: (a_to, b_1..b_j, w_to, out)
     # contract_bra_tensor_ignore_one_leg: B legs (nbs..., phys)
@@ -76,7 +91,7 @@ def env_update(topo: Topology, nid: str, to: str, A: np.ndarray, W: np.ndarray,
 
 
 def heff_apply(topo: Topology, nid: str, x: np.ndarray, W: np.ndarray,
-               envs: Env) -> np.ndarray:
+               envs: Env, op_topo: Topology = None) -> np.ndarray:
     """y = H_eff x at node ``nid`` (reference order: all envs into the ket, then W)."""
     t = x
     nbs = topo.neighbours(nid)
@@ -85,7 +100,7 @@ def heff_apply(topo: Topology, nid: str, x: np.ndarray, W: np.ndarray,
     # t legs: (phys, w_1, b_1, ..., w_k, b_k)
     k = len(nbs)
     t_legs = [1 + 2 * i for i in range(k)] + [0]
-    w_legs = list(range(k)) + [k + 1]
+    w_legs = [_wleg(topo, op_topo, nid, nb) for nb in nbs] + [k + 1]
     return np.tensordot(t, W, axes=(t_legs, w_legs))        # (b_1..b_k, out)
 
 
@@ -97,7 +112,8 @@ def bond_apply(x: np.ndarray, env_p: np.ndarray, env_c: np.ndarray) -> np.ndarra
 
 
 def two_site_heff_apply(topo: Topology, n1: str, n2: str, x: np.ndarray,
-                        W1: np.ndarray, W2: np.ndarray, envs: Env) -> np.ndarray:
+                        W1: np.ndarray, W2: np.ndarray, envs: Env,
+                        op_topo: Topology = None) -> np.ndarray:
     """Two-site action on x with legs (outer neighbours of n1, outer of n2, p1, p2).
 
     The outer neighbours are listed in each node's neighbour order, skipping the
@@ -107,13 +123,17 @@ def two_site_heff_apply(topo: Topology, n1: str, n2: str, x: np.ndarray,
     """
     nb1 = [n for n in topo.neighbours(n1) if n != n2]
     nb2 = [n for n in topo.neighbours(n2) if n != n1]
-    i12 = topo.neighbour_index(n1, n2)
-    i21 = topo.neighbour_index(n2, n1)
-    # merged operator legs: (outer1..., outer2..., out1, in1, out2, in2)
+    otopo = op_topo or topo
+    i12 = otopo.neighbour_index(n1, n2)
+    i21 = otopo.neighbour_index(n2, n1)
+    # merged operator legs: (outer1..., out1, in1, outer2..., out2, in2) with the
+    # outer legs in the operator's own neighbour order
     Wm = np.tensordot(W1, W2, axes=([i12], [i21]))
     k1, k2 = len(nb1), len(nb2)
-    # reorder to (outer1, outer2, out1, out2, in1, in2)
-    perm = list(range(k1)) + list(range(k1 + 2, k1 + 2 + k2)) + \
+    onb1 = [n for n in otopo.neighbours(n1) if n != n2]
+    onb2 = [n for n in otopo.neighbours(n2) if n != n1]
+    # reorder to (outer1, outer2 in the state's neighbour order, out1, out2, in1, in2)
+    perm = [onb1.index(n) for n in nb1] + [k1 + 2 + onb2.index(n) for n in nb2] + \
         [k1, k1 + 2 + k2, k1 + 1, k1 + 3 + k2]
     Wm = Wm.transpose(perm)
     t = x
@@ -126,19 +146,19 @@ def two_site_heff_apply(topo: Topology, n1: str, n2: str, x: np.ndarray,
     return np.tensordot(t, Wm, axes=(t_legs, w_legs))       # (b..., out1, out2)
 
 
-def build_envs_to_root(topo: Topology, A: dict, W: dict) -> Env:
+def build_envs_to_root(topo: Topology, A: dict, W: dict, op_topo: Topology = None) -> Env:
     envs: Env = {}
     for nid in topo.linearise()[:-1]:
         p = topo.parent[nid]
-        envs[(nid, p)] = env_update(topo, nid, p, A[nid], W[nid], envs)
+        envs[(nid, p)] = env_update(topo, nid, p, A[nid], W[nid], envs, op_topo=op_topo)
     return envs
 
 
-def expectation_value(topo: Topology, A: dict, W: dict) -> complex:
+def expectation_value(topo: Topology, A: dict, W: dict, op_topo: Topology = None) -> complex:
     """<psi|H|psi> (unnormalised), children-first, root contracted last."""
-    envs = build_envs_to_root(topo, A, W)
+    envs = build_envs_to_root(topo, A, W, op_topo)
     r = topo.root
-    y = heff_apply(topo, r, A[r], W[r], envs)
+    y = heff_apply(topo, r, A[r], W[r], envs, op_topo=op_topo)
     return complex(np.vdot(A[r], y))
 
 

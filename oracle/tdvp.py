@@ -87,11 +87,14 @@ class OracleTDVP:
 
     def __init__(self, state: OracleState, W: Dict[str, np.ndarray], dt: float,
                  kind: str = "second1", krylov_tol: float = 1e-10,
-                 max_krylov_dim: Optional[int] = 10, svd_params: Optional[dict] = None):
-        assert kind in ("first1", "second1", "second2")
+                 max_krylov_dim: Optional[int] = 10, svd_params: Optional[dict] = None,
+                 op_topo: Optional[Topology] = None):
+        assert kind in ("first1", "second1", "second2", "first2")
         self.s = state.copy()
         self.topo = state.topo
         self.W = W
+        # the operator's own tree (its children order may differ from the state's)
+        self.op_topo = op_topo
         self.dt = dt
         self.kind = kind
         self.kopts = dict(krylov_tol=krylov_tol, max_krylov_dim=max_krylov_dim)
@@ -99,7 +102,7 @@ class OracleTDVP:
         self.update_path = self.topo.update_path()
         self.orth_path = self.topo.orthogonalisation_path(self.update_path)
         self.back_path = list(reversed(self.update_path))
-        if kind == "second2":
+        if kind in ("second2", "first2"):
             self.back_orth = [list(reversed(p)) for p in reversed(self.orth_path)]
         else:
             self.back_orth = self.topo.second_order_one_site_back_orth_path(self.back_path)
@@ -123,7 +126,7 @@ class OracleTDVP:
 
     def update_env(self, nid: str, to: str):
         self.envs[(nid, to)] = C.env_update(self.topo, nid, to, self.s.A[nid],
-                                            self.W[nid], self.envs)
+                                            self.W[nid], self.envs, op_topo=self.op_topo)
 
     def _krylov(self, x, action, dt, exp_factor):
         st = {}
@@ -134,7 +137,8 @@ class OracleTDVP:
     # -- elementary updates ----------------------------------------------------
     def update_site(self, nid: str, factor: float = 1.0):
         assert self.s.center == nid
-        act = lambda y: C.heff_apply(self.topo, nid, y, self.W[nid], self.envs)
+        act = lambda y: C.heff_apply(self.topo, nid, y, self.W[nid], self.envs,
+                                     op_topo=self.op_topo)
         self.s.A[nid] = self._krylov(self.s.A[nid], act, self.dt * factor, -1j)
 
     def move_orth_and_update_cache(self, path: List[str]):
@@ -208,7 +212,7 @@ class OracleTDVP:
         perm = list(range(k1)) + list(range(k1 + 1, k1 + 1 + k2)) + [k1, k1 + 1 + k2]
         x = np.transpose(x, perm)
         act = lambda y: C.two_site_heff_apply(topo, n1, n2, y, self.W[n1], self.W[n2],
-                                              self.envs)
+                                              self.envs, op_topo=self.op_topo)
         x = self._krylov(x, act, self.dt * factor, -1j)
         u_legs = list(range(k1)) + [k1 + k2]
         v_legs = list(range(k1, k1 + k2)) + [k1 + k2 + 1]
@@ -245,17 +249,33 @@ class OracleTDVP:
                 self.update_site(target, -0.5)
                 self.two_site_update(target, bp[i + 1], 0.5)
 
+    def _first_order_two_site_step(self):
+        """First-order two-site sweep (``firstordertwosite.py:9-30``, with the pairs
+        and moves of the second-order forward sweep: the reference loop pairs
+        non-adjacent nodes), then the centre back to the start with the cache."""
+        up, op = self.update_path, self.orth_path
+        for i, nid in enumerate(up[:-2]):
+            if i > 0:
+                self.move_orth_and_update_cache(op[i - 1])
+            nxt = op[i][0]
+            self.two_site_update(nid, nxt, 1.0)
+            self.update_site(nxt, -1.0)
+        self.two_site_update(up[-2], up[-1], 1.0)
+        self.move_orth_and_update_cache(self.topo.path_from_to(self.s.center, up[0]))
+
     def run_one_time_step(self):
         if self.kind == "first1":
             self._first_order_step()
         elif self.kind == "second1":
             self._second_order_one_site_step()
+        elif self.kind == "first2":
+            self._first_order_two_site_step()
         else:
             self._second_order_two_site_step()
 
     # -- observables -------------------------------------------------------------
     def energy(self) -> complex:
-        return C.expectation_value(self.topo, self.s.A, self.W)
+        return C.expectation_value(self.topo, self.s.A, self.W, op_topo=self.op_topo)
 
     def expect_site(self, nid: str, op: np.ndarray) -> complex:
         return C.single_site_expectation(self.topo, self.s.A, nid, op)

@@ -21,12 +21,17 @@ from .splitting import (SplitMode, SVDParameters, ContractionMode, tensor_qr_dec
                         renormalise_singular_values)
 from .time_evolution import (TDVPConfig, TwoSiteTDVPConfig, TDVPAlgorithm, OneSiteTDVP,
                              FirstOrderOneSiteTDVP, SecondOrderOneSiteTDVP, TwoSiteTDVP,
+                             FirstOrderTwoSiteTDVP,
                              SecondOrderTwoSiteTDVP, Results, find_orthogonalisation_path,
                              BatchedTDVP, TTNTimeEvolution)
 from .tebd import TEBD, SWAPlist, TrotterSplitting, TrotterStep, swap_gate
 from .bug import (BUG, FixedBUG, BUGConfig, FixedBUGConfig, CommonBUGConfig, svd_truncation,
                   find_path_leaf_to_leaf)
 from .dmrg import DMRGAlgorithm
+from .effective_hamiltonians import (find_tensor_leg_permutation, contract_all_except_node,
+                                     get_effective_single_site_hamiltonian,
+                                     get_effective_two_site_hamiltonian,
+                                     get_effective_bond_hamiltonian)
 from .random_states import random_tree_state, complete_tree, build_state
 from . import engine
 

@@ -334,3 +334,69 @@ def tensor_product_expectation_value(state: TreeTensorNetworkState,
         for c in node.children:
             cache.pop((c, nid))
     return complex(block(state.root_id, None).cpu().item())
+
+
+def _as_matrix(op, nid: str):
+    if isinstance(op, TensorProduct):
+        if list(op.keys()) != [nid]:
+            raise ValueError(f"operator {op} is not a single-site operator on {nid}")
+        op = op[nid]
+    return np.asarray(op)
+
+
+def multi_single_site_expectation_value(ttns: TreeTensorNetworkState, operators,
+                                        move_orth: bool = False) -> Dict[str, List[complex]]:
+    """<psi| O |psi> (unnormalised) for many single-site operators in two passes
+    (``ttns/ttns_util.py:95-158``).
+
+    ``operators`` maps node ids to a matrix, a single-site ``TensorProduct`` or a
+    list of them (or is one ``TensorProduct`` of single-site factors); the result
+    maps each node id to the list of its values.  One upward and one downward pass
+    build every directed ket/bra block of the tree (2(N-1) contractions), and each
+    value is one contraction of the node with its operator and its blocks, so the
+    cost is O(N) contractions instead of one full sweep per operator.  The
+    reference moves the orthogonality centre to the root and uses identity blocks
+    below it; these blocks are contracted, so the values are exact in any gauge
+    and the state is not modified (``move_orth`` moves the centre to the root
+    afterwards, with KEEP QR, as the reference's option does).
+    """
+    from .splitting import SplitMode
+    if isinstance(operators, TensorProduct):
+        prepared = {nid: [op] for nid, op in operators.items()}
+    else:
+        prepared = {nid: (list(ops) if isinstance(ops, (list, tuple)) else [ops])
+                    for nid, ops in operators.items()}
+    dev = ttns.device
+    blocks: Dict[Tuple[str, str], torch.Tensor] = {}
+    for nid in ttns.linearise()[:-1]:          # children first: blocks towards the root
+        node = ttns.nodes[nid]
+        nbs = node.neighbouring_nodes()
+        blocks[(nid, node.parent)] = _overlap_block(
+            node, ttns.tensors[nid], ttns.tensors[nid], [blocks.get((nb, nid)) for nb in nbs],
+            0, True)
+    for nid in reversed(ttns.linearise()):     # parents first: blocks away from the root
+        node = ttns.nodes[nid]
+        nbs = node.neighbouring_nodes()
+        for c in node.children:
+            blocks[(nid, c)] = _overlap_block(
+                node, ttns.tensors[nid], ttns.tensors[nid],
+                [blocks.get((nb, nid)) for nb in nbs], nbs.index(c), True)
+    out: Dict[str, List[complex]] = {}
+    for nid, ops in prepared.items():
+        node, ket = ttns[nid]
+        k = node.nneighbours()
+        base_t = [ket, ket]
+        base_l = [[("a", i) for i in range(k)] + ["p"], [("b", i) for i in range(k)] + ["q"]]
+        for i, nb in enumerate(node.neighbouring_nodes()):
+            base_t.append(blocks[(nb, nid)])
+            base_l.append([("a", i), ("b", i)])
+        vals = []
+        for op in ops:
+            m = torch.as_tensor(_as_matrix(op, nid), dtype=torch.complex128).to(dev)
+            res = E.contract_network(base_t + [m], base_l + [["q", "p"]], [],
+                                     conj=[False, True] + [False] * (k + 1))
+            vals.append(res)
+        out[nid] = [complex(v) for v in torch.stack(vals).cpu().tolist()] if vals else []
+    if move_orth:
+        ttns.ensure_root_orth_center(mode=SplitMode.KEEP)
+    return out

@@ -47,25 +47,21 @@ def storage_dtype(precision: str) -> torch.dtype:
     return torch.complex128 if precision == FP64 else torch.complex64
 
 
-_STREAM: Optional[int] = None
+_raw_stream = torch._C._cuda_getCurrentRawStream
+_cur_device = torch._C._cuda_getDevice
 
 
 def stream_handle() -> int:
-    """HIP stream the library launches on: torch's current stream, cached.
-
-    Call ``refresh_stream()`` after switching torch streams; the time-evolution
-    drivers refresh it at the start of every step.
-    """
-    global _STREAM
-    if _STREAM is None:
-        _STREAM = torch.cuda.current_stream().cuda_stream
-    return _STREAM
+    """HIP stream the library launches on: torch's current stream on the current
+    device, read at every call (two C calls, no Python stream object), so work
+    issued inside a ``torch.cuda.stream(...)`` block is ordered with the
+    allocations torch makes on that stream."""
+    return _raw_stream(_cur_device())
 
 
 def refresh_stream() -> int:
-    global _STREAM
-    _STREAM = torch.cuda.current_stream().cuda_stream
-    return _STREAM
+    """Kept for callers that refreshed a cached handle; the handle is never cached."""
+    return stream_handle()
 
 
 def require_device(*ts: torch.Tensor):

@@ -25,7 +25,7 @@ from __future__ import annotations
 import ctypes
 import math
 from enum import Enum
-from typing import Any, Callable, Dict, Optional, Union
+from typing import Any, Callable, Dict, List, Optional, Union
 
 import numpy as np
 import torch
@@ -277,15 +277,30 @@ def _orthonormalize_chunked(lib, stream, n: int, Q: torch.Tensor, nvec: int, w: 
             first = tmp[cnt].real.clone()
         if last:
             packed[nvec + 1] = torch.complex(first, tmp[cnt].imag)
-_PINNED: Dict[int, torch.Tensor] = {}
+_PINNED: List[Optional[torch.Tensor]] = [None]
 
 
 def _pinned(count: int) -> torch.Tensor:
-    buf = _PINNED.get(count)
-    if buf is None:
-        buf = torch.empty(count, dtype=torch.complex128, pin_memory=True)
-        _PINNED[count] = buf
-    return buf
+    """Pinned host staging of at least ``count`` complex128 scalars.  One buffer,
+    grown by doubling and reused by every solve (each solve has read it back
+    before it returns), so its size follows the longest recurrence run."""
+    buf = _PINNED[0]
+    if buf is None or buf.numel() < count:
+        size = max(count, 64 if buf is None else 2 * buf.numel())
+        buf = torch.empty(size, dtype=torch.complex128, pin_memory=True)
+        _PINNED[0] = buf
+    return buf[:count]
+
+
+def _grow_hess(hess: np.ndarray, cap: int) -> np.ndarray:
+    """The (cap+1) x (cap+1) Hessenberg buffer holding ``hess`` in its top-left corner."""
+    out = np.zeros((cap + 1, cap + 1), dtype=complex)
+    out[: hess.shape[0], : hess.shape[1]] = hess
+    return out
+
+
+# columns of T held at the start of a recurrence; buffers double as it runs longer
+KRYLOV_START_CAP = 16
 
 
 def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.Tensor],
@@ -316,14 +331,18 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
     lib = _lib.load()
     work = _work(dev)
     stream = ctypes.c_void_p(E.stream_handle())
-    Q = torch.empty((min(max_steps + 1, 16), n), dtype=torch.complex128, device=dev)
+    # Q, T and the scalar staging grow by doubling (the reference grows its
+    # Hessenberg buffer the same way, time_evolution.py:782-787): with no
+    # max_krylov_dim the recurrence may run up to n steps, but it stops long
+    # before that, and buffers sized for n would need n^2 elements.
+    cap = min(max_steps, KRYLOV_START_CAP)
+    Q = torch.empty((cap + 1, n), dtype=torch.complex128, device=dev)
     # packed[0] = (||psi||^2, 0); packed[1..j+2) = T[:j+1, j]; packed[j+2] = (w0sq, beta^2)
-    packed = torch.empty(max_steps + 3, dtype=torch.complex128, device=dev)
+    packed = torch.empty(cap + 3, dtype=torch.complex128, device=dev)
     _lib.check(lib.ptn_normalize(stream, n, _vp(v0), _vp(Q[0]), _vp(packed), _vp(work)),
                "ptn_normalize")
-    hess = np.zeros((max_steps + 1, max_steps + 1), dtype=complex)
+    hess = np.zeros((cap + 1, cap + 1), dtype=complex)
     breakdown = np.finfo(float).eps * max(n, 1)
-    host_buf = _pinned(max_steps + 3)
     done = torch.cuda.Event()
     lookahead = n <= LOOKAHEAD_N
     beta0 = 0.0
@@ -354,11 +373,18 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
         return action(Q[k].view(shape))
 
     for j in range(max_steps):
-        if j + 2 > Q.shape[0]:
-            grown = torch.empty((min(2 * Q.shape[0], max_steps + 1), n),
-                                dtype=torch.complex128, device=dev)
+        if j >= cap:
+            # column j of T and basis vector j+1 do not fit: double every buffer
+            # (the look-ahead application only reads Q[j], which is copied)
+            cap = min(2 * cap, max_steps)
+            grown = torch.empty((cap + 1, n), dtype=torch.complex128, device=dev)
             grown[: Q.shape[0]].copy_(Q)
             Q = grown
+            grown_p = torch.empty(cap + 3, dtype=torch.complex128, device=dev)
+            grown_p[: packed.numel()].copy_(packed)
+            packed = grown_p
+            hess = _grow_hess(hess, cap)
+        host_buf = _pinned(j + 3)
         w = w_next if w_next is not None else apply(j)
         w_next = None
         w = _flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
@@ -370,12 +396,12 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
                                                      _vp(work)), "ptn_krylov_orthonormalize")
         else:
             _orthonormalize_chunked(lib, stream, n, Q, j + 1, w, packed, work)
-        host_buf[: j + 3].copy_(packed[: j + 3], non_blocking=True)
+        host_buf.copy_(packed[: j + 3], non_blocking=True)
         done.record()
         if lookahead and j + 1 < max_steps:
             w_next = apply(j + 1)
         done.synchronize()
-        host = host_buf[: j + 3].numpy()
+        host = host_buf.numpy()
         if j == 0:
             beta0 = math.sqrt(max(host[0].real, 0.0))
             if beta0 == 0.0:
@@ -519,9 +545,13 @@ def krylov_exp_action_batched(psis, apply: Callable, time_difference: float,
     by_n: Dict[int, List[int]] = {}
     for b, n in enumerate(ns):
         by_n.setdefault(n, []).append(b)
-    Q = [torch.empty((steps[b] + 1, ns[b]), dtype=torch.complex128, device=dev)
+    # every buffer starts at KRYLOV_START_CAP columns and doubles as the
+    # recurrence runs longer (see krylov_exp_action)
+    caps = [min(st, KRYLOV_START_CAP) for st in steps]
+    Q = [torch.empty((caps[b] + 1, ns[b]), dtype=torch.complex128, device=dev)
          for b in range(count)]
-    packed = torch.empty((count, mmax + 3), dtype=torch.complex128, device=dev)
+    pcap = min(mmax, KRYLOV_START_CAP)
+    packed = torch.empty((count, pcap + 3), dtype=torch.complex128, device=dev)
     pk = packed.data_ptr()
     prow = packed.stride(0) * 16
     for n, mem in by_n.items():
@@ -532,9 +562,9 @@ def krylov_exp_action_batched(psis, apply: Callable, time_difference: float,
         _lib.check(lib.ptn_normalize_batched(stream, n, len(mem), _cp(xs), _cp(ys), _cp(outs),
                                              ctypes.c_void_p(work.data_ptr())),
                    "ptn_normalize_batched")
-    host_buf = torch.empty((count, mmax + 3), dtype=torch.complex128, pin_memory=True)
+    host_buf = torch.empty((count, pcap + 3), dtype=torch.complex128, pin_memory=True)
     done_ev = torch.cuda.Event()
-    hess = [np.zeros((steps[b] + 1, steps[b] + 1), dtype=complex) for b in range(count)]
+    hess = [np.zeros((caps[b] + 1, caps[b] + 1), dtype=complex) for b in range(count)]
     beta0 = [0.0] * count
     m = [0] * count
     active = list(range(count))
@@ -543,6 +573,22 @@ def krylov_exp_action_batched(psis, apply: Callable, time_difference: float,
         active = [b for b in active if j < steps[b]]
         if not active:
             break
+        if j >= pcap:
+            pcap = min(2 * pcap, mmax)
+            grown = torch.empty((count, pcap + 3), dtype=torch.complex128, device=dev)
+            grown[:, : packed.shape[1]].copy_(packed)
+            packed = grown
+            pk = packed.data_ptr()
+            prow = packed.stride(0) * 16
+            host_buf = torch.empty((count, pcap + 3), dtype=torch.complex128,
+                                   pin_memory=True)
+        for b in active:
+            if j >= caps[b]:
+                caps[b] = min(2 * caps[b], steps[b])
+                grown = torch.empty((caps[b] + 1, ns[b]), dtype=torch.complex128, device=dev)
+                grown[: Q[b].shape[0]].copy_(Q[b])
+                Q[b] = grown
+                hess[b] = _grow_hess(hess[b], caps[b])
         ws = apply(active, [Q[b][j].view(shapes[b]) for b in active])
         ws = [_flat(w) if w.dtype == torch.complex128 else _flat(w.to(torch.complex128))
               for w in ws]
@@ -551,6 +597,12 @@ def krylov_exp_action_batched(psis, apply: Callable, time_difference: float,
         for b in active:
             groups.setdefault(ns[b], []).append(b)
         for n, mem in groups.items():
+            if j + 1 > MAXV_PER_CALL:
+                # bases longer than one call: the single-instance chunked form
+                for b in mem:
+                    _orthonormalize_chunked(lib, stream, n, Q[b], j + 1, wmap[b], packed[b],
+                                            _work(dev))
+                continue
             work = _bwork(dev, len(mem))
             qs = _ptrs([Q[b] for b in mem])
             ld = np.array([Q[b].stride(0) for b in mem], dtype=np.int64)

@@ -710,6 +710,34 @@ class TwoSiteTDVP(TDVPAlgorithm):
         return "TwoSite_" + node_id + "_contr_" + next_node_id
 
 
+class FirstOrderTwoSiteTDVP(TwoSiteTDVP):
+    """First-order two-site TDVP (``tdvp_algorithms/firstordertwosite.py:9-30``): one
+    sweep along the update path at the full time step -- per pair of the path the
+    two-site forward update, then the backward single-site update of the next
+    site, the last pair updated in one go -- after which the orthogonality centre
+    returns to the start of the path (KEEP QR).
+
+    The reference loop names its pairs as (update_path[i],
+    orthogonalization_path[i+1][0]) and moves the centre along
+    orthogonalization_path[i] after the pair's update; on every tree that pairs
+    nodes which are not neighbours, so it cannot run as written.  This class runs
+    the sweep its docstring describes, with the pairs and moves of the
+    second-order two-site forward sweep, and updates the environment cache on
+    the way back so the next step starts from a complete cache."""
+
+    def _time_step_gen(self):
+        up, op = self.update_path, self.orthogonalization_path
+        for i, node_id in enumerate(up[:-2]):
+            if i > 0:
+                self._move_orth_and_update_cache_for_path(op[i - 1])
+            nxt = op[i][0]
+            yield from self._update_two_site_nodes_gen(node_id, nxt)
+            yield from self._single_site_backwards_update_gen(nxt)
+        yield from self._update_two_site_nodes_gen(up[-2], up[-1])
+        self._move_orth_and_update_cache_for_path(
+            self.state.path_from_to(self.state.orthogonality_center_id, up[0]))
+
+
 class SecondOrderTwoSiteTDVP(TwoSiteTDVP):
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
@@ -805,9 +833,12 @@ class BatchedTDVP:
         """Device bytes one instance needs inside a batched Krylov solve: the
         basis, the vectors of one step and the live intermediates of its H_eff
         network (planned order)."""
+        from .krylov import KRYLOV_START_CAP
         net = req.action
         n = req.x.numel()
-        m = n if max_dim is None else min(n, int(max_dim))
+        # the basis starts at KRYLOV_START_CAP vectors and doubles only while the
+        # recurrence runs on; with no max_krylov_dim the budget takes twice that
+        m = min(n, int(max_dim) if max_dim is not None else 2 * KRYLOV_START_CAP)
         dims: Dict[Any, int] = {}
         for t, lab in zip(net.tensors, net.labels):
             t = req.x if t is None else t

@@ -137,20 +137,77 @@ class TreeTensorNetworkOperator(TreeTensorNetwork):
         return hit
 
     @classmethod
-    def from_tensors(cls, structure: TreeTensorNetwork,
-                     tensors: Dict[str, np.ndarray]) -> "TreeTensorNetworkOperator":
-        """TTNO on ``structure``'s tree from canonical-layout tensors."""
+    def from_tensors(cls, structure: TreeTensorNetwork, tensors: Dict[str, np.ndarray],
+                     children: Optional[Dict[str, Sequence[str]]] = None
+                     ) -> "TreeTensorNetworkOperator":
+        """TTNO on ``structure``'s tree (same nodes and parents).
+
+        Each tensor has legs (parent, children..., out, in) with the children in
+        the operator's own order: ``children[nid]`` when given, else the order of
+        ``structure``.  The reference allows a TTNO whose children order differs
+        from the state's and matches legs by neighbour identifier
+        (``contractions/contraction_util.py:40-80``); every contraction here does
+        the same.  Raises ValueError when a node's children are not the
+        structure's, a tensor has the wrong number of legs, a bond has different
+        dimensions at its two ends, or the out/in legs do not match the state's
+        physical dimension."""
         op = cls()
         dev = structure.device
         for nid in _parents_first(structure):
             snode = structure.nodes[nid]
             n = Node(identifier=nid)
-            n.parent, n.children = snode.parent, list(snode.children)
+            kids = list(snode.children) if children is None or nid not in children \
+                else list(children[nid])
+            if sorted(kids) != sorted(snode.children):
+                raise ValueError(f"operator children {kids} of {nid} are not the "
+                                 f"state's children {list(snode.children)}")
+            n.parent, n.children = snode.parent, kids
             t = as_device_tensor(tensors[nid], device=dev)
             n.shape = tuple(t.shape)
             op._nodes[nid] = n
             op._tensors[nid] = t
         op._root_id = structure.root_id
+        op.check_structure(structure)
+        return op
+
+    def check_structure(self, state: Optional[TreeTensorNetwork] = None):
+        """Check every node tensor's leg count and every bond's two dimensions, and,
+        with ``state``, the out/in dimensions against the state's physical legs."""
+        for nid, n in self._nodes.items():
+            shape = tuple(self._tensors[nid].shape)
+            if len(shape) != n.nneighbours() + 2:
+                raise ValueError(f"operator tensor {nid} has {len(shape)} legs; expected "
+                                 f"{n.nneighbours()} virtual legs plus out and in")
+            for c in n.children:
+                dc = self._tensors[c].shape[0]
+                if shape[n.neighbour_index(c)] != dc:
+                    raise ValueError(f"bond {nid}-{c} of the operator has dimension "
+                                     f"{shape[n.neighbour_index(c)]} at {nid} and {dc} at {c}")
+            if state is not None:
+                phys = state.nodes[nid].shape[state.nodes[nid].nneighbours():]
+                if tuple(shape[-2:]) != (int(np.prod(phys)),) * 2:
+                    raise ValueError(f"operator tensor {nid} acts on {shape[-2:]}; the state "
+                                     f"node has physical legs {tuple(phys)}")
+
+    def with_children_order(self, children: Dict[str, Sequence[str]]
+                            ) -> "TreeTensorNetworkOperator":
+        """The same operator with the children of the listed nodes reordered (legs
+        permuted to match); the other nodes keep their order."""
+        op = type(self)()
+        for nid, n in self._nodes.items():
+            m = Node(identifier=nid)
+            kids = list(children.get(nid, n.children))
+            if sorted(kids) != sorted(n.children):
+                raise ValueError(f"{kids} is not a reordering of {nid}'s children")
+            m.parent, m.children = n.parent, kids
+            np_ = n.nparents()
+            perm = list(range(np_)) + [np_ + n.children.index(c) for c in kids] + \
+                [len(n.shape) - 2, len(n.shape) - 1]
+            t = self._tensors[nid].permute(perm).contiguous()
+            m.shape = tuple(t.shape)
+            op._nodes[nid] = m
+            op._tensors[nid] = t
+        op._root_id = self._root_id
         return op
 
     @classmethod

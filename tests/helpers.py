@@ -76,3 +76,25 @@ def oracle_problem(branching, depth, chi, seed):
     ost.canonical_form(topo.root)
     ost.normalise()
     return topo, ost, W, H
+
+
+def reversed_children_operator(topo: Topology, W: dict):
+    """(operator tree, tensors) of the same TTNO with every node's children in reverse
+    order: the operator's own tree then differs from the state's wherever a node
+    has two or more children, and each tensor's child legs are permuted to match."""
+    op = Topology()
+    for nid in topo.node_ids:
+        if topo.parent[nid] is None:
+            op.add_root(nid)
+        else:
+            op.add_child(nid, topo.parent[nid])
+    Wp = {}
+    for nid in topo.node_ids:
+        kids = list(topo.children[nid])
+        op.children[nid] = list(reversed(kids))
+        npar = 0 if topo.parent[nid] is None else 1
+        t = W[nid]
+        perm = list(range(npar)) + [npar + kids.index(c) for c in op.children[nid]] + \
+            [t.ndim - 2, t.ndim - 1]
+        Wp[nid] = np.ascontiguousarray(np.transpose(t, perm))
+    return op, Wp

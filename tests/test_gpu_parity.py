@@ -96,11 +96,11 @@ def _run_pair(kind, branching, depth, chi, seed, model, dt, steps, krylov_tol=1e
     st, H, topo, tensors, W = _setup(branching, depth, chi, seed, model)
     ops = {nid: ptn.TensorProduct({nid: PAULI_Z}) for nid in list(st.nodes)[:4]}
     opts = {"krylov_tol": krylov_tol, "max_krylov_dim": 10}
-    if kind == "second2":
+    if kind in ("second2", "first2"):
         cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
                                     **(svd or {}))
-        alg = ptn.SecondOrderTwoSiteTDVP(st, H, dt, dt * steps, ops, config=cfg,
-                                         solver_options=opts)
+        cls = ptn.SecondOrderTwoSiteTDVP if kind == "second2" else ptn.FirstOrderTwoSiteTDVP
+        alg = cls(st, H, dt, dt * steps, ops, config=cfg, solver_options=opts)
     else:
         cls = ptn.FirstOrderOneSiteTDVP if kind == "first1" else ptn.SecondOrderOneSiteTDVP
         cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision)
@@ -127,10 +127,11 @@ def test_tdvp_one_site_matches_oracle(gpu, kind, model, tol):
             assert abs(z - otd.expect_site(nid, PAULI_Z)) < REL * abs(e_ref), nid
 
 
-def test_tdvp_two_site_truncation_matches_oracle(gpu):
-    _say("tdvp second2 heisenberg truncation")
+@pytest.mark.parametrize("kind", ["second2", "first2"])
+def test_tdvp_two_site_truncation_matches_oracle(gpu, kind):
+    _say(f"tdvp {kind} heisenberg truncation")
     svd = dict(max_bond_dim=6, rel_tol=1e-10, total_tol=1e-10)
-    alg, otd, ops = _run_pair("second2", 2, 3, 4, 17, "heis", 0.05, 2, svd=svd)
+    alg, otd, ops = _run_pair(kind, 2, 3, 4, 17, "heis", 0.05, 2, svd=svd)
     otd_dims = []
     for step in range(2):
         alg.run_one_time_step()

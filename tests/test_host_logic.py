@@ -200,3 +200,85 @@ def test_ttn_load_rejects_missing_and_inconsistent_files(tmp_path):
                                "shape": [2, 2]}}}, open(str(tmp_path / "bad.json"), "w"))
     with pytest.raises(ValueError, match="bond r-c"):
         ptn.TreeTensorNetworkState.load(str(tmp_path / "bad"), device="cpu")
+
+
+with open(os.path.join(GOLD, "effective_hamiltonian.json")) as f:
+    EFF = json.load(f)
+
+
+def _node(nid, nbs, n_open, has_parent):
+    n = ptn.Node(identifier=nid)
+    if has_parent:
+        n.parent, n.children = nbs[0], list(nbs[1:])
+    else:
+        n.parent, n.children = None, list(nbs)
+    n.shape = (2,) * (len(nbs) + n_open)
+    return n
+
+
+@pytest.mark.parametrize("case", EFF["leg_permutations"], ids=lambda c: c["name"])
+def test_product_leg_permutation_kats(case):
+    from pytreenet_amd.effective_hamiltonians import find_tensor_leg_permutation
+    has_parent = not case["name"].startswith("root")
+    s = _node("node", case["state_nbs"], case["n_open"], has_parent)
+    h = _node("node", case["ham_nbs"], case["n_open"], has_parent)
+    assert find_tensor_leg_permutation(s, h) == tuple(case["perm"])
+
+
+def test_ttno_from_tensors_own_children_order_and_checks():
+    """A TTNO may list its children in its own order (legs permuted to match); bond,
+    leg-count and physical-dimension mismatches raise."""
+    import torch
+    st = _cpu_state("r", [("a", "r"), ("b", "r"), ("c", "a")], chi=3)
+    rng = np.random.default_rng(0)
+    W = {"r": rng.standard_normal((4, 5, 2, 2)), "a": rng.standard_normal((4, 6, 2, 2)),
+         "b": rng.standard_normal((5, 2, 2)), "c": rng.standard_normal((6, 2, 2))}
+    op = ptn.TTNO.from_tensors(st, W)
+    assert op.nodes["r"].children == ["a", "b"]
+    Wrev = dict(W, r=np.transpose(W["r"], (1, 0, 2, 3)))
+    op2 = ptn.TTNO.from_tensors(st, Wrev, children={"r": ["b", "a"]})
+    assert op2.nodes["r"].children == ["b", "a"] and op2.nodes["r"].neighbour_index("a") == 1
+    op3 = op.with_children_order({"r": ["b", "a"]})
+    assert torch.equal(op3.tensors["r"], op2.tensors["r"])
+    with pytest.raises(ValueError):          # children order given, legs not permuted
+        ptn.TTNO.from_tensors(st, W, children={"r": ["b", "a"]})
+    with pytest.raises(ValueError):          # not a reordering of the state's children
+        ptn.TTNO.from_tensors(st, W, children={"r": ["a", "c"]})
+    with pytest.raises(ValueError):          # wrong physical dimension
+        ptn.TTNO.from_tensors(st, dict(W, c=rng.standard_normal((6, 3, 3))))
+    with pytest.raises(ValueError):          # missing leg
+        ptn.TTNO.from_tensors(st, dict(W, b=rng.standard_normal((5, 2))))
+
+
+def test_krylov_hessenberg_buffer_grows_by_doubling():
+    """The Krylov buffers start small and double (no n x n allocation when
+    max_krylov_dim is None); the grown buffer keeps the old entries."""
+    from pytreenet_amd.krylov import KRYLOV_START_CAP, _grow_hess
+    h = np.arange(9, dtype=complex).reshape(3, 3)
+    g = _grow_hess(h, 5)
+    assert g.shape == (6, 6) and np.array_equal(g[:3, :3], h) and not g[3:].any()
+    assert KRYLOV_START_CAP <= 32
+
+
+def test_interchange_files_between_product_and_oracle(tmp_path):
+    """The oracle's reader/writer of the reference's two files agrees with the
+    product's save/load, children order included."""
+    import torch
+    from oracle.io import load_ttn, save_ttn
+    from pytreenet_amd.random_states import build_state, random_state_tensors
+    from tests.helpers import reversed_children_operator, topology_from_spec
+    ids, parent, children = complete_tree(2, 2)
+    tensors = random_state_tensors(ids, parent, children, 3, 5)
+    st = build_state(ids, parent, children, tensors, device="cpu")
+    st.save(str(tmp_path / "s"))
+    topo, A, center = load_ttn(str(tmp_path / "s"))
+    assert center is None and topo.root == "site0"
+    assert all(np.array_equal(A[n], tensors[n]) for n in ids)
+    assert all(topo.children[n] == children[n] for n in ids)
+    W = {n: np.asarray(t) for n, t in ptn.transverse_field_ising(st).tensors.items()}
+    op_topo, Wp = reversed_children_operator(topology_from_spec(ids, parent, children), W)
+    save_ttn(str(tmp_path / "w"), op_topo, Wp, center="site1")
+    H = ptn.TTNO.load(str(tmp_path / "w"), device="cpu")
+    assert H.nodes["site0"].children == ["site2", "site1"]
+    assert H.orthogonality_center_id == "site1"
+    assert all(torch.equal(H.tensors[n], torch.as_tensor(Wp[n])) for n in ids)

@@ -178,7 +178,7 @@ def test_oracle_heff_is_projected_hamiltonian():
     np.testing.assert_allclose(y, heff @ x.reshape(-1), atol=1e-11)
 
 
-@pytest.mark.parametrize("kind", ["first1", "second1", "second2"])
+@pytest.mark.parametrize("kind", ["first1", "second1", "second2", "first2"])
 def test_oracle_tdvp_exact_at_full_bond_dims(kind):
     """With uncapped bond dims (no truncation), TDVP equals exact evolution."""
     ids, parent, children, tensors = small_tree(2, 2, 64, seed=5)   # 7 sites, caps not hit
