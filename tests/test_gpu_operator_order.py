@@ -212,3 +212,19 @@ def test_public_entry_points_follow_the_current_stream(gpu):
         e2 = ptn.expectation_value(st, Hp)
     torch.cuda.synchronize()
     assert _rel(e2, e_ref) < REL
+
+
+def test_multi_single_site_expectation_values_match_oracle(gpu):
+    _say("<Z_i>, <X_i> for every site in two passes, binary depth 3 chi 8")
+    from pytreenet_amd.contractions import multi_single_site_expectation_value
+    from pytreenet_amd.ttno import PAULI_X
+    st, H, Hp, topo, op_topo, tensors, Wp = _setup(2, 3, 8, 37)
+    ops = {nid: [PAULI_Z, ptn.TensorProduct({nid: PAULI_X})] for nid in st.nodes}
+    got = multi_single_site_expectation_value(st, ops)
+    n2 = abs(OC.overlap(topo, tensors, tensors))
+    for nid in st.nodes:
+        assert abs(got[nid][0] - OC.single_site_expectation(topo, tensors, nid, PAULI_Z)) \
+            < REL * n2
+        assert abs(got[nid][1] - OC.single_site_expectation(topo, tensors, nid, PAULI_X)) \
+            < REL * n2
+    assert st.orthogonality_center_id is None
