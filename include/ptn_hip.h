@@ -210,6 +210,18 @@ int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, int64_t ldq
 int ptn_krylov_orthonormalize(void *stream, int64_t n, const void *Q, int64_t ldq,
                               int32_t nvec, void *w, void *qnext, void *out, void *work);
 
+/* ptn_krylov_orthonormalize with its work buffer taken from a device array
+ * inside the library instead of an argument, so the call allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The array holds the partials of at most 128 basis
+ * vectors (ptn_krylov_work_bytes(n, 128) = 2.1 MB) on every GPU the library
+ * runs on, until the process ends; nvec > 128 returns PTN_ERANGE.  Every call
+ * on one GPU shares that GPU's array and leaves its contents behind: calls
+ * that overlap on one GPU (other streams or threads) race.                 */
+int ptn_krylov_orthonormalize_noalloc(void *stream, int64_t n, const void *Q, int64_t ldq,
+                                      int32_t nvec, void *w, void *qnext, void *out);
+
 /* y = x / ||x|| (y may alias x; zero when ||x|| = 0) and out[0] = (||x||^2, 0)
  * (complex128, device): the start vector q_0 = psi / ||psi|| of
  * krylov_exp_action (time_evolution/time_evolution.py:775-781).             */

@@ -15,10 +15,12 @@ LIB_PATH = _PKG / "libptnhip.so"
 PTN_C128, PTN_C64 = 0, 1
 PTN_COMPUTE_F64_4M, PTN_COMPUTE_F32_3M, PTN_COMPUTE_BF16_3M = 0, 1, 2
 PTN_GEMM_RULE, PTN_GEMM_WAVE, PTN_GEMM_TILE = 0, 1, 2
+PTN_OK, PTN_EINVAL, PTN_EHIP, PTN_ESOLVER, PTN_ENOCONV, PTN_ERANGE = 0, -1, -2, -3, -4, -5
 
 EXPORTED = (
     "ptn_gemm_gather", "ptn_gemm_gather_config", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
-    "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize", "ptn_normalize",
+    "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize",
+    "ptn_krylov_orthonormalize_noalloc", "ptn_normalize",
     "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc", "ptn_krylov_small_work_bytes",
     "ptn_krylov_expm_small",
     "ptn_gemm_strided", "ptn_gemm_strided_ok", "ptn_qr", "ptn_svd",
@@ -90,6 +92,7 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_krylov_work_bytes": (i64, [i64, i32]),
         "ptn_krylov_orthogonalize": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp]),
         "ptn_krylov_orthonormalize": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp, vp]),
+        "ptn_krylov_orthonormalize_noalloc": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp]),
         "ptn_normalize": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
         "ptn_krylov_combine": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp]),
         "ptn_norm_sq": (ctypes.c_int, [vp, i64, vp, vp, vp]),

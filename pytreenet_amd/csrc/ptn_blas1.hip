@@ -657,6 +657,61 @@ extern "C" int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, 
   return ptn_krylov_orthonormalize(stream, n, Q, ldq, nvec, w, nullptr, out, work);
 }
 
+// ---- allocation-free orthonormalisation step -----------------------------------
+// The work buffer of ptn_krylov_orthonormalize (per-block partial dot products and
+// the reduced coefficients) as a file-scope device array: every GPU the library
+// runs on holds its own copy (2.1 MB) until the process ends.  Host code never
+// passes the array's name to a kernel: the name is a host-side stand-in, and each
+// device's address comes from hipGetSymbolAddress, taken once per device with
+// that device current and kept in noalloc_ptr.
+namespace noalloc {
+constexpr int64_t SCRATCH_ELEMS = (int64_t)MAXBLK * (MAXV + 2) + 2 * MAXV + 8;
+__device__ __align__(256) double2 scratch[SCRATCH_ELEMS];
+constexpr int MAX_DEVICES = 64;
+std::mutex mu;
+void *noalloc_ptr[MAX_DEVICES] = {};
+
+int scratch_of_current_device(void **out) {
+  int dev = -1;
+  PTN_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    ptn::set_error("ptn_krylov_orthonormalize_noalloc: device %d beyond %d", dev, MAX_DEVICES);
+    return PTN_ERANGE;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!noalloc_ptr[dev]) {
+    void *p = nullptr;
+    PTN_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return PTN_OK;
+}
+}  // namespace noalloc
+
+extern "C" int ptn_krylov_orthonormalize_noalloc(void *stream, int64_t n, const void *Q,
+                                                 int64_t ldq, int32_t nvec, void *w,
+                                                 void *qnext, void *out) {
+  if (nvec > MAXV) {
+    ptn::set_error("ptn_krylov_orthonormalize_noalloc: nvec=%d beyond the maximum %d "
+                   "(the library's scratch holds the partials of %d basis vectors)",
+                   nvec, MAXV, MAXV);
+    return PTN_ERANGE;
+  }
+  if (ptn_krylov_work_bytes(n, nvec) > (int64_t)sizeof(noalloc::scratch)) {
+    ptn::set_error("ptn_krylov_orthonormalize_noalloc: needs %lld scratch bytes, the "
+                   "library holds %lld", (long long)ptn_krylov_work_bytes(n, nvec),
+                   (long long)sizeof(noalloc::scratch));
+    return PTN_ERANGE;
+  }
+  void *ptr = nullptr;
+  const int rc = noalloc::scratch_of_current_device(&ptr);
+  if (rc) return rc;
+  // every partial and coefficient the kernels read is written earlier in the same
+  // call (no accumulation across calls), so the scratch needs no clearing
+  return ptn_krylov_orthonormalize(stream, n, Q, ldq, nvec, w, qnext, out, ptr);
+}
+
 extern "C" int ptn_normalize(void *stream, int64_t n, const void *xv, void *yv, void *outv,
                              void *work) {
   if (n <= 0 || !xv || !yv || !outv || !work) {
