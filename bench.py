@@ -26,7 +26,6 @@ import json
 import math
 import os
 import platform
-import subprocess
 import sys
 import time
 
@@ -76,6 +75,8 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
     import pytreenet_amd as ptn
     from pytreenet_amd import engine as E
     from pytreenet_amd import krylov as K
+    from pytreenet_amd.contractions import multi_single_site_expectation_value
+    from pytreenet_amd.ttno import PAULI_Z
     cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
                          relax_tol=relax_tol, relax_bf16=relax_bf16)
     tag = precision + ("-bf16" if relax_bf16 else "")
@@ -109,7 +110,9 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
     low, apps = K.STATS.low_applies - low0, K.STATS.applies - app0
     E.PROFILER.phase = f"{tag}-eval"
     energy = ptn.expectation_value(alg.state, H).real
+    z = multi_single_site_expectation_value(alg.state, {n: PAULI_Z for n in alg.state.nodes})
     return {"elapsed": elapsed, "energy": energy, "alg": alg,
+            "z": {n: v[0].real for n, v in z.items()},
             "applies": alg.apply_log[nlog:],
             "gemm_flops": fl1 - fl0, "launches": n1 - n0,
             "low_fraction": low / max(apps, 1),
@@ -204,13 +207,15 @@ def calibrate_copy():
 
 
 def host_facts():
+    # read from /proc (no child process: the bench leaves nothing running behind it)
     model = ""
     try:
-        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        for line in out.splitlines():
-            if line.startswith("Model name"):
-                model = line.split(":", 1)[1].strip()
-    except Exception:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
         pass
     blas = ""
     try:
@@ -259,25 +264,31 @@ def main():
            "fp64_gemm_tflops": r64["gemm_flops"] / r64["elapsed"] / 1e12,
            "fp64_launches_per_step": r64["launches"] / args.steps,
            "fp64_applies_per_step": sum(m for _, _, m in r64["applies"]) / args.steps}
-    # (path, run, sweeps/s) of every path that meets |dE| <= 1e-8 against FP64
+    # (path, run, sweeps/s) of every path within the accuracy bar against FP64 after the
+    # same steps from the same state: |dE| <= 1e-8, max_i |d<Z_i>| <= 1e-8 over all
+    # sites, and infidelity 1 - |<psi_fp64|psi>| <= 1e-8 (both states normalised)
     ok = [("fp64", r64, sps64)]
+    from pytreenet_amd.contractions import contract_two_ttns
     if not args.no_mixed:
         for prec, key, bf16 in (("relaxed", "relaxed", False),
                                 ("relaxed", "relaxed_bf16", True), ("fp32", "mixed", False)):
             r = run_tdvp(st, H, prec, args.warmup, args.steps, profile=True, relax_bf16=bf16)
             sps = args.steps / r["elapsed"]
             de = abs(r["energy"] - r64["energy"])
+            dz = max(abs(r["z"][n] - r64["z"][n]) for n in r64["z"])
+            infid = 1.0 - abs(contract_two_ttns(r["alg"].state, r64["alg"].state,
+                                                conj_second=True))
             res.update({f"{key}_sweeps_per_s": sps, f"{key}_energy": r["energy"],
-                        f"{key}_abs_energy_error": de, f"{key}_speedup": sps / sps64,
+                        f"{key}_abs_energy_error": de, f"{key}_max_abs_z_error": dz,
+                        f"{key}_infidelity": infid, f"{key}_speedup": sps / sps64,
                         f"{key}_gemm_tflops": r["gemm_flops"] / r["elapsed"] / 1e12,
                         f"{key}_low_precision_applies": r["low_fraction"]
                         if prec == "relaxed" else 1.0})
             if bf16:
                 res[f"{key}_bf16_applies"] = r["lower_fraction"]
-            # the bf16 level meets the energy bar but lets <Z_i> drift to 1.6e-8
-            # over the configuration's 20 steps: reported, not taken as `value`
-            if de <= 1e-8 and not bf16:
+            if de <= 1e-8 and dz <= 1e-8 and infid <= 1e-8:
                 ok.append((key if prec == "relaxed" else prec, r, sps))
+            del r["alg"]
     path, run, value = max(ok, key=lambda t: t[2])
     # the roofline follows the path whose rate is `value`
     kern = run["kernels"]

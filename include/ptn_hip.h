@@ -300,6 +300,35 @@ int ptn_qr_split_small(void *stream, int32_t rank, const int64_t *dims, const in
 int ptn_svd(void *stream, int64_t m, int64_t n, void *A, void *S, void *U,
             void *Vh, void *E);
 
+/* Thin SVD of a small m x n complex128 matrix by one-sided Jacobi (Hestenes) in
+ * one workgroup: A (column-major, leading dimension lda; not modified) =
+ * U diag(S) Vh with k = min(m, n), S (double, k, descending), U (m x k,
+ * column-major, ld m), Vh (k x n, column-major, ld k).  on_rows = 0 rotates the
+ * columns of A (needs m >= n), on_rows = 1 the columns of A^H (needs m <= n; for
+ * the triangle R of a QR factorisation this is the preconditioned form, which
+ * converges in about ten sweeps also for graded spectra).  Limits: short side
+ * <= 256, long side <= 4096 (PTN_ERANGE beyond).  work: ptn_svd_jacobi_work_bytes
+ * (m, n) bytes of device memory; info (device int32): 0 converged, 1 when 64
+ * sweeps did not converge.  Small singular values come out with high relative
+ * accuracy.  Replaces np.linalg.svd / scipy gesvd in tensor_svd
+ * (util/tensor_splitting.py:122-177) for the panels of truncated splits.      */
+int64_t ptn_svd_jacobi_work_bytes(int64_t m, int64_t n);
+int ptn_svd_jacobi(void *stream, int64_t m, int64_t n, const void *A, int64_t lda,
+                   int32_t on_rows, void *S, void *U, void *Vh, void *work, void *info);
+
+/* ptn_svd_jacobi over `count` independent matrices in one launch (one workgroup
+ * each; shapes may differ): per-matrix arguments are host arrays, ws[b] holds
+ * ptn_svd_jacobi_work_bytes(m[b], n[b]) bytes.  The library stages the job
+ * descriptors into `work` (ptn_svd_jacobi_batched_work_bytes(count) bytes).  The
+ * batched two-site TDVP replaces the per-instance SVDs of a split point
+ * (util/tensor_splitting.py:122-177, run once per instance by the reference)
+ * with this one call.                                                         */
+int64_t ptn_svd_jacobi_batched_work_bytes(int32_t count);
+int ptn_svd_jacobi_batched(void *stream, int32_t count, const int64_t *m, const int64_t *n,
+                           const void *const *A, const int64_t *lda, const int32_t *on_rows,
+                           void *const *S, void *const *U, void *const *Vh,
+                           void *const *ws, void *const *info, void *work);
+
 /* Block-sparse operator application (the step of an H_eff / environment
  * contraction where the TTNO node tensor meets a large tensor):
  *   out[o_r[r] + o_g[g] + o_j[j]] =

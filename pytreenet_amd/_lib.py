@@ -28,6 +28,8 @@ EXPORTED = (
     "ptn_krylov_orthonormalize_batched", "ptn_normalize_batched", "ptn_krylov_combine_batched",
     "ptn_ptr_batch_work_bytes", "ptn_fill_batched", "ptn_permute_batched", "ptn_potrf_lower", "ptn_trtri_lower", "ptn_block_apply",
     "ptn_qr_split_small", "ptn_block_apply_staged",
+    "ptn_svd_jacobi_work_bytes", "ptn_svd_jacobi", "ptn_svd_jacobi_batched_work_bytes",
+    "ptn_svd_jacobi_batched",
     "ptn_last_error", "ptn_version",
 )
 
@@ -93,6 +95,11 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
         "ptn_krylov_orthogonalize": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp]),
         "ptn_krylov_orthonormalize": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp, vp]),
         "ptn_krylov_orthonormalize_noalloc": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp, vp]),
+        "ptn_svd_jacobi_work_bytes": (ctypes.c_int64, [i64, i64]),
+        "ptn_svd_jacobi": (ctypes.c_int, [vp, i64, i64, vp, i64, i32, vp, vp, vp, vp, vp]),
+        "ptn_svd_jacobi_batched_work_bytes": (ctypes.c_int64, [i32]),
+        "ptn_svd_jacobi_batched": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                                  vp, vp]),
         "ptn_normalize": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
         "ptn_krylov_combine": (ctypes.c_int, [vp, i64, vp, i64, i32, vp, vp]),
         "ptn_norm_sq": (ctypes.c_int, [vp, i64, vp, vp, vp]),

@@ -1,0 +1,335 @@
+// One-sided Jacobi SVD of small complex matrices, one workgroup per matrix.
+//
+// Hestenes' method: the columns of G (m_g x n_g, column-major, m_g >= n_g) are
+// rotated pairwise by 2 x 2 unitary Jacobi rotations until every pair is
+// orthogonal to working precision; then sigma_j = |g_j|, U = G diag(1/sigma)
+// and V is the product of the rotations.  The rotation of a pair (a, b) with
+// alpha = |a|^2, beta = |b|^2, gamma = a^H b = |gamma| e^{i phi} diagonalises
+// the pair's Gram matrix:
+//   zeta = (beta - alpha) / (2 |gamma|),  t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
+//   c = 1 / sqrt(1 + t^2),  s = c t,
+//   a' = c a - s e^{-i phi} b,   b' = s e^{i phi} a + c b   (same on the columns of V).
+// A pair is rotated when |gamma| > sqrt(m_g) eps sqrt(alpha beta); a sweep that
+// rotates nothing ends the iteration.  The method computes small singular values
+// to high relative accuracy, so the truncation rules see them as LAPACK's
+// bidiagonal SVD does (to O(eps |A|)).  Applied to R^H of a QR factor (on_rows
+// of a triangle) it converges in ~10 sweeps also for graded spectra.
+//
+// Parallel order: the round-robin tournament (circle method) on the n_g columns
+// padded to even N: N - 1 rounds of N/2 disjoint pairs per sweep, every pair once.
+// A group of TPP lanes (one wave or a power-of-two part of it) owns one pair:
+// the lanes split the rows, reduce alpha, beta, gamma by shuffles within the
+// group, and update both columns of G and V.  Pairs of a round touch disjoint
+// columns, so one workgroup barrier per round orders the rounds.
+//
+// Replaces np.linalg.svd / scipy gesvd in tensor_svd (util/tensor_splitting.py:122-177)
+// for the panels of the truncated two-site splits.
+#include "ptn_common.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+namespace {
+
+constexpr int SVD_NT = 1024;        // threads per workgroup
+constexpr int SVD_MAXN = 256;       // largest n_g (columns rotated)
+constexpr int SVD_MAXM = 4096;      // largest m_g (rows per column)
+constexpr int SVD_MAX_SWEEPS = 64;
+
+struct SvdJob {
+  int64_t m, n;          // A is m x n, column-major with leading dimension lda
+  const double2 *A;
+  int64_t lda;
+  int32_t on_rows;       // 0: rotate the columns of A (m >= n); 1: of A^H (m <= n)
+  int32_t _pad;
+  double *S;             // k = min(m, n) singular values, descending
+  double2 *U;            // m x k, column-major (ld m)
+  double2 *Vh;           // k x n, column-major (ld k)
+  double2 *ws;           // m n + k k complex128 workspace
+  int32_t *info;         // 0: converged; 1: not within SVD_MAX_SWEEPS sweeps
+  int32_t _pad2[2];
+};
+
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// pair i of round r in the circle method on N players (N even): player 0 stays,
+// players 1..N-1 rotate
+__device__ __forceinline__ void rr_pair(int N, int r, int i, int &p, int &q) {
+  const int L = N - 1;
+  if (i == 0) {
+    p = 0;
+    q = 1 + (r % L);
+  } else {
+    p = 1 + ((r + i) % L);
+    q = 1 + ((r - i + L) % L);
+  }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+template <int TPP>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = TPP / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int TPP>
+__device__ void svd_jacobi_body(const SvdJob &job) {
+  __shared__ int rotated;
+  __shared__ double sig[SVD_MAXN];
+  __shared__ int rank_of[SVD_MAXN];
+  const bool rows = job.on_rows != 0;
+  const int64_t mg = rows ? job.n : job.m;  // rows of G
+  const int ng = (int)(rows ? job.m : job.n);  // columns of G
+  double2 *G = job.ws;                     // mg x ng, ld mg
+  double2 *V = job.ws + mg * ng;           // ng x ng, ld ng
+  const int tid = threadIdx.x;
+  // G = op(A), V = I
+  for (int64_t e = tid; e < mg * ng; e += SVD_NT) {
+    const int64_t i = e % mg, j = e / mg;
+    if (!rows) {
+      G[e] = job.A[i + j * job.lda];
+    } else {
+      const double2 a = job.A[j + i * job.lda];  // G(i, j) = conj(A(j, i))
+      G[e] = make_double2(a.x, -a.y);
+    }
+  }
+  for (int e = tid; e < ng * ng; e += SVD_NT)
+    V[e] = make_double2((e % ng) == (e / ng) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+
+  const int N = ng + (ng & 1);
+  const int npairs = N / 2;
+  const int grp = tid / TPP, gl = tid % TPP;
+  const double tol = sqrt((double)mg) * 2.220446049250313e-16;
+  int sweep = 0;
+  bool converged = ng < 2;
+  for (; sweep < SVD_MAX_SWEEPS && !converged; ++sweep) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < N - 1; ++r) {
+      if (grp < npairs) {
+        int p, q;
+        rr_pair(N, r, grp, p, q);
+        if (q < ng) {
+          double2 *a = G + (int64_t)p * mg, *b = G + (int64_t)q * mg;
+          double al = 0, be = 0, gr = 0, gi = 0;
+          for (int64_t i = gl; i < mg; i += TPP) {
+            const double2 x = a[i], y = b[i];
+            al += x.x * x.x + x.y * x.y;
+            be += y.x * y.x + y.y * y.y;
+            const double2 g = cmulc(x, y);
+            gr += g.x;
+            gi += g.y;
+          }
+          al = group_sum<TPP>(al);
+          be = group_sum<TPP>(be);
+          gr = group_sum<TPP>(gr);
+          gi = group_sum<TPP>(gi);
+          const double ag = hypot(gr, gi);
+          if (ag > tol * sqrt(al) * sqrt(be) && ag > 0.0) {
+            const double zeta = (be - al) / (2.0 * ag);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
+            const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+            const double er = gr / ag, ei = gi / ag;  // e^{i phi}
+            // a' = c a - s conj(e) b,  b' = s e a + c b
+            for (int64_t i = gl; i < mg; i += TPP) {
+              const double2 x = a[i], y = b[i];
+              a[i] = make_double2(c * x.x - s * (er * y.x + ei * y.y),
+                                  c * x.y - s * (er * y.y - ei * y.x));
+              b[i] = make_double2(s * (er * x.x - ei * x.y) + c * y.x,
+                                  s * (er * x.y + ei * x.x) + c * y.y);
+            }
+            double2 *va = V + (int64_t)p * ng, *vb = V + (int64_t)q * ng;
+            for (int i = gl; i < ng; i += TPP) {
+              const double2 x = va[i], y = vb[i];
+              va[i] = make_double2(c * x.x - s * (er * y.x + ei * y.y),
+                                   c * x.y - s * (er * y.y - ei * y.x));
+              vb[i] = make_double2(s * (er * x.x - ei * x.y) + c * y.x,
+                                   s * (er * x.y + ei * x.x) + c * y.y);
+            }
+            if (gl == 0) rotated = 1;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    converged = rotated == 0;
+    __syncthreads();  // every thread has read `rotated` before the next reset
+  }
+  // singular values: group g reduces the norms of columns 2g and 2g + 1
+  for (int cbase = 2 * grp; cbase < ng; cbase += 2 * (SVD_NT / TPP)) {
+    for (int h = 0; h < 2; ++h) {
+      const int j = cbase + h;
+      if (j >= ng) break;
+      const double2 *g = G + (int64_t)j * mg;
+      double acc = 0;
+      for (int64_t i = gl; i < mg; i += TPP) acc += g[i].x * g[i].x + g[i].y * g[i].y;
+      acc = group_sum<TPP>(acc);
+      if (gl == 0) sig[j] = sqrt(acc);
+    }
+  }
+  __syncthreads();
+  // descending order, ties in column order
+  for (int j = tid; j < ng; j += SVD_NT) {
+    const double sj = sig[j];
+    int rk = 0;
+    for (int k = 0; k < ng; ++k) rk += (sig[k] > sj || (sig[k] == sj && k < j)) ? 1 : 0;
+    rank_of[j] = rk;
+    job.S[rk] = sj;
+  }
+  __syncthreads();
+  // A = U S Vh.  Columns: G = Ug S Vg^H, U = Ug, Vh = Vg^H.  Rows: A^H = G, so
+  // A = Vg S Ug^H: U = Vg, Vh = Ug^H.  k = n_g in both cases.
+  const int k = ng;
+  for (int64_t e = tid; e < mg * ng; e += SVD_NT) {
+    const int64_t i = e % mg;
+    const int j = (int)(e / mg);
+    const double inv = sig[j] > 0.0 ? 1.0 / sig[j] : 0.0;
+    const double2 u = make_double2(G[e].x * inv, G[e].y * inv);
+    const int rk = rank_of[j];
+    if (!rows) job.U[i + (int64_t)rk * job.m] = u;            // U (m x k)
+    else job.Vh[rk + i * (int64_t)k] = make_double2(u.x, -u.y);  // Vh (k x n), n = mg
+  }
+  for (int e = tid; e < ng * ng; e += SVD_NT) {
+    const int i = e % ng, j = e / ng;
+    const double2 v = V[e];
+    const int rk = rank_of[j];
+    if (!rows) job.Vh[rk + (int64_t)i * k] = make_double2(v.x, -v.y);  // Vh (k x n), n = ng
+    else job.U[i + (int64_t)rk * job.m] = v;                            // U (m x k), m = ng
+  }
+  if (tid == 0) *job.info = converged ? 0 : 1;
+}
+
+template <int TPP>
+__global__ __launch_bounds__(SVD_NT) void svd_jacobi_one(SvdJob job) {
+  svd_jacobi_body<TPP>(job);
+}
+
+template <int TPP>
+__global__ __launch_bounds__(SVD_NT) void svd_jacobi_many(const SvdJob *__restrict__ jobs) {
+  const SvdJob job = jobs[blockIdx.x];
+  svd_jacobi_body<TPP>(job);
+}
+
+int tpp_for(int ng) {
+  const int npairs = (ng + 1) / 2;
+  if (npairs <= 16) return 64;
+  if (npairs <= 32) return 32;
+  if (npairs <= 64) return 16;
+  return 8;
+}
+
+int check_job(const SvdJob &j) {
+  const int64_t mg = j.on_rows ? j.n : j.m, ng = j.on_rows ? j.m : j.n;
+  if (j.m <= 0 || j.n <= 0 || !j.A || !j.S || !j.U || !j.Vh || !j.ws || !j.info ||
+      j.lda < j.m) {
+    ptn::set_error("ptn_svd_jacobi: bad arguments (m=%lld n=%lld lda=%lld)", (long long)j.m,
+                   (long long)j.n, (long long)j.lda);
+    return PTN_EINVAL;
+  }
+  if (mg < ng) {
+    ptn::set_error("ptn_svd_jacobi: on_rows=%d needs %s (m=%lld n=%lld)", j.on_rows,
+                   j.on_rows ? "m <= n" : "m >= n", (long long)j.m, (long long)j.n);
+    return PTN_EINVAL;
+  }
+  if (ng > SVD_MAXN || mg > SVD_MAXM) {
+    ptn::set_error("ptn_svd_jacobi: %lld x %lld beyond the one-workgroup limits "
+                   "(short side <= %d, long side <= %d)", (long long)j.m, (long long)j.n,
+                   SVD_MAXN, SVD_MAXM);
+    return PTN_ERANGE;
+  }
+  return PTN_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t ptn_svd_jacobi_work_bytes(int64_t m, int64_t n) {
+  const int64_t k = m < n ? m : n;
+  return (int64_t)sizeof(double2) * (m * n + k * k);
+}
+
+extern "C" int ptn_svd_jacobi(void *stream, int64_t m, int64_t n, const void *A, int64_t lda,
+                              int32_t on_rows, void *S, void *U, void *Vh, void *work,
+                              void *info) {
+  SvdJob j;
+  memset(&j, 0, sizeof(j));
+  j.m = m;
+  j.n = n;
+  j.A = static_cast<const double2 *>(A);
+  j.lda = lda;
+  j.on_rows = on_rows;
+  j.S = static_cast<double *>(S);
+  j.U = static_cast<double2 *>(U);
+  j.Vh = static_cast<double2 *>(Vh);
+  j.ws = static_cast<double2 *>(work);
+  j.info = static_cast<int32_t *>(info);
+  const int rc = check_job(j);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (tpp_for((int)(on_rows ? m : n))) {
+    case 64: hipLaunchKernelGGL(svd_jacobi_one<64>, dim3(1), dim3(SVD_NT), 0, s, j); break;
+    case 32: hipLaunchKernelGGL(svd_jacobi_one<32>, dim3(1), dim3(SVD_NT), 0, s, j); break;
+    case 16: hipLaunchKernelGGL(svd_jacobi_one<16>, dim3(1), dim3(SVD_NT), 0, s, j); break;
+    default: hipLaunchKernelGGL(svd_jacobi_one<8>, dim3(1), dim3(SVD_NT), 0, s, j); break;
+  }
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+extern "C" int64_t ptn_svd_jacobi_batched_work_bytes(int32_t count) {
+  return (int64_t)sizeof(SvdJob) * (count > 0 ? count : 0);
+}
+
+extern "C" int ptn_svd_jacobi_batched(void *stream, int32_t count, const int64_t *m,
+                                      const int64_t *n, const void *const *A, const int64_t *lda,
+                                      const int32_t *on_rows, void *const *S, void *const *U,
+                                      void *const *Vh, void *const *ws, void *const *info,
+                                      void *work) {
+  if (count <= 0 || !work) {
+    ptn::set_error("ptn_svd_jacobi_batched: bad arguments (count=%d)", count);
+    return PTN_EINVAL;
+  }
+  std::vector<SvdJob> jobs((size_t)count);
+  int tpp = 0;
+  for (int b = 0; b < count; ++b) {
+    SvdJob &j = jobs[(size_t)b];
+    memset(&j, 0, sizeof(j));
+    j.m = m[b];
+    j.n = n[b];
+    j.A = static_cast<const double2 *>(A[b]);
+    j.lda = lda[b];
+    j.on_rows = on_rows[b];
+    j.S = static_cast<double *>(S[b]);
+    j.U = static_cast<double2 *>(U[b]);
+    j.Vh = static_cast<double2 *>(Vh[b]);
+    j.ws = static_cast<double2 *>(ws[b]);
+    j.info = static_cast<int32_t *>(info[b]);
+    const int rc = check_job(j);
+    if (rc) return rc;
+    // one group size per launch: the smallest over the jobs (fewer lanes per pair
+    // never drops a pair: 1024 / tpp groups cover the largest pair count)
+    const int t = tpp_for((int)(j.on_rows ? j.m : j.n));
+    tpp = (tpp == 0 || t < tpp) ? t : tpp;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int rc = ptn::stage_to_device(s, jobs.data(), sizeof(SvdJob) * (size_t)count, work);
+  if (rc) return rc;
+  const SvdJob *dj = static_cast<const SvdJob *>(work);
+  switch (tpp) {
+    case 64: hipLaunchKernelGGL(svd_jacobi_many<64>, dim3(count), dim3(SVD_NT), 0, s, dj); break;
+    case 32: hipLaunchKernelGGL(svd_jacobi_many<32>, dim3(count), dim3(SVD_NT), 0, s, dj); break;
+    case 16: hipLaunchKernelGGL(svd_jacobi_many<16>, dim3(count), dim3(SVD_NT), 0, s, dj); break;
+    default: hipLaunchKernelGGL(svd_jacobi_many<8>, dim3(count), dim3(SVD_NT), 0, s, dj); break;
+  }
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}

@@ -23,7 +23,7 @@ import warnings
 from dataclasses import dataclass
 from enum import Enum
 from numbers import Integral
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -334,7 +334,8 @@ def absorb_into_neighbour(ttn, node_id: str, from_id: str, r: torch.Tensor):
     ttn.tensors[node_id] = E.contract(r, ["new", "old"], t, lt, lo)
 
 
-SVD_STATS = {"direct": 0, "qr_tall": 0, "qr_wide": 0, "qr_fallback": 0}
+SVD_STATS = {"direct": 0, "qr_tall": 0, "qr_wide": 0, "qr_fallback": 0, "jacobi": 0,
+             "jacobi_fallback": 0}
 # aspect ratio from which a panel is reduced by QR before the SVD
 SVD_QR_RATIO = 4
 
@@ -369,7 +370,8 @@ def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
         if not ok:
             return None
         B = E.permute_copy(R, [1, 0])      # B col-major n x n = R
-        s, Ub, Vhb, k = _svd_direct(B.reshape(-1), n, n)
+        # Jacobi on the rows of the triangle (columns of R^H): QR-preconditioned
+        s, Ub, Vhb, k = _svd_direct(B.reshape(-1), n, n, on_rows=1)
         U = torch.empty(m * k, dtype=torch.complex128, device=dev)
         E.contract(q, ["r", "i"], torch.as_strided(Ub, (n, k), (1, n)), ["r", "c"], ["i", "c"],
                    out=torch.as_strided(U, (m, k), (1, m)))
@@ -380,7 +382,8 @@ def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
     if not ok:
         return None
     B = R.contiguous().clone()             # col-major m x m of R^T: B(i, r) = R[r][i]
-    s, Ub, Whb, k = _svd_direct(B.reshape(-1), m, m)
+    # Jacobi on the rows of R^T (columns of conj(R)): QR-preconditioned
+    s, Ub, Whb, k = _svd_direct(B.reshape(-1), m, m, on_rows=1)
     Vh = torch.empty(k * n, dtype=torch.complex128, device=dev)
     E.contract(torch.as_strided(Whb, (k, m), (1, k)), ["c", "r"], q, ["j", "r"], ["c", "j"],
                out=torch.as_strided(Vh, (k, n), (1, k)))
@@ -388,8 +391,47 @@ def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
     return s, Ub, Vh, k
 
 
-def _svd_direct(panel: torch.Tensor, m: int, n: int):
+# panels whose short side is at most SVD_JACOBI_MAX_SHORT (and long side at most
+# SVD_JACOBI_MAX_LONG) go to the one-workgroup Jacobi SVD (ptn_svd_jacobi); larger
+# ones, and the rare panel on which Jacobi does not converge in 64 sweeps, to
+# rocSOLVER's bidiagonal SVD (ptn_svd)
+SVD_JACOBI_MAX_SHORT = 256
+SVD_JACOBI_MAX_LONG = 4096
+
+
+def _svd_jacobi(panel: torch.Tensor, m: int, n: int, on_rows: int):
     k = min(m, n)
+    dev = panel.device
+    lib = _lib.load()
+    S = torch.empty(k, dtype=torch.float64, device=dev)
+    U = torch.empty(m * k, dtype=torch.complex128, device=dev)
+    Vh = torch.empty(k * n, dtype=torch.complex128, device=dev)
+    ws = torch.empty(int(lib.ptn_svd_jacobi_work_bytes(m, n)) // 16, dtype=torch.complex128,
+                     device=dev)
+    info = torch.zeros(2, dtype=torch.float64, device=dev)   # int32 info in the first word
+    _lib.check(lib.ptn_svd_jacobi(ctypes.c_void_p(E.stream_handle()), m, n,
+                                  ctypes.c_void_p(panel.data_ptr()), m, on_rows,
+                                  ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(U.data_ptr()),
+                                  ctypes.c_void_p(Vh.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                                  ctypes.c_void_p(info.data_ptr())), "ptn_svd_jacobi")
+    host = torch.cat([S, info]).cpu().numpy()
+    if int(np.frombuffer(host[k:k + 1].tobytes(), dtype=np.int32)[0]) != 0:
+        return None
+    SVD_STATS["jacobi"] += 1
+    return host[:k].copy(), U, Vh, k
+
+
+def _svd_direct(panel: torch.Tensor, m: int, n: int, on_rows: Optional[int] = None):
+    """Thin SVD of the column-major panel: one-sided Jacobi (on the columns of the
+    panel when m >= n, else of its conjugate transpose, or as ``on_rows`` says)
+    within the Jacobi limits, rocSOLVER beyond them."""
+    k = min(m, n)
+    if k <= SVD_JACOBI_MAX_SHORT and max(m, n) <= SVD_JACOBI_MAX_LONG:
+        rows = int(m < n) if on_rows is None else on_rows
+        hit = _svd_jacobi(panel, m, n, rows)
+        if hit is not None:
+            return hit
+        SVD_STATS["jacobi_fallback"] += 1
     dev = panel.device
     S = torch.empty(k, dtype=torch.float64, device=dev)
     U = torch.empty(m * k, dtype=torch.complex128, device=dev)
