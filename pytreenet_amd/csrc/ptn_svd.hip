@@ -81,16 +81,80 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-template <int TPP>
-__device__ void svd_jacobi_body(const SvdJob &job) {
+// one rotation sweep step of the pair (a, b) of G (rows mg) and (va, vb) of V
+// (rows ng) by the TPP lanes of a group; returns true when the pair was rotated.
+// Rows are handled UNR at a time with every load issued before its use, so a
+// lane waits for one memory latency per UNR rows, not per row.
+template <int TPP, int UNR>
+__device__ __forceinline__ bool rotate_pair(double2 *a, double2 *b, int64_t mg, double2 *va,
+                                            double2 *vb, int ng, int gl, double tol) {
+  double al = 0, be = 0, gr = 0, gi = 0;
+  for (int64_t i0 = gl; i0 < mg; i0 += (int64_t)TPP * UNR) {
+    double2 x[UNR], y[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int64_t i = i0 + (int64_t)u * TPP;
+      x[u] = i < mg ? a[i] : make_double2(0, 0);
+      y[u] = i < mg ? b[i] : make_double2(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      al += x[u].x * x[u].x + x[u].y * x[u].y;
+      be += y[u].x * y[u].x + y[u].y * y[u].y;
+      const double2 g = cmulc(x[u], y[u]);
+      gr += g.x;
+      gi += g.y;
+    }
+  }
+  al = group_sum<TPP>(al);
+  be = group_sum<TPP>(be);
+  gr = group_sum<TPP>(gr);
+  gi = group_sum<TPP>(gi);
+  const double ag = hypot(gr, gi);
+  if (!(ag > tol * sqrt(al) * sqrt(be) && ag > 0.0)) return false;
+  const double zeta = (be - al) / (2.0 * ag);
+  const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
+  const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+  const double er = gr / ag, ei = gi / ag;  // e^{i phi}
+  // a' = c a - s conj(e) b,  b' = s e a + c b
+  auto rot = [&](double2 *pa, double2 *pb, int64_t len) {
+    for (int64_t i0 = gl; i0 < len; i0 += (int64_t)TPP * UNR) {
+      double2 x[UNR], y[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int64_t i = i0 + (int64_t)u * TPP;
+        x[u] = i < len ? pa[i] : make_double2(0, 0);
+        y[u] = i < len ? pb[i] : make_double2(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int64_t i = i0 + (int64_t)u * TPP;
+        if (i < len) {
+          pa[i] = make_double2(c * x[u].x - s * (er * y[u].x + ei * y[u].y),
+                               c * x[u].y - s * (er * y[u].y - ei * y[u].x));
+          pb[i] = make_double2(s * (er * x[u].x - ei * x[u].y) + c * y[u].x,
+                               s * (er * x[u].y + ei * x[u].x) + c * y[u].y);
+        }
+      }
+    }
+  };
+  rot(a, b, mg);
+  rot(va, vb, ng);
+  return true;
+}
+
+// LDS_ELEMS complex128 of LDS hold G and V when mg ng + ng ng fits (128 KB)
+constexpr int64_t LDS_ELEMS = 8192;
+
+template <int TPP, bool IN_LDS>
+__device__ void svd_jacobi_body(const SvdJob &job, double2 *G, double2 *V) {
   __shared__ int rotated;
   __shared__ double sig[SVD_MAXN];
   __shared__ int rank_of[SVD_MAXN];
+  constexpr int UNR = IN_LDS ? 2 : 4;
   const bool rows = job.on_rows != 0;
   const int64_t mg = rows ? job.n : job.m;  // rows of G
   const int ng = (int)(rows ? job.m : job.n);  // columns of G
-  double2 *G = job.ws;                     // mg x ng, ld mg
-  double2 *V = job.ws + mg * ng;           // ng x ng, ld ng
   const int tid = threadIdx.x;
   // G = op(A), V = I
   for (int64_t e = tid; e < mg * ng; e += SVD_NT) {
@@ -119,46 +183,11 @@ __device__ void svd_jacobi_body(const SvdJob &job) {
       if (grp < npairs) {
         int p, q;
         rr_pair(N, r, grp, p, q);
-        if (q < ng) {
-          double2 *a = G + (int64_t)p * mg, *b = G + (int64_t)q * mg;
-          double al = 0, be = 0, gr = 0, gi = 0;
-          for (int64_t i = gl; i < mg; i += TPP) {
-            const double2 x = a[i], y = b[i];
-            al += x.x * x.x + x.y * x.y;
-            be += y.x * y.x + y.y * y.y;
-            const double2 g = cmulc(x, y);
-            gr += g.x;
-            gi += g.y;
-          }
-          al = group_sum<TPP>(al);
-          be = group_sum<TPP>(be);
-          gr = group_sum<TPP>(gr);
-          gi = group_sum<TPP>(gi);
-          const double ag = hypot(gr, gi);
-          if (ag > tol * sqrt(al) * sqrt(be) && ag > 0.0) {
-            const double zeta = (be - al) / (2.0 * ag);
-            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
-            const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-            const double er = gr / ag, ei = gi / ag;  // e^{i phi}
-            // a' = c a - s conj(e) b,  b' = s e a + c b
-            for (int64_t i = gl; i < mg; i += TPP) {
-              const double2 x = a[i], y = b[i];
-              a[i] = make_double2(c * x.x - s * (er * y.x + ei * y.y),
-                                  c * x.y - s * (er * y.y - ei * y.x));
-              b[i] = make_double2(s * (er * x.x - ei * x.y) + c * y.x,
-                                  s * (er * x.y + ei * x.x) + c * y.y);
-            }
-            double2 *va = V + (int64_t)p * ng, *vb = V + (int64_t)q * ng;
-            for (int i = gl; i < ng; i += TPP) {
-              const double2 x = va[i], y = vb[i];
-              va[i] = make_double2(c * x.x - s * (er * y.x + ei * y.y),
-                                   c * x.y - s * (er * y.y - ei * y.x));
-              vb[i] = make_double2(s * (er * x.x - ei * x.y) + c * y.x,
-                                   s * (er * x.y + ei * x.x) + c * y.y);
-            }
-            if (gl == 0) rotated = 1;
-          }
-        }
+        if (q < ng &&
+            rotate_pair<TPP, UNR>(G + (int64_t)p * mg, G + (int64_t)q * mg, mg,
+                                  V + (int64_t)p * ng, V + (int64_t)q * ng, ng, gl, tol) &&
+            gl == 0)
+          rotated = 1;
       }
       __syncthreads();
     }
@@ -210,14 +239,25 @@ __device__ void svd_jacobi_body(const SvdJob &job) {
 }
 
 template <int TPP>
+__device__ __forceinline__ void svd_jacobi_dispatch(const SvdJob &job) {
+  __shared__ double2 lds[LDS_ELEMS];
+  const int64_t mg = job.on_rows ? job.n : job.m, ng = job.on_rows ? job.m : job.n;
+  if (mg * ng + ng * ng <= LDS_ELEMS) {
+    svd_jacobi_body<TPP, true>(job, lds, lds + mg * ng);
+  } else {
+    svd_jacobi_body<TPP, false>(job, job.ws, job.ws + mg * ng);
+  }
+}
+
+template <int TPP>
 __global__ __launch_bounds__(SVD_NT) void svd_jacobi_one(SvdJob job) {
-  svd_jacobi_body<TPP>(job);
+  svd_jacobi_dispatch<TPP>(job);
 }
 
 template <int TPP>
 __global__ __launch_bounds__(SVD_NT) void svd_jacobi_many(const SvdJob *__restrict__ jobs) {
   const SvdJob job = jobs[blockIdx.x];
-  svd_jacobi_body<TPP>(job);
+  svd_jacobi_dispatch<TPP>(job);
 }
 
 int tpp_for(int ng) {

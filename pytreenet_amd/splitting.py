@@ -340,6 +340,32 @@ SVD_STATS = {"direct": 0, "qr_tall": 0, "qr_wide": 0, "qr_fallback": 0, "jacobi"
 SVD_QR_RATIO = 4
 
 
+class SvdRequest:
+    """One pending thin SVD of a column-major panel (the small dense step of a
+    truncated split): answered by ``answer_svd`` one at a time, or together with
+    other instances' requests by ``answer_svd_batched``."""
+    __slots__ = ("panel", "m", "n", "on_rows", "result")
+
+    def __init__(self, panel: torch.Tensor, m: int, n: int, on_rows: Optional[int] = None):
+        self.panel, self.m, self.n, self.on_rows = panel, m, n, on_rows
+        self.result = None
+
+
+def answer_svd(req: SvdRequest):
+    req.result = _svd_direct(req.panel, req.m, req.n, req.on_rows)
+
+
+def drive_svd(gen):
+    """Run a generator of SvdRequests, answering each one at a time."""
+    try:
+        req = next(gen)
+        while True:
+            answer_svd(req)
+            req = gen.send(None)
+    except StopIteration as stop:
+        return stop.value
+
+
 def svd_colmajor(panel: torch.Tensor, m: int, n: int):
     """Thin SVD of a column-major m x n panel (destroyed): (s host, U_cm, Vh_cm, k).
 
@@ -351,17 +377,25 @@ def svd_colmajor(panel: torch.Tensor, m: int, n: int):
     direct SVD to O(u ||A||).  A collapsed CholeskyQR pivot (numerically rank
     deficient panel) falls back to the direct SVD.
     """
+    return drive_svd(svd_colmajor_gen(panel, m, n))
+
+
+def svd_colmajor_gen(panel: torch.Tensor, m: int, n: int):
+    """``svd_colmajor`` as a generator: the dense SVD step is handed out as an
+    ``SvdRequest`` (``result = yield request``)."""
     short, long_ = min(m, n), max(m, n)
     if 8 <= short <= 1024 and long_ >= SVD_QR_RATIO * short:
-        hit = _svd_via_qr(panel, m, n)
+        hit = yield from _svd_via_qr_gen(panel, m, n)
         if hit is not None:
             return hit
         SVD_STATS["qr_fallback"] += 1
     SVD_STATS["direct"] += 1
-    return _svd_direct(panel, m, n)
+    req = SvdRequest(panel, m, n)
+    yield req
+    return req.result
 
 
-def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
+def _svd_via_qr_gen(panel: torch.Tensor, m: int, n: int):
     dev = panel.device
     T = panel.view(n, m)                 # T[j][i] = A(i, j)
     if m >= n:
@@ -371,7 +405,9 @@ def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
             return None
         B = E.permute_copy(R, [1, 0])      # B col-major n x n = R
         # Jacobi on the rows of the triangle (columns of R^H): QR-preconditioned
-        s, Ub, Vhb, k = _svd_direct(B.reshape(-1), n, n, on_rows=1)
+        req = SvdRequest(B.reshape(-1), n, n, on_rows=1)
+        yield req
+        s, Ub, Vhb, k = req.result
         U = torch.empty(m * k, dtype=torch.complex128, device=dev)
         E.contract(q, ["r", "i"], torch.as_strided(Ub, (n, k), (1, n)), ["r", "c"], ["i", "c"],
                    out=torch.as_strided(U, (m, k), (1, m)))
@@ -383,7 +419,9 @@ def _svd_via_qr(panel: torch.Tensor, m: int, n: int):
         return None
     B = R.contiguous().clone()             # col-major m x m of R^T: B(i, r) = R[r][i]
     # Jacobi on the rows of R^T (columns of conj(R)): QR-preconditioned
-    s, Ub, Whb, k = _svd_direct(B.reshape(-1), m, m, on_rows=1)
+    req = SvdRequest(B.reshape(-1), m, m, on_rows=1)
+    yield req
+    s, Ub, Whb, k = req.result
     Vh = torch.empty(k * n, dtype=torch.complex128, device=dev)
     E.contract(torch.as_strided(Whb, (k, m), (1, k)), ["c", "r"], q, ["j", "r"], ["c", "j"],
                out=torch.as_strided(Vh, (k, n), (1, k)))
@@ -419,6 +457,76 @@ def _svd_jacobi(panel: torch.Tensor, m: int, n: int, on_rows: int):
         return None
     SVD_STATS["jacobi"] += 1
     return host[:k].copy(), U, Vh, k
+
+
+def _jacobi_ok(m: int, n: int) -> bool:
+    return min(m, n) <= SVD_JACOBI_MAX_SHORT and max(m, n) <= SVD_JACOBI_MAX_LONG
+
+
+def answer_svd_batched(reqs: Sequence[SvdRequest]):
+    """Answer many independent SVD requests: those within the Jacobi limits in one
+    ``ptn_svd_jacobi_batched`` launch (one workgroup per matrix) and one host
+    read-back; the others, and any that did not converge, one at a time."""
+    todo = [r for r in reqs if _jacobi_ok(r.m, r.n)]
+    for r in reqs:
+        if not _jacobi_ok(r.m, r.n):
+            answer_svd(r)
+    if not todo:
+        return
+    if len(todo) == 1:
+        answer_svd(todo[0])
+        return
+    lib = _lib.load()
+    dev = todo[0].panel.device
+    ks = [min(r.m, r.n) for r in todo]
+    outs = []
+    ptrs = {k: [] for k in ("A", "S", "U", "Vh", "ws")}
+    meta = {"m": [], "n": [], "lda": [], "rows": []}
+    kt = sum(ks)
+    S_all = torch.empty(kt + 2 * len(todo), dtype=torch.float64, device=dev)
+    info = S_all[kt:].view(torch.int32)           # one int32 per request, after the S blocks
+    off = 0
+    for r, k in zip(todo, ks):
+        U = torch.empty(r.m * k, dtype=torch.complex128, device=dev)
+        Vh = torch.empty(k * r.n, dtype=torch.complex128, device=dev)
+        ws = torch.empty(int(lib.ptn_svd_jacobi_work_bytes(r.m, r.n)) // 16,
+                         dtype=torch.complex128, device=dev)
+        outs.append((U, Vh, ws, off))
+        ptrs["A"].append(r.panel.data_ptr())
+        ptrs["S"].append(S_all[off:].data_ptr())
+        ptrs["U"].append(U.data_ptr())
+        ptrs["Vh"].append(Vh.data_ptr())
+        ptrs["ws"].append(ws.data_ptr())
+        meta["m"].append(r.m)
+        meta["n"].append(r.n)
+        meta["lda"].append(r.m)
+        meta["rows"].append(int(r.m < r.n) if r.on_rows is None else int(r.on_rows))
+        off += k
+    info_p = [info[i:].data_ptr() for i in range(len(todo))]
+    arr = {k: np.ascontiguousarray(np.array(v, dtype=np.int64)) for k, v in ptrs.items()}
+    m_a = np.ascontiguousarray(np.array(meta["m"], dtype=np.int64))
+    n_a = np.ascontiguousarray(np.array(meta["n"], dtype=np.int64))
+    ld_a = np.ascontiguousarray(np.array(meta["lda"], dtype=np.int64))
+    rw_a = np.ascontiguousarray(np.array(meta["rows"], dtype=np.int32))
+    in_a = np.ascontiguousarray(np.array(info_p, dtype=np.int64))
+    work = torch.empty(int(lib.ptn_svd_jacobi_batched_work_bytes(len(todo))), dtype=torch.uint8,
+                       device=dev)
+    cp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    _lib.check(lib.ptn_svd_jacobi_batched(ctypes.c_void_p(E.stream_handle()), len(todo),
+                                          cp(m_a), cp(n_a), cp(arr["A"]), cp(ld_a), cp(rw_a),
+                                          cp(arr["S"]), cp(arr["U"]), cp(arr["Vh"]),
+                                          cp(arr["ws"]), cp(in_a),
+                                          ctypes.c_void_p(work.data_ptr())),
+               "ptn_svd_jacobi_batched")
+    host = S_all.cpu().numpy()
+    infos = np.frombuffer(host[kt:].tobytes(), dtype=np.int32)
+    for i, (r, k, (U, Vh, _ws, o)) in enumerate(zip(todo, ks, outs)):
+        if infos[i] != 0:
+            SVD_STATS["jacobi_fallback"] += 1
+            answer_svd(r)
+            continue
+        SVD_STATS["jacobi"] += 1
+        r.result = (host[o:o + k].copy(), U, Vh, k)
 
 
 def _svd_direct(panel: torch.Tensor, m: int, n: int, on_rows: Optional[int] = None):
@@ -526,6 +634,12 @@ def merge_two_sites(ttn, n1: str, n2: str) -> torch.Tensor:
 
 def split_two_sites(ttn, x: torch.Tensor, n1: str, n2: str, svd_params,
                     contr_mode: "ContractionMode" = None) -> np.ndarray:
+    """``split_two_sites_gen`` with its SVD answered at once."""
+    return drive_svd(split_two_sites_gen(ttn, x, n1, n2, svd_params, contr_mode))
+
+
+def split_two_sites_gen(ttn, x: torch.Tensor, n1: str, n2: str, svd_params,
+                        contr_mode: "ContractionMode" = None):
     """Split a merged two-site tensor (legs as ``merge_two_sites`` returns them)
     back into n1 and n2 by a truncated SVD, and write both node tensors.
 
@@ -533,7 +647,8 @@ def split_two_sites(ttn, x: torch.Tensor, n1: str, n2: str, svd_params,
     ``contr_truncated_svd_splitting``, ``util/tensor_splitting.py:541-572``, as
     used by ``TTN.split_node_svd``, ``core/ttn.py:1476-1497``); UCONTR puts S on
     U, EQUAL splits sqrt(S) over both.  The new bond takes the place of the old
-    one in each node.  Returns the kept singular values (host)."""
+    one in each node.  Returns the kept singular values (host).  A generator: the
+    dense SVD is handed out as an ``SvdRequest``."""
     if contr_mode is None:
         contr_mode = ContractionMode.VCONTR
     A1, A2 = ttn.tensors[n1], ttn.tensors[n2]
@@ -547,7 +662,7 @@ def split_two_sites(ttn, x: torch.Tensor, n1: str, n2: str, svd_params,
     v_legs = list(range(k1, k1 + k2)) + [k1 + k2 + 1]
     m, n = int(np.prod(udims)), int(np.prod(vdims))
     panel = _colmajor_panel(x, u_legs, v_legs)
-    svals, U, Vh, kfull = svd_colmajor(panel, m, n)
+    svals, U, Vh, kfull = yield from svd_colmajor_gen(panel, m, n)
     s_new, _ = truncate_singular_values(svals, svd_params)
     r = len(s_new)
     dev = x.device

@@ -10,6 +10,7 @@ Classes and methods keep the reference names, arguments and sweep schedules:
 * ``SecondOrderOneSiteTDVP`` -- ``tdvp_algorithms/secondorderonesite.py:11-193``
 * ``TwoSiteTDVPConfig`` / ``TwoSiteTDVP`` -- ``tdvp_algorithms/twositetdvp.py:18-146``
 * ``SecondOrderTwoSiteTDVP`` -- ``tdvp_algorithms/secondordertwosite.py:11-193``
+* ``FirstOrderTwoSiteTDVP`` -- ``tdvp_algorithms/firstordertwosite.py:9-30``
 * update / orthogonalisation paths -- ``time_evo_util/update_path.py:63-420``
 
 All tensors stay on the device.  The transient link tensor of a one-site
@@ -38,7 +39,9 @@ from .contractions import (SandwichCache, bond_apply, contract_any,
                            operator_tensor, scalar_product)
 from . import krylov as _K
 from .krylov import EvoDirection, TimeEvoMode, krylov_exp_action
-from .splitting import SVDParameters, SplitMode, merge_two_sites, split_two_sites
+from .splitting import (SVDParameters, SplitMode, SvdRequest, answer_svd,
+                        answer_svd_batched, merge_two_sites, split_two_sites,
+                        split_two_sites_gen)
 from .ttn import TreeTensorNetworkState
 from .ttno import TensorProduct, TreeTensorNetworkOperator
 
@@ -404,9 +407,12 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         try:
             req = next(gen)
             while True:
-                a0 = STATS.applies
-                req.result = self._krylov(req.x, req.action, req.dt, req.direction)
-                req.applies = STATS.applies - a0
+                if isinstance(req, SvdRequest):
+                    answer_svd(req)
+                else:
+                    a0 = STATS.applies
+                    req.result = self._krylov(req.x, req.action, req.dt, req.direction)
+                    req.applies = STATS.applies - a0
                 req = gen.send(None)
         except StopIteration as stop:
             return stop.value
@@ -693,8 +699,9 @@ class TwoSiteTDVP(TDVPAlgorithm):
         x = yield from self._evolve(x, make(x), self.time_step_size * time_step_factor,
                                     EvoDirection.FORWARD)
         self.apply_log.append(("two_site", (n1, n2), self._last_applies))
-        # truncated SVD, S absorbed into V (VCONTR)
-        s_new = split_two_sites(s, x, n1, n2, self.config)
+        # truncated SVD, S absorbed into V (VCONTR); the dense SVD is handed out as
+        # an SvdRequest (a BatchedTDVP answers all instances' in one launch)
+        s_new = yield from split_two_sites_gen(s, x, n1, n2, self.config)
         self.kept_dims.append(len(s_new))
         s.orthogonality_center_id = n2
         self.update_tree_cache(n1, n2)
@@ -920,7 +927,12 @@ class BatchedTDVP:
                 return
             if any(r is None for r in reqs):
                 raise RuntimeError("batched instances left the common sweep schedule")
-            self._solve(reqs)
+            if isinstance(reqs[0], SvdRequest):
+                if not all(isinstance(r, SvdRequest) for r in reqs):
+                    raise RuntimeError("batched instances left the common sweep schedule")
+                answer_svd_batched(reqs)
+            else:
+                self._solve(reqs)
             reqs = []
             for g in gens:
                 try:
