@@ -30,11 +30,15 @@ extern "C" {
 enum { PTN_C128 = 0, PTN_C64 = 1 };
 /* Compute modes of ptn_gemm_gather: FP64 (complex128 operands, FP64 MFMA),
  * 3M FP32 (FP32 MFMA, FP32 accumulation; complex64 or complex128 operands
- * rounded on load) and 3M bf16 (operands rounded to bf16 on load, FP32
- * accumulation; the reduced levels of the relaxed Krylov form, which the
- * reference does not have).  Small and thin products run on the FP64 wave
- * kernel in every mode; grouped launches take FP64 and FP32 only.           */
-enum { PTN_COMPUTE_F64_4M = 0, PTN_COMPUTE_F32_3M = 1, PTN_COMPUTE_BF16_3M = 2 };
+ * rounded on load), 3M bf16 (operands rounded to bf16 on load, FP32
+ * accumulation) and 3M bf16x6 (every FP32 operand value split into three bf16
+ * pieces, six piece products per real product on the bf16 MFMA, FP32
+ * accumulation: the accuracy of the FP32 mode at the bf16 MFMA rate).  The
+ * reduced modes serve the relaxed Krylov form, which the reference does not
+ * have.  Small and thin products run on the FP64 wave kernel in every mode;
+ * grouped launches take FP64 and FP32 only.                                  */
+enum { PTN_COMPUTE_F64_4M = 0, PTN_COMPUTE_F32_3M = 1, PTN_COMPUTE_BF16_3M = 2,
+       PTN_COMPUTE_BF16X6_3M = 3 };
 enum {
   PTN_OK = 0,
   PTN_EINVAL = -1,     /* bad argument / shape mismatch                       */

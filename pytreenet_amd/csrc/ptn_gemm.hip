@@ -544,6 +544,228 @@ __global__ __launch_bounds__(NT, 2) void gemm_bf16_kernel(KArgs p) {
   gather_tile_bf16<TA, TB, TC>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
 }
 
+// ---- BF16X6_3M: FP32-accurate 3M products on the bf16 MFMA -------------------------
+// Every FP32 operand value x is split into three bf16 pieces x = h + m + l (+ a
+// remainder below 2^-24 |x|): h = bf16(x), m = bf16(x - h), l = bf16(x - h - m),
+// each difference exact in FP32.  A real product a b is accumulated as the six
+// piece products of order >= 2^-16: hh + hm + mh + mm + hl + lh, with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16, 8 bf16 pairs per lane per MFMA), so
+// the GEMM carries the error of an FP32 GEMM (Henry, Tang and Heinecke, ARITH
+// 2019, "bf16x6").  The complex product uses Gauss's three real products
+// (T1 = ar br, T2 = ai bi, T3 = (ar + ai)(br + bi); the sums formed in FP32
+// before splitting): 18 MFMAs per 16 x 16 block product and K step 32.
+// Operands are gathered through the offset tables into registers (two
+// consecutive k per element pair), split, and stored as nine bf16 LDS planes
+// [re, im, re + im][h, m, l][row][k] per operand; 16-byte k chunks of a row are
+// XOR-swizzled by row bits 2-3, so the MFMA operand reads of 16 rows
+// (lane l: row l % 16, k = 8 (l / 16) + 0..7) hit distinct banks.  One LDS
+// stage (72 KB, two workgroups per CU) with the next stage's values prefetched
+// into registers during the MFMAs.
+constexpr int XBK = 32;
+typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
+
+typedef __bf16 v2bf16 __attribute__((ext_vector_type(2)));
+
+// the three-piece splits of two values x0, x1 (consecutive k), each piece pair
+// packed as one 32-bit word (x0's piece in the low half): one v_cvt_pk_bf16_f32
+// (round to nearest even) per piece pair
+__device__ __forceinline__ void split3_pair(float x0, float x1, unsigned &h, unsigned &m,
+                                            unsigned &l) {
+  const v2bf16 hv = {(__bf16)x0, (__bf16)x1};
+  h = __builtin_bit_cast(unsigned, hv);
+  float r0 = x0 - __uint_as_float(h << 16), r1 = x1 - __uint_as_float(h & 0xFFFF0000u);
+  const v2bf16 mv = {(__bf16)r0, (__bf16)r1};
+  m = __builtin_bit_cast(unsigned, mv);
+  r0 -= __uint_as_float(m << 16);
+  r1 -= __uint_as_float(m & 0xFFFF0000u);
+  const v2bf16 lv = {(__bf16)r0, (__bf16)r1};
+  l = __builtin_bit_cast(unsigned, lv);
+}
+
+__device__ __forceinline__ int xswz(int row, int k) {  // element index within a row
+  return ((((k >> 3) ^ ((row >> 2) & 3))) << 3) | (k & 7);
+}
+
+template <typename TA, typename TB, typename TC>
+__device__ __forceinline__ void gather_tile_bf16x6(const KArgs &p, int64_t bid, int64_t split,
+                                                   bool splitk) {
+  // [plane re/im/sum][piece h/m/l][row][k], as 32-bit words of two k
+  __shared__ __attribute__((aligned(16))) unsigned sA[3][3][BM][XBK / 2];
+  __shared__ __attribute__((aligned(16))) unsigned sB[3][3][BN][XBK / 2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const TA *A = static_cast<const TA *>(p.A);
+  const TB *B = static_cast<const TB *>(p.B);
+  // each thread stages four (row, k pair) slots of A and of B: 4 x 2 = 8 elements.
+  // k-major operand: the pair k = 2 (tid % 16) + {0,1}, rows tid / 16 + 16 i;
+  // m-major operand: row tid % 64, pairs k = 8 i + 2 (tid / 64) + {0,1}
+  int a_row[4], a_k[4], b_row[4], b_k[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (p.akmaj) {
+      a_k[i] = 2 * (tid & 15);
+      a_row[i] = (tid >> 4) + 16 * i;
+    } else {
+      a_row[i] = tid & 63;
+      a_k[i] = 8 * i + 2 * (tid >> 6);
+    }
+    if (p.bkmaj) {
+      b_k[i] = 2 * (tid & 15);
+      b_row[i] = (tid >> 4) + 16 * i;
+    } else {
+      b_row[i] = tid & 63;
+      b_k[i] = 8 * i + 2 * (tid >> 6);
+    }
+  }
+  int64_t a_moff[4], b_noff[4];
+  bool a_mval[4], b_nval[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t gm = m0 + a_row[i];
+    a_mval[i] = gm < p.M;
+    a_moff[i] = a_mval[i] ? p.aom[gm] : 0;
+    const int64_t gn = n0 + b_row[i];
+    b_nval[i] = gn < p.N;
+    b_noff[i] = b_nval[i] ? p.bon[gn] : 0;
+  }
+  float ra[4][2][2], rb[4][2][2];  // [slot][k of the pair][re, im]
+  const float asg = p.aconj ? -1.0f : 1.0f, bsg = p.bconj ? -1.0f : 1.0f;
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int64_t ka = k0 + a_k[i] + e;
+        const bool oka = a_mval[i] && ka < kend;
+        const int64_t offa = oka ? a_moff[i] + p.aok[ka] : 0;
+        float re, im;
+        ptn::cload(A + offa, re, im);
+        ra[i][e][0] = oka ? re : 0.0f;
+        ra[i][e][1] = oka ? asg * im : 0.0f;
+        const int64_t kb = k0 + b_k[i] + e;
+        const bool okb = b_nval[i] && kb < kend;
+        const int64_t offb = okb ? p.bok[kb] + b_noff[i] : 0;
+        ptn::cload(B + offb, re, im);
+        rb[i][e][0] = okb ? re : 0.0f;
+        rb[i][e][1] = okb ? bsg * im : 0.0f;
+      }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      unsigned h, m, l;
+      const int w = xswz(a_row[i], a_k[i]) >> 1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float x0 = q < 2 ? ra[i][0][q] : ra[i][0][0] + ra[i][0][1];
+        const float x1 = q < 2 ? ra[i][1][q] : ra[i][1][0] + ra[i][1][1];
+        split3_pair(x0, x1, h, m, l);
+        sA[q][0][a_row[i]][w] = h;
+        sA[q][1][a_row[i]][w] = m;
+        sA[q][2][a_row[i]][w] = l;
+      }
+      const int wb = xswz(b_row[i], b_k[i]) >> 1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float x0 = q < 2 ? rb[i][0][q] : rb[i][0][0] + rb[i][0][1];
+        const float x1 = q < 2 ? rb[i][1][q] : rb[i][1][0] + rb[i][1][1];
+        split3_pair(x0, x1, h, m, l);
+        sB[q][0][b_row[i]][wb] = h;
+        sB[q][1][b_row[i]][wb] = m;
+        sB[q][2][b_row[i]][wb] = l;
+      }
+    }
+  };
+
+  v4f acc[3][2][2];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[q][i][j] = v4f{0, 0, 0, 0};
+
+  const int64_t KT = kend > kbeg ? (kend - kbeg + XBK - 1) / XBK : 0;
+  const int fr = lane & 15, fk = lane >> 4;  // operand row, 8-k chunk
+  if (KT > 0) gload(kbeg);
+  for (int64_t kt = 0; kt < KT; ++kt) {
+    sstore();
+    __syncthreads();
+    if (kt + 1 < KT) gload(kbeg + (kt + 1) * XBK);  // in flight during the MFMAs
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      v8bf16 a[3][2], b[3][2];
+#pragma unroll
+      for (int pc = 0; pc < 3; ++pc)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const int ra_ = wm * 32 + i * 16 + fr, rb_ = wn * 32 + i * 16 + fr;
+          a[pc][i] = *reinterpret_cast<const v8bf16 *>(&sA[q][pc][ra_][xswz(ra_, fk * 8) >> 1]);
+          b[pc][i] = *reinterpret_cast<const v8bf16 *>(&sB[q][pc][rb_][xswz(rb_, fk * 8) >> 1]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          v4f c = acc[q][i][j];
+          // smallest pieces first
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], c, 0, 0, 0);
+          acc[q][i][j] = c;
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue (accumulator row of (lane, r) = 4 (lane / 16) + r) -------------------
+  TC *C = static_cast<TC *>(p.C);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t gn = n0 + wn * 32 + j * 16 + fr;
+    if (gn >= p.N) continue;
+    const int64_t noff = p.con[gn];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm * 32 + i * 16 + fk * 4 + r;
+        if (gm >= p.M) continue;
+        const double t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[2][i][j][r];
+        const double re = t1 - t2, im = t3 - t1 - t2;
+        if (splitk) {
+          p.ws[(split * p.M + gm) * p.N + gn] = make_double2(re, im);
+          continue;
+        }
+        double ore = p.are * re - p.aim * im;
+        double oim = p.are * im + p.aim * re;
+        TC *dst = C + p.com[gm] + noff;
+        if (p.beta_one) {
+          double xr, xi;
+          ptn::cload(dst, xr, xi);
+          ore += xr;
+          oim += xi;
+        }
+        ptn::cstore(dst, ore, oim);
+      }
+    }
+  }
+}
+
+template <typename TA, typename TB, typename TC, int LARGE>
+__global__ __launch_bounds__(NT, 2) void gemm_bf16x6_kernel(KArgs p) {
+  gather_tile_bf16x6<TA, TB, TC>(p, remapped_tile(p), blockIdx.y, gridDim.y > 1);
+}
+
 // ---- FP64 kernel with direct global->LDS tile loads ---------------------------
 // Each wave fills whole LDS rows of a tile with global_load_lds_dwordx4: lane l
 // copies element (k, m0 + l) straight into LDS, no register staging.  With m
@@ -1173,6 +1395,13 @@ int launch(hipStream_t s, const KArgs &a, int64_t tiles, int ksplit) {
       else
         hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC, 0>), dim3((unsigned)tiles, ksplit),
                            dim3(NT), 0, s, a);
+    } else if constexpr (MODE == PTN_COMPUTE_BF16X6_3M) {
+      if (large)
+        hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, 1>), dim3((unsigned)tiles, 1),
+                           dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, 0>), dim3((unsigned)tiles, ksplit),
+                           dim3(NT), 0, s, a);
     } else if (large && a.cmaj)
       hipLaunchKernelGGL((gemm_gather_kernel<MODE, TA, TB, TC, 1, true>),
                          dim3((unsigned)tiles, 1), dim3(NT), 0, s, a);
@@ -1226,7 +1455,7 @@ int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
   const bool dt_ok = (unsigned)d->a_dtype <= 1 && (unsigned)d->b_dtype <= 1 &&
                      (unsigned)d->c_dtype <= 1;
   if (!dt_ok || (d->compute != PTN_COMPUTE_F64_4M && d->compute != PTN_COMPUTE_F32_3M &&
-                 d->compute != PTN_COMPUTE_BF16_3M)) {
+                 d->compute != PTN_COMPUTE_BF16_3M && d->compute != PTN_COMPUTE_BF16X6_3M)) {
     ptn::set_error("ptn_gemm_gather: bad dtype/compute code");
     return PTN_EINVAL;
   }
@@ -1324,6 +1553,13 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
                                                       ksplit);
     return dispatch_b<PTN_COMPUTE_BF16_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
   }
+  if (d->compute == PTN_COMPUTE_BF16X6_3M) {
+    if (d->a_dtype == PTN_C128)
+      return dispatch_b<PTN_COMPUTE_BF16X6_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype,
+                                                        ksplit);
+    return dispatch_b<PTN_COMPUTE_BF16X6_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype,
+                                                     ksplit);
+  }
   if (d->a_dtype == PTN_C128)
     return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ksplit);
@@ -1380,6 +1616,12 @@ extern "C" int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *d, int3
       return dispatch_b<PTN_COMPUTE_BF16_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
     return dispatch_b<PTN_COMPUTE_BF16_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
   }
+  if (d->compute == PTN_COMPUTE_BF16X6_3M) {
+    if (d->a_dtype == PTN_C128)
+      return dispatch_b<PTN_COMPUTE_BF16X6_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype,
+                                                        ks);
+    return dispatch_b<PTN_COMPUTE_BF16X6_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
+  }
   if (d->a_dtype == PTN_C128)
     return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
@@ -1400,8 +1642,8 @@ extern "C" int ptn_gemm_gather_grouped(void *stream, const ptn_gemm_desc *descs,
   std::vector<int64_t> prefix(1, 0);
   ka.reserve(count);
   const ptn_gemm_desc &d0 = descs[0];
-  if (d0.compute == PTN_COMPUTE_BF16_3M) {
-    ptn::set_error("ptn_gemm_gather_grouped: no grouped form of the BF16 compute mode");
+  if (d0.compute == PTN_COMPUTE_BF16_3M || d0.compute == PTN_COMPUTE_BF16X6_3M) {
+    ptn::set_error("ptn_gemm_gather_grouped: no grouped form of the bf16 compute modes");
     return PTN_EINVAL;
   }
   for (int i = 0; i < count; ++i) {

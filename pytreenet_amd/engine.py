@@ -35,11 +35,14 @@ FP64 = "fp64"
 FP32 = "fp32"
 # 3M bf16 products with FP32 accumulation (the lowest level of RelaxedNetwork)
 BF16 = "bf16"
+# 3M products of three-piece bf16 splits (six piece products, FP32 accumulation):
+# FP32 accuracy at the bf16 MFMA rate (a low level of RelaxedNetwork)
+BF16X6 = "bf16x6"
 # FP64 environments and Krylov applications, each application run in 3M FP32 when
 # its contribution to the Krylov result is small enough (RelaxedNetwork)
 RELAXED = "relaxed"
 _COMPUTE = {FP64: _lib.PTN_COMPUTE_F64_4M, FP32: _lib.PTN_COMPUTE_F32_3M,
-            BF16: _lib.PTN_COMPUTE_BF16_3M}
+            BF16: _lib.PTN_COMPUTE_BF16_3M, BF16X6: _lib.PTN_COMPUTE_BF16X6_3M}
 _DTYPE_CODE = {torch.complex128: _lib.PTN_C128, torch.complex64: _lib.PTN_C64}
 
 
@@ -1203,7 +1206,8 @@ class RelaxedNetwork:
 
     ``high`` contracts complex128 inputs with FP64 GEMMs; ``low`` contracts
     complex64 copies of the same environments and operator with 3M FP32 GEMMs
-    (FP32 accumulation).  Calling the object runs ``high``.  The Krylov loop runs
+    (FP32 accumulation), either on the FP32 MFMA or as bf16x6 products on the
+    bf16 MFMA (the same accuracy).  Calling the object runs ``high``.  The Krylov loop runs
     ``low`` for an application whose error cannot move the result by more than
     ``relax_tol`` times the input's norm (``krylov.krylov_exp_action``), and the
     optional ``lower`` network (3M bf16 products, FP32 accumulation, relative error
@@ -1222,10 +1226,10 @@ class RelaxedNetwork:
     def __init__(self, high: "CompiledNetwork", low: "CompiledNetwork", relax_tol: float,
                  low_eps: Optional[float] = None, lower: Optional["CompiledNetwork"] = None,
                  lower_eps: Optional[float] = None):
-        if high.precision != FP64 or low.precision != FP32 or \
+        if high.precision != FP64 or low.precision not in (FP32, BF16X6) or \
                 (lower is not None and lower.precision != BF16):
-            raise ValueError("RelaxedNetwork pairs an FP64 network with an FP32 one "
-                             "(and optionally a BF16 one)")
+            raise ValueError("RelaxedNetwork pairs an FP64 network with an FP32 or BF16X6 "
+                             "one (and optionally a BF16 one)")
         self.high, self.low, self.lower = high, low, lower
         self.relax_tol = float(relax_tol)
         self.low_eps = self.LOW_EPS if low_eps is None else float(low_eps)

@@ -84,6 +84,9 @@ class TDVPConfig(TTNTimeEvolutionConfig):
     # precision="relaxed": also run applications of the smallest weights as 3M
     # bf16 products (FP32 accumulation)
     relax_bf16: bool = False
+    # precision="relaxed": arithmetic of the FP32-accurate level, "fp32" (FP32 MFMA)
+    # or "bf16x6" (six products of three-piece bf16 splits on the bf16 MFMA)
+    relax_low: str = "fp32"
 
 
 @dataclass
@@ -494,13 +497,19 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             # complex64 output: the last product then has complex64 operands
             # and result, like the others (rocBLAS cgemm where it is strided)
             action = self._relaxed(action, E.CompiledNetwork(t32, l32, o32,
-                                                             precision=E.FP32,
+                                                             precision=self._low_precision(),
                                                              out_dtype=torch.complex64),
                                    (t32, l32, o32))
 
         self.state.tensors[node_id] = yield from self._evolve(
             tensor, action, self.time_step_size * time_step_factor, EvoDirection.FORWARD)
         self.apply_log.append(("site", node_id, self._last_applies))
+
+    def _low_precision(self) -> str:
+        low = getattr(self.config, "relax_low", E.FP32)
+        if low not in (E.FP32, E.BF16X6):
+            raise ValueError(f"relax_low must be 'fp32' or 'bf16x6', not {low!r}")
+        return low
 
     def _relaxed(self, high, low, net32):
         lower = None
@@ -677,7 +686,7 @@ class TwoSiteTDVP(TDVPAlgorithm):
                 t32 = [h.tensor_in(n1, torch.complex64), h.tensor_in(n2, torch.complex64)] + \
                     [e.to(torch.complex64) for e in tensors[3:]]
                 return self._relaxed(high, E.CompiledNetwork([y] + t32, labels, out,
-                                                             precision=E.FP32,
+                                                             precision=self._low_precision(),
                                                              out_dtype=torch.complex64),
                                      ([y] + t32, labels, out))
             return make, k1, k2

@@ -611,3 +611,48 @@ def test_gather_gemm_bf16_3m(gpu, case):
     want = base + (-0.5 + 0.25j) * ref
     err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
     assert err < 3e-2, err
+
+
+@pytest.mark.parametrize("case", [
+    ((4000, 37), "mk", (37, 2100), "kn", "mn", torch.complex64, (False, False)),  # LARGE grid
+    ((40, 100, 3, 13), "abkc", (13, 3, 2100), "ckn", "nab", torch.complex64, (True, False)),
+    ((100, 3000), "mk", (3000, 90), "kn", "nm", torch.complex128, (False, True)),  # split K
+    ((70, 33), "mk", (33, 50), "kn", "mn", torch.complex64, (False, False)),       # ragged
+    ((2048, 1280), "mk", (1280, 256), "kn", "mn", torch.complex64, (False, False)),  # k-major A
+    ((1280, 2048), "km", (1280, 256), "kn", "mn", torch.complex64, (True, True)),    # m-major A
+])
+def test_gather_gemm_bf16x6_3m(gpu, case):
+    """The BF16X6 compute mode (three-piece bf16 splits, six piece products per real
+    product, FP32 accumulation) against einsum: within the accuracy of the FP32
+    mode on the same complex64 operands (so FP32 accuracy), yet not bitwise the
+    FP32 kernel's result; accumulation with alpha."""
+    sa, la, sb, lb, lo, adt, conj = case
+    _say(f"gather bf16x6 3M {la},{lb}->{lo} A={adt} conj={conj}")
+    rng = np.random.default_rng(sum(sa) + 11)
+    a, b = _crand(rng, sa), _crand(rng, sb)
+    ta, tb = _dev(a, adt), _dev(b, torch.complex64)
+    a_in = ta.cpu().numpy().astype(np.complex128)
+    b_in = tb.cpu().numpy().astype(np.complex128)
+    ref = np.einsum(f"{la},{lb}->{lo}", a_in.conj() if conj[0] else a_in,
+                    b_in.conj() if conj[1] else b_in)
+    M = int(np.prod([d for d, l in zip(sa, la) if l not in lb]))
+    N = int(np.prod([d for d, l in zip(sb, lb) if l not in la]))
+    K = int(np.prod([d for d, l in zip(sa, la) if l in lb]))
+    assert E.gemm_kernel_class(M, N, K, E.BF16X6) != "wave"
+    got = E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.BF16X6,
+                     out_dtype=torch.complex128).cpu().numpy()
+    f32 = E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.FP32,
+                     out_dtype=torch.complex128).cpu().numpy()
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    err32 = np.abs(f32 - ref).max() / scale
+    assert err < max(4 * err32, 1e-7) and err < 2e-6, (err, err32)
+    assert not np.array_equal(got, f32)
+    base = _crand(rng, ref.shape)
+    out = _dev(base, torch.complex64)
+    base = out.cpu().numpy().astype(np.complex128)
+    E.contract(ta, la, tb, lb, lo, conj_a=conj[0], conj_b=conj[1], precision=E.BF16X6, out=out,
+               accumulate=True, alpha=-0.5 + 0.25j)
+    want = base + (-0.5 + 0.25j) * ref
+    err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
+    assert err < 2e-6, err
