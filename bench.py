@@ -184,6 +184,60 @@ def cpu_baseline(chi_sample: int, steps_applies, shapes_full, wshapes_full):
     return rate, f_step, t_env + t_apply
 
 
+def c4_instances(count: int, seed0: int = 1000):
+    """Instances of the batching configuration: 31-node binary tree, random-field
+    Heisenberg (h_i ~ U[-2, 2], seed per instance), second-order two-site TDVP with
+    truncation at 1e-10 and chi_max = 128, dt = 0.05, from bond dimension 8."""
+    import pytreenet_amd as ptn
+    from pytreenet_amd.random_states import build_state, complete_tree, random_state_tensors
+    from pytreenet_amd.ttno import random_field_heisenberg
+    ids, parent, children = complete_tree(2, 4)
+    algs = []
+    for b in range(count):
+        st = build_state(ids, parent, children,
+                         random_state_tensors(ids, parent, children, 8, seed=seed0 + b),
+                         device="cuda")
+        H = random_field_heisenberg(st, 2.0, seed=seed0 + 100000 + b)
+        cfg = ptn.TwoSiteTDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, max_bond_dim=128,
+                                    rel_tol=1e-10, total_tol=1e-10)
+        a = ptn.SecondOrderTwoSiteTDVP(st, H, 0.05, 0.1, [], config=cfg,
+                                       solver_options={"max_krylov_dim": 10,
+                                                       "krylov_tol": 1e-10})
+        c = a.state.orthogonality_center_id
+        a.state.tensors[c] = a.state.tensors[c] / a.state.norm()
+        algs.append(a)
+    return algs
+
+
+def run_c4(count: int):
+    """Instance-steps/s of the batching configuration on ``count`` instances: the
+    first step (bonds grow from 8 to 128) and the second (at chi_max), batched in
+    lockstep and one instance at a time from the same states."""
+    import torch
+    import pytreenet_amd as ptn
+    out = {"instances": count}
+    for mode in ("batched", "one_at_a_time"):
+        algs = c4_instances(count)
+        torch.cuda.synchronize()
+        for step in range(2):
+            t0 = time.perf_counter()
+            if mode == "batched":
+                ptn.BatchedTDVP(algs).run(1)
+            else:
+                for a in algs:
+                    a.run_one_time_step()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            key = "growing" if step == 0 else "at_chi_max"
+            out[f"{mode}_{key}_instance_steps_per_s"] = count / dt
+            log(f"[c4 {mode}] step {step}: {dt:.2f}s")
+        del algs
+        torch.cuda.empty_cache()
+    out["speedup_at_chi_max"] = (out["batched_at_chi_max_instance_steps_per_s"] /
+                                 out["one_at_a_time_at_chi_max_instance_steps_per_s"])
+    return out
+
+
 def calibrate_copy():
     """8 contiguous copies of 1 GiB (complex128) by the ptn_permute kernel.
 
@@ -237,6 +291,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-chi", type=int, default=64)
+    ap.add_argument("--c4-instances", type=int, default=16,
+                    help="instances of the batching configuration measured beside the "
+                         "bench (0: skip)")
     ap.add_argument("--autotune", action="store_true",
                     help="launch the non-LARGE GEMMs on autotuned configurations")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -369,6 +426,11 @@ def main():
             "roofline": roofline, "cpu_baseline": cpu}
     line.update(res)
     line["autotune"] = {"on": E.AUTOTUNE, "tuned_signatures": len(E.TUNE_LOG)}
+    if args.c4_instances > 0:
+        r64 = run = ok = alg = None   # free the chi=256 runs before the batching measurement
+        E.clear_caches()
+        torch.cuda.empty_cache()
+        line["batched_config"] = run_c4(args.c4_instances)
     print(json.dumps(line), flush=True)
 
 

@@ -798,6 +798,11 @@ class SecondOrderTwoSiteTDVP(TwoSiteTDVP):
 # ---------------------------------------------------------------------------------------
 # batched TDVP: independent instances stepped together
 # ---------------------------------------------------------------------------------------
+def _high(action):
+    """The FP64 network of a relaxed action (the action itself otherwise)."""
+    return action.high if isinstance(action, E.RelaxedNetwork) else action
+
+
 class BatchedTDVP:
     """Many TDVP runs on the same tree (disorder realisations, parameter sweeps)
     advanced in lockstep.
@@ -832,9 +837,8 @@ class BatchedTDVP:
                     a.krylov_options() != a0.krylov_options():
                 raise ValueError("batched instances must share the algorithm, tree, time "
                                  "step, precision and solver options")
-        if any(a.relaxed for a in algs):
-            raise NotImplementedError("precision='relaxed' runs single instances; batched "
-                                      "instances use 'fp64' or 'fp32'")
+        if len({a.relaxed for a in algs}) > 1:
+            raise ValueError("batched instances must share the precision")
         for a in algs:
             a.dense_site_operator = True
         self.algorithms = algs
@@ -850,7 +854,7 @@ class BatchedTDVP:
         basis, the vectors of one step and the live intermediates of its H_eff
         network (planned order)."""
         from .krylov import KRYLOV_START_CAP
-        net = req.action
+        net = _high(req.action)
         n = req.x.numel()
         # the basis starts at KRYLOV_START_CAP vectors and doubles only while the
         # recurrence runs on; with no max_krylov_dim the budget takes twice that
@@ -911,7 +915,9 @@ class BatchedTDVP:
         dims: List[int] = []
         for c0 in range(0, len(reqs), chunk):
             part = reqs[c0:c0 + chunk]
-            net = E.BatchedNetwork([r.action for r in part])
+            # relaxed instances: the local spaces solved here (below SEQUENTIAL_N)
+            # lie below relax_min_size, where the relaxed form runs FP64 anyway
+            net = E.BatchedNetwork([_high(r.action) for r in part])
             res = krylov_exp_action_batched([r.x for r in part], net, part[0].dt, exp_factor,
                                             krylov_tol=opts.get("krylov_tol", 1e-10),
                                             max_krylov_dim=max_dim,

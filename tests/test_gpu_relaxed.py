@@ -121,18 +121,54 @@ def test_relaxed_two_site_kept_dims(gpu):
     assert max(errs) < 1e-9, errs
 
 
-def test_relaxed_rejected_in_batched(gpu):
-    alg, _, _ = _pair("second1", "relaxed", 1e-12, steps=1)
-    with pytest.raises(NotImplementedError):
-        ptn.BatchedTDVP([alg])
+def test_relaxed_instances_batched(gpu):
+    """Relaxed instances step in lockstep: local spaces from SEQUENTIAL_N on run the
+    relaxed single-instance solve (applications drop to FP32), smaller ones the
+    batched FP64 solve; each instance stays within 1e-9 of its oracle run."""
+    _say("relaxed instances in BatchedTDVP")
+    pairs = [_pair("second1", "relaxed", 1e-12, seed=s, steps=2) for s in (5, 6)]
+    bt = ptn.BatchedTDVP([a for a, _, _ in pairs])
+    bt.SEQUENTIAL_N = 1000
+    low0 = K.STATS.low_applies
+    for _ in range(2):
+        bt.run_one_time_step()
+        for _, otd, _ in pairs:
+            otd.run_one_time_step()
+    assert K.STATS.low_applies > low0
+    for alg, otd, _ in pairs:
+        e, e_ref = ptn.expectation_value(alg.state, alg.hamiltonian), otd.energy()
+        assert abs(e - e_ref) < 1e-9 * abs(e_ref)
+    with pytest.raises(ValueError):
+        ptn.BatchedTDVP([pairs[0][0], _pair("second1", "fp64", 0.0, steps=1)[0]])
+
+
+def test_relaxed_bf16x6_level_matches_oracle(gpu):
+    """relax_low="bf16x6": the FP32-accurate level runs as six products of
+    three-piece bf16 splits; with relax_tol 1e-12 it stays within 1e-9 of the
+    oracle, as the FP32 level does."""
+    _say("relaxed precision, bf16x6 level")
+    ids, parent, children = complete_tree(2, 3)
+    tensors = random_state_tensors(ids, parent, children, 16, 5)
+    st = build_state(ids, parent, children, tensors, device="cuda")
+    H = ptn.random_xyz_model(st, 5)
+    cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision="relaxed",
+                         relax_tol=1e-12, relax_min_size=0, relax_low="bf16x6")
+    alg = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.15, [], config=cfg,
+                                     solver_options={"krylov_tol": 1e-10, "max_krylov_dim": 10})
+    otd = OracleTDVP(OracleState(topology_from_ttn(st), tensors), host_tensors(H), 0.05,
+                     kind="second1", krylov_tol=1e-10, max_krylov_dim=10)
+    low0 = K.STATS.low_applies
+    errs = _run(alg, otd, 3)
+    assert K.STATS.low_applies > low0
+    assert max(errs) < 1e-9, errs
 
 
 def test_config3_relaxed_20_steps_energy_error(gpu):
     """The chi=256 configuration (ternary tree N=40, random XYZ, dt=0.05) over its
     20 steps: the relaxed path at the benchmark's relax_tol, with and without the
-    bf16 level, stays within |dE| <= 1e-8 of the FP64 path, and so do <Z_i> on
-    four sites (root, level 1, level 2, leaf) for the FP32 level (5e-8 with the
-    bf16 level), while part of the applications ran in reduced precision."""
+    bf16 level, stays within |dE| <= 1e-8 of the FP64 path, and so does <Z_i> on
+    every one of the 40 sites for the FP32 level (1e-7 with the bf16 level), while
+    part of the applications ran in reduced precision."""
     _say("config C3 full size, 20 steps, fp64 vs relaxed")
     import bench
     from pytreenet_amd import engine as E
@@ -151,18 +187,19 @@ def test_config3_relaxed_20_steps_energy_error(gpu):
         low0 = K.STATS.low_applies
         for step in range(20):
             alg.run_one_time_step()
-        sites = ["site0", "site1", "site4", "site13"]
+        from pytreenet_amd.contractions import multi_single_site_expectation_value
+        zs = multi_single_site_expectation_value(alg.state, {n: PAULI_Z for n in alg.state.nodes})
         res[run] = (ptn.expectation_value(alg.state, H).real,
-                    [alg.state.single_site_operator_expectation_value(n, PAULI_Z).real
-                     for n in sites], K.STATS.low_applies - low0)
+                    [zs[n][0].real for n in sorted(zs)], K.STATS.low_applies - low0)
         _say(f"  {run}: E={res[run][0]:.15f} reduced-precision applications {res[run][2]}")
         del alg, st, H
         E.clear_caches()
         torch.cuda.empty_cache()
     e64, z64, _ = res["fp64"]
-    # the bf16 level keeps the energy within the bar; its <Z_i> drift reaches
-    # 1.6e-8 over the 20 steps (measured), so its bar there is 5e-8
-    for run, zbar in (("relaxed", 1e-8), ("relaxed_bf16", 5e-8)):
+    # the bf16 level keeps the energy within the bar; its <Z_i> drift over all 40
+    # sites reaches 6.6e-8 after the 20 steps (measured), so its bar there is 1e-7:
+    # it misses the 1e-8 bar the FP32 level meets, and the bench does not take it
+    for run, zbar in (("relaxed", 1e-8), ("relaxed_bf16", 1e-7)):
         er, zr, low = res[run]
         assert low > 0, run
         assert abs(er - e64) <= 1e-8, (run, er, e64)
