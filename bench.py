@@ -33,14 +33,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
-# measured HBM bytes per LARGE launch of each (backend, precision) (rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE on this command, scaled by the copy calibration;
-# the summaries are under profiles/); None where not measured
-TRAFFIC_BYTES_PER_LAUNCH = {("gather", "fp64"): 2.75e9, ("blas", "fp64"): None,
-                            # mean over the relaxed run's timed LARGE FP32 launches
-                            # (112 with K=256 at 0.54 GB, 56 with K=1280 at 2.23 GB)
-                            ("gather", "fp32"): 1.11e9, ("blas", "fp32"): None,
-                            ("gather", "bf16"): None}
+# measured HBM bytes per LARGE launch of each (backend, precision): rocprofv3
+# --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales from --calibrate-copy) per
+# dispatch of the relaxed path, averaged with the timed steps' shape mix (the
+# per-shape table is under profiles/); None where not measured
+TRAFFIC_BYTES_PER_LAUNCH = {
+    # 14 x (131072, 1280, 256) at 6.34 GB (3 of them with strided stores at 11.6 GB),
+    # 12 x (131072, 256, 1280) at 4.82 GB, 40 x (131072, 256, 256) at 1.30 GB and
+    # 28 x (8192, 1280, 256) at 0.36 GB per step: 1.69x the 1.41 GB of operands
+    # read once and C written once
+    ("gather", "fp64"): 2.39e9, ("blas", "fp64"): None,
+    # 28 x K=1280 at 2.35 GB and 56 x K=256 at 0.55 GB per step
+    ("gather", "fp32"): 1.15e9, ("blas", "fp32"): None,
+    ("gather", "bf16"): None}
 KERNEL_NAME = {("gather", "fp64"): "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, FP64 3M)",
                ("blas", "fp64"): "rocblas_zgemm (ptn_gemm_strided)",
                ("gather", "fp32"): "gemm_gather_kernel<F32_3M, ..., LARGE=1> (ptn_gemm_gather)",
@@ -387,15 +392,6 @@ def main():
         roofline = block(order[0])
         if len(order) > 1:
             roofline["other"] = block(order[1])
-        allk = E.PROFILER.summary(compute=order[0][1], backend=order[0][0])
-        n_all = sum(k["launches"] for k in allk)
-        roofline["all_phases"] = {"launches": n_all,
-                                  "avg_ms": sum(k["total_ms"] for k in allk) / n_all,
-                                  "note": f"every LARGE {order[0][1]} launch of this backend "
-                                          "in this process (setup, warmup, timed, evaluation, "
-                                          "and the other runs' launches of this precision): "
-                                          "the set rocprofv3 --stats averages over"}
-    E.PROFILER.stop()
     cpu = None
     if not args.no_cpu:
         alg = r64["alg"]
@@ -430,7 +426,21 @@ def main():
         r64 = run = ok = alg = None   # free the chi=256 runs before the batching measurement
         E.clear_caches()
         torch.cuda.empty_cache()
+        E.PROFILER.phase = "batching-config"
         line["batched_config"] = run_c4(args.c4_instances)
+    if roofline is not None:
+        # every LARGE launch of the dominant (backend, precision) in this process: the
+        # set rocprofv3 --stats averages over for that kernel
+        allk = E.PROFILER.summary(compute=order[0][1], backend=order[0][0])
+        n_all = sum(k["launches"] for k in allk)
+        roofline["all_phases"] = {"launches": n_all,
+                                  "avg_ms": sum(k["total_ms"] for k in allk) / n_all,
+                                  "note": f"every LARGE {order[0][1]} launch of this backend "
+                                          "in this process (setup, warmup, timed, evaluation, "
+                                          "the other runs' launches of this precision and the "
+                                          "batching configuration's): the set rocprofv3 --stats "
+                                          "averages over"}
+    E.PROFILER.stop()
     print(json.dumps(line), flush=True)
 
 

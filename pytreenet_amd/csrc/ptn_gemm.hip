@@ -52,6 +52,10 @@ static_assert(LA >= 1 && BM * BK % NT == 0, "tile / thread count mismatch");
 constexpr int64_t SPLITK_MIN_K = 128;
 constexpr int64_t SPLITK_TARGET_BLOCKS = 2048;  // 4 waves of 2 blocks x 256 CUs
 constexpr int64_t LARGE_TILES = 2048;
+#ifndef PTN_GEMM_GROUP_M
+#define PTN_GEMM_GROUP_M 8
+#endif
+constexpr int64_t GROUP_M = PTN_GEMM_GROUP_M;   // tile rows per raster group
 
 struct KArgs {
   int64_t M, N, K;
@@ -63,11 +67,35 @@ struct KArgs {
   const int64_t *com, *con;
   double are, aim;
   int64_t tiles_n;
+  int64_t tiles_m;
+  int64_t group_m;  // > 1: tile rows raster in groups of group_m (tile_coords)
   int64_t kchunk;   // K range per split (multiple of BK)
   double2 *ws;      // split-K partials [ksplit][M][N], null when gridDim.y == 1
   int aconj, akmaj, bconj, bkmaj, beta_one, remap;
   int cmaj;         // consecutive m contiguous in C (store-order hint)
 };
+
+// (tile row, tile column) of linear tile index bid.  With group_m > 1 the tile
+// rows come in groups of group_m and a group's tiles are numbered down its rows
+// first: the workgroups in flight on one XCD (a contiguous index range after the
+// remap) then share group_m A panels and a few B panels in that XCD's L2, where
+// the row-major order cycles every workgroup row through all tiles_n B panels
+// (5 MB for N = 1280 at K = 256, more than one XCD's 4 MB L2).
+__device__ __forceinline__ void tile_coords(const KArgs &p, int64_t bid, int64_t &tm,
+                                            int64_t &tn) {
+  if (p.group_m > 1) {
+    const int64_t span = p.group_m * p.tiles_n;
+    const int64_t g = bid / span;
+    const int64_t local = bid - g * span;
+    const int64_t left = p.tiles_m - g * p.group_m;
+    const int64_t rows = left < p.group_m ? left : p.group_m;
+    tm = g * p.group_m + local % rows;
+    tn = local / rows;
+  } else {
+    tm = bid / p.tiles_n;
+    tn = bid - tm * p.tiles_n;
+  }
+}
 
 template <int MODE>
 struct Acc;
@@ -112,7 +140,8 @@ __device__ __forceinline__ void gather_tile(const KArgs &p, int64_t bid, int64_t
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  int64_t tm, tn;
+  tile_coords(p, bid, tm, tn);
   const int64_t m0 = tm * BM, n0 = tn * BN;
   // split-K: this block covers k in [kbeg, kend)
   const int64_t kbeg = split * p.kchunk;
@@ -360,7 +389,8 @@ __device__ __forceinline__ void gather_tile_bf16(const KArgs &p, int64_t bid, in
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  int64_t tm, tn;
+  tile_coords(p, bid, tm, tn);
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
@@ -596,7 +626,8 @@ __device__ __forceinline__ void gather_tile_bf16x6(const KArgs &p, int64_t bid, 
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  int64_t tm, tn;
+  tile_coords(p, bid, tm, tn);
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
@@ -804,7 +835,8 @@ __device__ __forceinline__ void direct_tile(const KArgs &p, const int64_t *__res
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
-  const int64_t tm = bid / p.tiles_n, tn = bid - tm * p.tiles_n;
+  int64_t tm, tn;
+  tile_coords(p, bid, tm, tn);
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
@@ -1490,6 +1522,8 @@ int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
   a.beta_one = d->beta_one;
   const int64_t tiles_m = (d->M + BM - 1) / BM;
   a.tiles_n = (d->N + BN - 1) / BN;
+  a.tiles_m = tiles_m;
+  a.group_m = 0;
   tiles = tiles_m * a.tiles_n;
   if (tiles > (int64_t)(0xffffffffu / NT)) {
     ptn::set_error("ptn_gemm_gather: %lld tiles exceed one launch", (long long)tiles);
@@ -1522,6 +1556,8 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
                        d->b_dtype, d->c_dtype);
   }
   a.remap = (tiles % 8 == 0) ? 1 : 0;
+  // grouped raster for the LARGE grids with many tile columns
+  if (tiles >= LARGE_TILES && a.tiles_n >= 8) a.group_m = GROUP_M;
   // split-K when the tile grid cannot fill the chip and K is long: each split
   // keeps >= SPLITK_MIN_K of K, the partials (ksplit M N) stay within the
   // operand traffic (K (M + N), at least two splits' worth) and the caller's

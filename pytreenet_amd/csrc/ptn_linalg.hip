@@ -210,7 +210,7 @@ __global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 
   __shared__ double2 D[PB][PB + 1];
   __shared__ double2 Pn[PSTAGE];
   __shared__ double red[PT / 64];
-  __shared__ double sh_shift, sh_inv;
+  __shared__ double sh_shift;
   __shared__ int sh_fail;
   const int tid = threadIdx.x;
   double tr = 0;

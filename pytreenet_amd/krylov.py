@@ -25,6 +25,7 @@ from __future__ import annotations
 import ctypes
 import math
 from enum import Enum
+from contextlib import contextmanager
 from typing import Any, Callable, Dict, List, Optional, Union
 
 import numpy as np
@@ -253,6 +254,58 @@ class KrylovStats:
 STATS = KrylovStats()
 
 
+class PendingApplies:
+    """Krylov dimension of a one-launch solve whose info has not been read back
+    yet (``deferred_small_info``); ``resolve`` turns it into an int."""
+    __slots__ = ("index",)
+
+    def __init__(self, index: int):
+        self.index = index
+
+
+class _SmallDefer:
+    def __init__(self):
+        self.active = False
+        self.infos: List[torch.Tensor] = []
+
+
+_SMALL_DEFER = _SmallDefer()
+
+
+@contextmanager
+def deferred_small_info():
+    """Within the block, a one-launch Krylov solve whose depth cannot run out
+    (the allowed depth fits the kernel) returns its result without reading its
+    (m, zero, stopped) info back, which would stall the device queue once per
+    solve.  The solve's Krylov dimension is handed out as a ``PendingApplies``
+    (``last_pending``); ``resolve_pending`` reads every info of the block with
+    one copy, records the dimensions in STATS and maps each PendingApplies to
+    its int."""
+    prev = (_SMALL_DEFER.active, _SMALL_DEFER.infos)
+    _SMALL_DEFER.active, _SMALL_DEFER.infos = True, []
+    try:
+        yield _SMALL_DEFER
+    finally:
+        _SMALL_DEFER.active, _SMALL_DEFER.infos = prev
+
+
+def pending_count() -> int:
+    """Number of deferred one-launch solves so far in the current block."""
+    return len(_SMALL_DEFER.infos) if _SMALL_DEFER.active else 0
+
+
+def resolve_pending(defer: _SmallDefer) -> List[int]:
+    """Krylov dimensions of the block's deferred solves (one device -> host copy),
+    recorded in STATS in solve order."""
+    if not defer.infos:
+        return []
+    ms = [int(v) for v in torch.stack(defer.infos)[:, 0].cpu().tolist()]
+    for m in ms:
+        STATS.record(m)
+    defer.infos = []
+    return ms
+
+
 LOOKAHEAD_N = 1 << 16
 MAXV_PER_CALL = 128      # basis vectors per ptn_krylov_orthonormalize call
 
@@ -464,6 +517,12 @@ def krylov_exp_small(psi: torch.Tensor, hamiltonian: torch.Tensor, time_differen
                                          float(time_difference), f.real, f.imag,
                                          float(krylov_tol), 1 if hermitian else 0,
                                          _vp(work), _vp(info)), "ptn_krylov_expm_small")
+    if _SMALL_DEFER.active and steps == max_steps:
+        # the recurrence cannot need the general form (its whole allowed depth
+        # runs in the kernel), and a zero input leaves y = x: y is final, only
+        # the Krylov dimension is read later
+        _SMALL_DEFER.infos.append(info)
+        return y.view(psi.shape)
     dev_idx = dev.index or 0
     host = _PINNED_INFO.get(dev_idx)
     if host is None:

@@ -413,9 +413,12 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
                 if isinstance(req, SvdRequest):
                     answer_svd(req)
                 else:
-                    a0 = STATS.applies
+                    a0, p0 = STATS.applies, _K.pending_count()
                     req.result = self._krylov(req.x, req.action, req.dt, req.direction)
-                    req.applies = STATS.applies - a0
+                    # a one-launch solve of a deferred block hands out its Krylov
+                    # dimension later (resolved at the end of the time step)
+                    req.applies = (_K.PendingApplies(p0) if _K.pending_count() > p0
+                                   else STATS.applies - a0)
                 req = gen.send(None)
         except StopIteration as stop:
             return stop.value
@@ -458,7 +461,8 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             return
         from .splitting import deferred_qr_checks
         snap = self._snapshot()
-        with deferred_qr_checks() as checks:
+        nlog = len(self.apply_log)
+        with deferred_qr_checks() as checks, _K.deferred_small_info() as small:
             try:
                 self._drive(self._time_step_gen())
             except Exception:
@@ -468,6 +472,14 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
                 failed = True
             else:
                 failed = checks.failed()
+            if not failed:
+                # the Krylov dimensions of the step's one-launch solves, read once
+                ms = _K.resolve_pending(small)
+                self.apply_log[nlog:] = [
+                    (k, n, ms[a.index] if isinstance(a, _K.PendingApplies) else a)
+                    for k, n, a in self.apply_log[nlog:]]
+                if isinstance(self._last_applies, _K.PendingApplies):
+                    self._last_applies = ms[self._last_applies.index]
         if failed:
             TDVPAlgorithm.steps_redone += 1
             self._restore(snap)
