@@ -6,7 +6,8 @@ seeded random state.  ``golden/config2_chi64.json`` holds the oracle's energy,
 overlap <psi_0|psi_t> and <Z_i> of every site after each of its steps, at
 krylov_tol 0 (fixed Krylov dimension) and 1e-10 (``golden/make_config2_fixture.py``
 wrote it on the CPU).  Bar: 1e-12 relative on the energy and the overlap, 1e-12
-absolute on <Z_i> (the state is normalised)."""
+absolute on <Z_i> (the state is normalised); the Krylov dimension of every solve
+equals the oracle's."""
 import json
 import os
 
@@ -48,9 +49,15 @@ def test_config2_full_size_matches_oracle_series(gpu, tol):
     psi0 = alg.state.copy()
     first_miss = None
     worst = 0.0
+    dims_differ = []
     for step in range(cfg["steps"] + 1):
         if step:
+            nlog = len(alg.apply_log)
             alg.run_one_time_step()
+            # Krylov dimension of every site and bond solve, in sweep order
+            dims = [a for _, _, a in alg.apply_log[nlog:]]
+            if dims != ref["krylov_dims"][step - 1]:
+                dims_differ.append(step)
         e = ptn.expectation_value(alg.state, H)
         ov = ptn.contract_two_ttns(alg.state, psi0, conj_second=True)    # <psi_0|psi_t>
         z = multi_single_site_expectation_value(alg.state, {n: PAULI_Z for n in ids})
@@ -64,3 +71,4 @@ def test_config2_full_size_matches_oracle_series(gpu, tol):
         if first_miss is None and max(de, dov, dz) >= REL:
             first_miss = step
     assert first_miss is None, f"first step beyond 1e-12: {first_miss} (worst {worst:.1e})"
+    assert not dims_differ, f"Krylov dimensions differ from the oracle's at steps {dims_differ}"
