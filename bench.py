@@ -75,7 +75,7 @@ def make_problem(chi: int, device):
 
 
 def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
-             relax_tol: float = RELAX_TOL, relax_bf16: bool = False):
+             relax_tol: float = RELAX_TOL, relax_bf16: bool = False, relax_low: str = "fp32"):
     import torch
     import pytreenet_amd as ptn
     from pytreenet_amd import engine as E
@@ -83,7 +83,7 @@ def run_tdvp(st, H, precision: str, warmup: int, steps: int, profile: bool,
     from pytreenet_amd.contractions import multi_single_site_expectation_value
     from pytreenet_amd.ttno import PAULI_Z
     cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV, precision=precision,
-                         relax_tol=relax_tol, relax_bf16=relax_bf16)
+                         relax_tol=relax_tol, relax_bf16=relax_bf16, relax_low=relax_low)
     tag = precision + ("-bf16" if relax_bf16 else "")
     E.PROFILER.phase = f"{tag}-init"
     t0 = time.time()
@@ -299,6 +299,9 @@ def main():
     ap.add_argument("--c4-instances", type=int, default=16,
                     help="instances of the batching configuration measured beside the "
                          "bench (0: skip)")
+    ap.add_argument("--relax-low", choices=("fp32", "bf16x6"), default="fp32",
+                    help="arithmetic of the relaxed paths' FP32-accurate level: the FP32 "
+                         "MFMA, or six products of three-piece bf16 splits on the bf16 MFMA")
     ap.add_argument("--autotune", action="store_true",
                     help="launch the non-LARGE GEMMs on autotuned configurations")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -334,7 +337,8 @@ def main():
     if not args.no_mixed:
         for prec, key, bf16 in (("relaxed", "relaxed", False),
                                 ("relaxed", "relaxed_bf16", True), ("fp32", "mixed", False)):
-            r = run_tdvp(st, H, prec, args.warmup, args.steps, profile=True, relax_bf16=bf16)
+            r = run_tdvp(st, H, prec, args.warmup, args.steps, profile=True, relax_bf16=bf16,
+                         relax_low=args.relax_low)
             sps = args.steps / r["elapsed"]
             de = abs(r["energy"] - r64["energy"])
             dz = max(abs(r["z"][n] - r64["z"][n]) for n in r64["z"])
@@ -418,7 +422,8 @@ def main():
             "data": "synthetic",
             "config": {"workload": "2nd-order 1-site TDVP, ternary tree N=40, chi=256, d=2, "
                                    "random XYZ TTNO D=5, dt=0.05, krylov m<=10 tol 1e-10",
-                       "chi": args.chi, "nodes": 40, "precision": path},
+                       "chi": args.chi, "nodes": 40, "precision": path,
+                       "relax_low": args.relax_low},
             "roofline": roofline, "cpu_baseline": cpu}
     line.update(res)
     line["autotune"] = {"on": E.AUTOTUNE, "tuned_signatures": len(E.TUNE_LOG)}
