@@ -75,15 +75,23 @@ def _op_labels(ket_node: Node, op_node: Node) -> list:
 
 
 USE_BLOCK_SPARSE = True
+# Site tensors below this many elements meet the dense TTNO tensor: their networks
+# are bound by launches, and block sparsity splits each operator step into one GEMM
+# per block group (an environment update of a 256-element site tensor: about 30
+# launches instead of 9).
+BLOCK_SPARSE_MIN_SIZE = 1 << 16
 
 
 def operator_tensor(operator: TreeTensorNetworkOperator, node_id: str, precision: str,
-                    block: Optional[bool] = None):
+                    block: Optional[bool] = None, size: Optional[int] = None):
     """TTNO node tensor as handed to the networks: block sparse over its bond-state
-    legs (structural zeros skipped exactly) unless ``block`` (default
-    ``USE_BLOCK_SPARSE``) is off."""
+    legs (structural zeros skipped exactly) unless ``block`` is off; by default
+    (``block=None``) when ``USE_BLOCK_SPARSE`` is on and the site tensor it meets has
+    ``size`` >= ``BLOCK_SPARSE_MIN_SIZE`` elements (or its size is not given)."""
     dt = E.storage_dtype(precision)
-    if USE_BLOCK_SPARSE if block is None else block:
+    if block is None:
+        block = USE_BLOCK_SPARSE and (size is None or size >= BLOCK_SPARSE_MIN_SIZE)
+    if block:
         return operator.block_tensor(node_id, dt)
     return operator.tensor_in(node_id, dt)
 
@@ -109,7 +117,8 @@ def contract_any(node_id: str, next_node_id: str, state: TreeTensorNetworkState,
         if len(missing) != 1:
             raise KeyError(f"cannot infer the open leg of {node_id} towards {next_node_id}")
         to = missing[0]
-    return env_block(ket_node, ket, op_node, operator_tensor(operator, node_id, precision),
+    return env_block(ket_node, ket, op_node,
+                     operator_tensor(operator, node_id, precision, size=ket.numel()),
                      [dictionary.get((nb, node_id)) if i != to else None
                       for i, nb in enumerate(nbs)], to,
                      bra=None if bra_state is None else bra_state.tensors[node_id],
@@ -244,7 +253,7 @@ def expectation_value(state: TreeTensorNetworkState, operator: TreeTensorNetwork
     node, ket = state[r]
     op_node = operator.nodes[r]
     k = node.nneighbours()
-    tensors = [ket, operator_tensor(operator, r, precision), ket]
+    tensors = [ket, operator_tensor(operator, r, precision, size=ket.numel()), ket]
     labels = [[("a", i) for i in range(k)] + ["p"], _op_labels(node, op_node),
               [("b", i) for i in range(k)] + ["q"]]
     for i, nb in enumerate(node.neighbouring_nodes()):

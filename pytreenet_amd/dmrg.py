@@ -145,7 +145,7 @@ class DMRGAlgorithm:
 
     def _update_one_site(self, node_id: str) -> float:
         node, tensor = self.state[node_id]
-        op = operator_tensor(self.hamiltonian, node_id, self.precision)
+        op = operator_tensor(self.hamiltonian, node_id, self.precision, size=tensor.numel())
         envs = [self.partial_tree_cache.get_entry(nb, node_id)
                 for nb in node.neighbouring_nodes()]
         tensors, labels, out = heff_network(node, self.hamiltonian.nodes[node_id], op, envs)

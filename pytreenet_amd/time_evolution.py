@@ -492,7 +492,8 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         node, tensor = self.state[node_id]
         op_node = self.hamiltonian.nodes[node_id]
         op = operator_tensor(self.hamiltonian, node_id, self.precision,
-                             block=False if self.dense_site_operator else None)
+                             block=False if self.dense_site_operator else None,
+                             size=tensor.numel())
         cache = self.partial_tree_cache
         envs = [cache.get_entry(nb, node_id) for nb in node.neighbouring_nodes()]
         tensors, labels, out_l = heff_network(node, op_node, op, envs)
@@ -502,7 +503,8 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         if getattr(self, "relaxed", False) and \
                 tensor.numel() >= getattr(self.config, "relax_min_size", 0):
             op32 = operator_tensor(self.hamiltonian, node_id, E.FP32,
-                                   block=False if self.dense_site_operator else None)
+                                   block=False if self.dense_site_operator else None,
+                                   size=tensor.numel())
             t32, l32, o32 = heff_network(node, op_node, op32,
                                          [e.to(torch.complex64) for e in envs])
             t32[0] = tensor

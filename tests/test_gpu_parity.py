@@ -188,6 +188,8 @@ def test_block_sparse_ttno_matches_dense(gpu):
     envs = {}
     dist = topo.distance_to_node(c)
     cache_b, cache_d = ptn.PartialTreeCachDict(), ptn.PartialTreeCachDict()
+    min_size = PC.BLOCK_SPARSE_MIN_SIZE
+    PC.BLOCK_SPARSE_MIN_SIZE = 0     # block sparse at every size of this small tree
     try:
         for d in range(max(dist.values()), 0, -1):
             for n in [m for m in dist if dist[m] == d]:
@@ -211,6 +213,7 @@ def test_block_sparse_ttno_matches_dense(gpu):
         assert _rel(e_b, e_d) < REL
     finally:
         PC.USE_BLOCK_SPARSE = True
+        PC.BLOCK_SPARSE_MIN_SIZE = min_size
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
