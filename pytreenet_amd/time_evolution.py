@@ -831,7 +831,9 @@ class BatchedTDVP:
     (``engine.BatchedNetwork``), the Arnoldi vector work as batched launches, one
     host round trip per Krylov step.  Splits, truncation and environment updates
     run per instance between those points.  Each instance follows exactly the
-    single-instance algorithm.
+    single-instance algorithm.  ``lockstep_group`` steps the instances in groups of
+    that many, one group after another, to bound the peak memory (512 instances of
+    the 31-node chi = 128 configuration need groups of 256 on one 288 GB GPU).
     """
 
     # Krylov requests with local spaces of at least SEQUENTIAL_N elements are solved
@@ -859,6 +861,13 @@ class BatchedTDVP:
         self.krylov_dims: List[int] = []
         self.memory_fraction = 0.5   # share of free device memory a batched solve may use
         self.max_batch: Optional[int] = None
+        # called as progress(points_done, kind) after every lockstep point of a step
+        # ("krylov" or "svd"): for callers that report long steps as they go
+        self.progress = None
+        # instances stepped together: every instance of a lockstep group holds its
+        # merged two-site tensor and QR factors while the group's SVDs are answered,
+        # so groups bound the peak memory (None: all instances in one group)
+        self.lockstep_group: Optional[int] = None
 
     def __len__(self):
         return len(self.algorithms)
@@ -882,20 +891,22 @@ class BatchedTDVP:
         esize = torch.empty((), dtype=E.storage_dtype(net.precision)).element_size()
         return 16 * n * (m + 5) + esize * live
 
-    def _solve(self, reqs: List[KrylovRequest]):
+    def _solve(self, reqs: List[KrylovRequest],
+               algorithms: Optional[Sequence[TDVPAlgorithm]] = None):
         from .krylov import krylov_exp_action_batched
-        a0 = self.algorithms[0]
+        algorithms = self.algorithms if algorithms is None else algorithms
+        a0 = algorithms[0]
         if a0._mode is not TimeEvoMode.KRYLOV and not (
                 a0._mode is TimeEvoMode.FASTEST and reqs[0].x.numel() > a0.DENSE_MAX_DIM):
             # dense exponential modes: one dense solve per instance
-            for a, r in zip(self.algorithms, reqs):
+            for a, r in zip(algorithms, reqs):
                 r.result, r.applies = a._krylov(r.x, r.action, r.dt, r.direction), 0
             return
         # large local spaces: one instance at a time on the single-instance path
         # (its LARGE GEMMs fill the chip; the grouped launches and the batched
         # buffers only add memory traffic and allocations)
         small = []
-        for a, r in zip(self.algorithms, reqs):
+        for a, r in zip(algorithms, reqs):
             if r.x.numel() >= self.SEQUENTIAL_N:
                 from .krylov import STATS
                 n0 = STATS.applies
@@ -947,10 +958,16 @@ class BatchedTDVP:
 
     def run_one_time_step(self):
         E.refresh_stream()
-        gens = [a._time_step_gen() for a in self.algorithms]
+        size = self.lockstep_group or len(self.algorithms)
+        for g0 in range(0, len(self.algorithms), size):
+            self._lockstep(self.algorithms[g0:g0 + size])
+
+    def _lockstep(self, algorithms: Sequence[TDVPAlgorithm]):
+        gens = [a._time_step_gen() for a in algorithms]
         reqs = []
         for g in gens:
             reqs.append(next(g, None))
+        done = 0
         while True:
             if all(r is None for r in reqs):
                 return
@@ -960,8 +977,13 @@ class BatchedTDVP:
                 if not all(isinstance(r, SvdRequest) for r in reqs):
                     raise RuntimeError("batched instances left the common sweep schedule")
                 answer_svd_batched(reqs)
+                kind = "svd"
             else:
-                self._solve(reqs)
+                self._solve(reqs, algorithms)
+                kind = "krylov"
+            done += 1
+            if self.progress is not None:
+                self.progress(done, kind)
             reqs = []
             for g in gens:
                 try:

@@ -195,13 +195,15 @@ def _observables(a):
     return np.array([e, st.norm()] + zs)
 
 
-@pytest.mark.parametrize("two_site,seq_n", [(True, None), (False, None), (True, 200)])
-def test_batched_tdvp_matches_single(gpu, two_site, seq_n):
+@pytest.mark.parametrize("two_site,seq_n,group", [(True, None, None), (False, None, None),
+                                                   (True, 200, None), (True, None, 2)])
+def test_batched_tdvp_matches_single(gpu, two_site, seq_n, group):
     """Lockstep batched TDVP = each instance run alone: observables to 1e-12 and the
     same kept bond dimensions; instances start from two different bond dims (uneven
     shapes, several shape groups).  seq_n: requests with local spaces from this size
-    on are solved one instance at a time, the others batched."""
-    _say(f"batched tdvp two_site={two_site} seq_n={seq_n}")
+    on are solved one instance at a time, the others batched.  group: instances per
+    lockstep group (5 instances in groups of 2, 2 and 1)."""
+    _say(f"batched tdvp two_site={two_site} seq_n={seq_n} group={group}")
     count, steps = 5, 2
     single = _instances(count, 3, [3, 6], 11, two_site)
     batch = _instances(count, 3, [3, 6], 11, two_site)
@@ -211,6 +213,7 @@ def test_batched_tdvp_matches_single(gpu, two_site, seq_n):
     bt = ptn.BatchedTDVP(batch)
     if seq_n is not None:
         bt.SEQUENTIAL_N = seq_n
+    bt.lockstep_group = group
     bt.run(steps)
     for a, b in zip(single, batch):
         oa, ob = _observables(a), _observables(b)

@@ -54,7 +54,11 @@ def test_config4_512_instances_one_batched_step(gpu):
     torch.cuda.synchronize()
     print(f"  built {COUNT} instances in {time.time() - t0:.1f} s", flush=True)
     t0 = time.time()
-    ptn.BatchedTDVP(algs).run(1)
+    bt = ptn.BatchedTDVP(algs)
+    bt.progress = lambda done, kind: (print(f"  {done} lockstep points ({kind}), "
+                                            f"{time.time() - t0:.0f} s", flush=True)
+                                      if done % 25 == 0 else None)
+    bt.run(1)
     torch.cuda.synchronize()
     dt = time.time() - t0
     print(f"  batched step: {dt:.1f} s = {COUNT / dt:.2f} instance-steps/s", flush=True)
