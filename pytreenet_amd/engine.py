@@ -87,8 +87,38 @@ def _offsets(dims: Sequence[int], strides: Sequence[int]) -> np.ndarray:
     return off
 
 
+class _TableSpec:
+    """An offset table given by its modes, built where it is cheapest."""
+    __slots__ = ("dims", "strides", "n")
+
+    def __init__(self, dims: Sequence[int], strides: Sequence[int]):
+        self.dims, self.strides = tuple(dims), tuple(strides)
+        self.n = math.prod(self.dims)
+
+
+def _device_offsets(dims: Sequence[int], strides: Sequence[int], out: torch.Tensor) -> None:
+    """The offset table of the modes written into the int64 device vector ``out``."""
+    dev = out.device
+    off = torch.zeros(1, dtype=torch.int64, device=dev)
+    for i, (d, s) in enumerate(zip(dims, strides)):
+        step = torch.arange(d, dtype=torch.int64, device=dev) * s
+        if i == len(dims) - 1:
+            torch.add(off[:, None], step[None, :], out=out.view(-1, d))
+            return
+        off = (off[:, None] + step[None, :]).reshape(-1)
+    out.copy_(off)
+
+
 class _OffsetCache:
-    """Device-resident offset tables keyed by contraction signature (LRU)."""
+    """Device-resident offset tables keyed by contraction signature (LRU).
+
+    A builder returns host tables (numpy) or ``_TableSpec``s.  Specs of more than
+    DEVICE_BUILD_MIN entries are built on the device (a few elementwise kernels)
+    instead of on the host and copied: the tables of the large two-site networks
+    run to millions of entries, and their signatures change whenever truncation
+    changes a bond dimension."""
+
+    DEVICE_BUILD_MIN = 1 << 16
 
     def __init__(self, capacity: int = 4096):
         self.cap = capacity
@@ -100,13 +130,28 @@ class _OffsetCache:
             self.d.move_to_end(key)
             return hit
         tables = builder()
-        sizes = [len(t) for t in tables]
-        host = np.concatenate([t if len(t) else np.zeros(1, np.int64) for t in tables])
-        dev = torch.from_numpy(host).to(device)
-        ptrs, pos = [], 0
-        for t, n in zip(tables, sizes):
-            ptrs.append(dev.data_ptr() + pos * 8)
-            pos += max(n, 1)
+        on_dev = [isinstance(t, _TableSpec) and t.n > self.DEVICE_BUILD_MIN for t in tables]
+        host = [t if not isinstance(t, _TableSpec) else
+                (None if d else _offsets(t.dims, t.strides)) for t, d in zip(tables, on_dev)]
+        sizes = [t.n if d else len(h) for t, d, h in zip(tables, on_dev, host)]
+        # host tables first (one copy), device-built ones after them
+        order = [i for i in range(len(tables)) if not on_dev[i]] + \
+                [i for i in range(len(tables)) if on_dev[i]]
+        pos, start = 0, [0] * len(tables)
+        for i in order:
+            start[i] = pos
+            pos += max(sizes[i], 1)
+        dev = torch.empty(pos, dtype=torch.int64, device=device)
+        nhost = sum(max(sizes[i], 1) for i in order if not on_dev[i])
+        if nhost:
+            cat = np.concatenate([host[i] if sizes[i] else np.zeros(1, np.int64)
+                                  for i in order if not on_dev[i]])
+            dev[:nhost].copy_(torch.from_numpy(cat))
+        for i in order:
+            if on_dev[i]:
+                _device_offsets(tables[i].dims, tables[i].strides,
+                                dev[start[i]:start[i] + sizes[i]])
+        ptrs = [dev.data_ptr() + start[i] * 8 for i in range(len(tables))]
         val = (dev, ptrs)
         self.d[key] = val
         if len(self.d) > self.cap:
@@ -274,8 +319,9 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     mdims, kdims, ndims = key[0], key[1], key[2]
 
     def build():
-        return [_offsets(mdims, key[3]), _offsets(kdims, key[4]), _offsets(kdims, key[5]),
-                _offsets(ndims, key[6]), _offsets(mdims, key[7]), _offsets(ndims, key[8])]
+        return [_TableSpec(mdims, key[3]), _TableSpec(kdims, key[4]),
+                _TableSpec(kdims, key[5]), _TableSpec(ndims, key[6]),
+                _TableSpec(mdims, key[7]), _TableSpec(ndims, key[8])]
 
     tabs, ptrs = _CACHE.get(key, build, A.device)
     M = math.prod(mdims)

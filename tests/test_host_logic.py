@@ -282,3 +282,16 @@ def test_interchange_files_between_product_and_oracle(tmp_path):
     assert H.nodes["site0"].children == ["site2", "site1"]
     assert H.orthogonality_center_id == "site1"
     assert all(torch.equal(H.tensors[n], torch.as_tensor(Wp[n])) for n in ids)
+
+
+def test_device_built_offset_tables_equal_host_tables():
+    """Large offset tables are built with elementwise tensor ops instead of numpy; the
+    builder runs on any torch device, here the CPU, and must give the host tables."""
+    import torch
+    from pytreenet_amd import engine as E
+    cases = [((4,), (3,)), ((3, 5), (10, 1)), ((2, 3, 4), (1, 100, 7)),
+             ((7, 1, 3), (5, 999, 2)), ((64, 2, 64), (2, 1, 8192))]
+    for dims, strides in cases:
+        out = torch.empty(int(np.prod(dims)), dtype=torch.int64)
+        E._device_offsets(dims, strides, out)
+        assert np.array_equal(out.numpy(), E._offsets(dims, strides)), (dims, strides)
