@@ -146,54 +146,16 @@ __device__ __forceinline__ bool rotate_pair(double2 *a, double2 *b, int64_t mg, 
 // LDS_ELEMS complex128 of LDS hold G and V when mg ng + ng ng fits (128 KB)
 constexpr int64_t LDS_ELEMS = 8192;
 
-template <int TPP, bool IN_LDS>
-__device__ void svd_jacobi_body(const SvdJob &job, double2 *G, double2 *V) {
-  __shared__ int rotated;
+// singular values, their descending order, U and Vh from the converged G and V
+// (one workgroup of SVD_NT threads; groups of TPP lanes reduce the column norms)
+template <int TPP>
+__device__ void svd_finish(const SvdJob &job, const double2 *G, const double2 *V, int64_t mg,
+                           int ng, bool converged) {
   __shared__ double sig[SVD_MAXN];
   __shared__ int rank_of[SVD_MAXN];
-  constexpr int UNR = IN_LDS ? 2 : 4;
   const bool rows = job.on_rows != 0;
-  const int64_t mg = rows ? job.n : job.m;  // rows of G
-  const int ng = (int)(rows ? job.m : job.n);  // columns of G
   const int tid = threadIdx.x;
-  // G = op(A), V = I
-  for (int64_t e = tid; e < mg * ng; e += SVD_NT) {
-    const int64_t i = e % mg, j = e / mg;
-    if (!rows) {
-      G[e] = job.A[i + j * job.lda];
-    } else {
-      const double2 a = job.A[j + i * job.lda];  // G(i, j) = conj(A(j, i))
-      G[e] = make_double2(a.x, -a.y);
-    }
-  }
-  for (int e = tid; e < ng * ng; e += SVD_NT)
-    V[e] = make_double2((e % ng) == (e / ng) ? 1.0 : 0.0, 0.0);
-  __syncthreads();
-
-  const int N = ng + (ng & 1);
-  const int npairs = N / 2;
   const int grp = tid / TPP, gl = tid % TPP;
-  const double tol = sqrt((double)mg) * 2.220446049250313e-16;
-  int sweep = 0;
-  bool converged = ng < 2;
-  for (; sweep < SVD_MAX_SWEEPS && !converged; ++sweep) {
-    if (tid == 0) rotated = 0;
-    __syncthreads();
-    for (int r = 0; r < N - 1; ++r) {
-      if (grp < npairs) {
-        int p, q;
-        rr_pair(N, r, grp, p, q);
-        if (q < ng &&
-            rotate_pair<TPP, UNR>(G + (int64_t)p * mg, G + (int64_t)q * mg, mg,
-                                  V + (int64_t)p * ng, V + (int64_t)q * ng, ng, gl, tol) &&
-            gl == 0)
-          rotated = 1;
-      }
-      __syncthreads();
-    }
-    converged = rotated == 0;
-    __syncthreads();  // every thread has read `rotated` before the next reset
-  }
   // singular values: group g reduces the norms of columns 2g and 2g + 1
   for (int cbase = 2 * grp; cbase < ng; cbase += 2 * (SVD_NT / TPP)) {
     for (int h = 0; h < 2; ++h) {
@@ -238,6 +200,55 @@ __device__ void svd_jacobi_body(const SvdJob &job, double2 *G, double2 *V) {
   if (tid == 0) *job.info = converged ? 0 : 1;
 }
 
+template <int TPP, bool IN_LDS>
+__device__ void svd_jacobi_body(const SvdJob &job, double2 *G, double2 *V) {
+  __shared__ int rotated;
+  constexpr int UNR = IN_LDS ? 2 : 4;
+  const bool rows = job.on_rows != 0;
+  const int64_t mg = rows ? job.n : job.m;  // rows of G
+  const int ng = (int)(rows ? job.m : job.n);  // columns of G
+  const int tid = threadIdx.x;
+  // G = op(A), V = I
+  for (int64_t e = tid; e < mg * ng; e += SVD_NT) {
+    const int64_t i = e % mg, j = e / mg;
+    if (!rows) {
+      G[e] = job.A[i + j * job.lda];
+    } else {
+      const double2 a = job.A[j + i * job.lda];  // G(i, j) = conj(A(j, i))
+      G[e] = make_double2(a.x, -a.y);
+    }
+  }
+  for (int e = tid; e < ng * ng; e += SVD_NT)
+    V[e] = make_double2((e % ng) == (e / ng) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+
+  const int N = ng + (ng & 1);
+  const int npairs = N / 2;
+  const int grp = tid / TPP, gl = tid % TPP;
+  const double tol = sqrt((double)mg) * 2.220446049250313e-16;
+  int sweep = 0;
+  bool converged = ng < 2;
+  for (; sweep < SVD_MAX_SWEEPS && !converged; ++sweep) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < N - 1; ++r) {
+      if (grp < npairs) {
+        int p, q;
+        rr_pair(N, r, grp, p, q);
+        if (q < ng &&
+            rotate_pair<TPP, UNR>(G + (int64_t)p * mg, G + (int64_t)q * mg, mg,
+                                  V + (int64_t)p * ng, V + (int64_t)q * ng, ng, gl, tol) &&
+            gl == 0)
+          rotated = 1;
+      }
+      __syncthreads();
+    }
+    converged = rotated == 0;
+    __syncthreads();  // every thread has read `rotated` before the next reset
+  }
+  svd_finish<TPP>(job, G, V, mg, ng, converged);
+}
+
 template <int TPP>
 __device__ __forceinline__ void svd_jacobi_dispatch(const SvdJob &job) {
   __shared__ double2 lds[LDS_ELEMS];
@@ -258,6 +269,99 @@ template <int TPP>
 __global__ __launch_bounds__(SVD_NT) void svd_jacobi_many(const SvdJob *__restrict__ jobs) {
   const SvdJob job = jobs[blockIdx.x];
   svd_jacobi_dispatch<TPP>(job);
+}
+
+// ---- wide form: one matrix over many workgroups ------------------------------------
+// For a single matrix with many columns the one-workgroup kernel is bound by the L2
+// bandwidth of one CU (a 256-column triangle: ~17 ms per sweep).  The wide form runs
+// the same rotations in the same round-robin order with one launch per round: one
+// wave per pair, npairs workgroups, so a round takes a few microseconds on the
+// whole chip.  G, V and one "rotated" flag per sweep live in the workspace; the
+// rounds of sweep s return at once when sweep s - 1 rotated nothing.
+constexpr int WIDE_MIN_COLS = 65;   // columns from which a single matrix runs wide
+constexpr int WIDE_FIRST_SWEEPS = 8;
+constexpr int WIDE_MORE_SWEEPS = 2;
+
+__device__ __forceinline__ int *wide_flags(const SvdJob &job, int64_t mg, int64_t ng) {
+  return reinterpret_cast<int *>(job.ws + mg * ng + ng * ng);
+}
+
+__global__ __launch_bounds__(256) void svd_wide_init(SvdJob job) {
+  const bool rows = job.on_rows != 0;
+  const int64_t mg = rows ? job.n : job.m, ng = rows ? job.m : job.n;
+  double2 *G = job.ws, *V = job.ws + mg * ng;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t e = t0; e < mg * ng; e += stride) {
+    const int64_t i = e % mg, j = e / mg;
+    if (!rows) {
+      G[e] = job.A[i + j * job.lda];
+    } else {
+      const double2 a = job.A[j + i * job.lda];  // G(i, j) = conj(A(j, i))
+      G[e] = make_double2(a.x, -a.y);
+    }
+  }
+  for (int64_t e = t0; e < ng * ng; e += stride)
+    V[e] = make_double2((e % ng) == (e / ng) ? 1.0 : 0.0, 0.0);
+  if (t0 < SVD_MAX_SWEEPS) wide_flags(job, mg, ng)[t0] = 0;
+}
+
+__global__ __launch_bounds__(64) void svd_wide_round(SvdJob job, int r, int sweep) {
+  const bool rows = job.on_rows != 0;
+  const int64_t mg = rows ? job.n : job.m;
+  const int ng = (int)(rows ? job.m : job.n);
+  int *flags = wide_flags(job, mg, ng);
+  if (sweep > 0 && flags[sweep - 1] == 0) return;   // converged before this sweep
+  const int N = ng + (ng & 1);
+  int p, q;
+  rr_pair(N, r, (int)blockIdx.x, p, q);
+  if (q >= ng) return;
+  double2 *G = job.ws, *V = job.ws + mg * ng;
+  const double tol = sqrt((double)mg) * 2.220446049250313e-16;
+  const bool rot = rotate_pair<64, 4>(G + (int64_t)p * mg, G + (int64_t)q * mg, mg,
+                                      V + (int64_t)p * ng, V + (int64_t)q * ng, ng,
+                                      (int)threadIdx.x, tol);
+  if (rot && threadIdx.x == 0) flags[sweep] = 1;
+}
+
+__global__ __launch_bounds__(SVD_NT) void svd_wide_finish(SvdJob job, int converged) {
+  const bool rows = job.on_rows != 0;
+  const int64_t mg = rows ? job.n : job.m;
+  const int ng = (int)(rows ? job.m : job.n);
+  svd_finish<8>(job, job.ws, job.ws + mg * ng, mg, ng, converged != 0);
+}
+
+// The wide solve: init, sweeps in chunks (the flags of a chunk read back once, the
+// host waits on the stream), finish.
+int svd_wide(hipStream_t s, const SvdJob &j) {
+  const int64_t mg = j.on_rows ? j.n : j.m, ng = j.on_rows ? j.m : j.n;
+  const int N = (int)(ng + (ng & 1));
+  const int npairs = N / 2;
+  hipLaunchKernelGGL(svd_wide_init, dim3(256), dim3(256), 0, s, j);
+  PTN_LAUNCH_CHECK();
+  int done = 0, converged = 0;
+  int flags[SVD_MAX_SWEEPS];
+  while (done < SVD_MAX_SWEEPS && !converged) {
+    const int chunk = done == 0 ? WIDE_FIRST_SWEEPS : WIDE_MORE_SWEEPS;
+    const int upto = done + chunk < SVD_MAX_SWEEPS ? done + chunk : SVD_MAX_SWEEPS;
+    for (int sw = done; sw < upto; ++sw)
+      for (int r = 0; r < N - 1; ++r)
+        hipLaunchKernelGGL(svd_wide_round, dim3(npairs), dim3(64), 0, s, j, r, sw);
+    PTN_LAUNCH_CHECK();
+    const int *dflags = reinterpret_cast<const int *>(j.ws + mg * ng + ng * ng);
+    if (hipMemcpyAsync(flags, dflags, sizeof(int) * upto, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      ptn::set_error("ptn_svd_jacobi: reading the sweep flags failed");
+      return PTN_EHIP;
+    }
+    for (int sw = done; sw < upto; ++sw)
+      if (flags[sw] == 0) converged = 1;
+    done = upto;
+  }
+  hipLaunchKernelGGL(svd_wide_finish, dim3(1), dim3(SVD_NT), 0, s, j, converged);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
 }
 
 int tpp_for(int ng) {
@@ -294,7 +398,8 @@ int check_job(const SvdJob &j) {
 
 extern "C" int64_t ptn_svd_jacobi_work_bytes(int64_t m, int64_t n) {
   const int64_t k = m < n ? m : n;
-  return (int64_t)sizeof(double2) * (m * n + k * k);
+  // G and V, then the wide form's per-sweep flags (256 bytes)
+  return (int64_t)sizeof(double2) * (m * n + k * k) + 256;
 }
 
 extern "C" int ptn_svd_jacobi(void *stream, int64_t m, int64_t n, const void *A, int64_t lda,
@@ -315,6 +420,7 @@ extern "C" int ptn_svd_jacobi(void *stream, int64_t m, int64_t n, const void *A,
   const int rc = check_job(j);
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((on_rows ? m : n) >= WIDE_MIN_COLS) return svd_wide(s, j);
   switch (tpp_for((int)(on_rows ? m : n))) {
     case 64: hipLaunchKernelGGL(svd_jacobi_one<64>, dim3(1), dim3(SVD_NT), 0, s, j); break;
     case 32: hipLaunchKernelGGL(svd_jacobi_one<32>, dim3(1), dim3(SVD_NT), 0, s, j); break;
