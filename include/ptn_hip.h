@@ -304,8 +304,11 @@ int ptn_qr_split_small(void *stream, int32_t rank, const int64_t *dims, const in
 int ptn_svd(void *stream, int64_t m, int64_t n, void *A, void *S, void *U,
             void *Vh, void *E);
 
-/* Thin SVD of a small m x n complex128 matrix by one-sided Jacobi (Hestenes) in
- * one workgroup: A (column-major, leading dimension lda; not modified) =
+/* Thin SVD of a small m x n complex128 matrix by one-sided Jacobi (Hestenes):
+ * one workgroup up to 64 rotated columns; from 65 columns one launch per
+ * rotation round over the whole chip, and then the call waits on the stream
+ * (it reads the per-sweep convergence flags back) before it returns.
+ * A (column-major, leading dimension lda; not modified) =
  * U diag(S) Vh with k = min(m, n), S (double, k, descending), U (m x k,
  * column-major, ld m), Vh (k x n, column-major, ld k).  on_rows = 0 rotates the
  * columns of A (needs m >= n), on_rows = 1 the columns of A^H (needs m <= n; for
