@@ -361,8 +361,10 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             self.precision = E.FP64
         self.apply_log: List[tuple] = []   # (kind, node or edge, Krylov applies)
         self._last_applies = 0
-        # dense TTNO tensors in the site networks (the batched replay needs them)
-        self.dense_site_operator = False
+        # dense TTNO tensors in the site networks of local spaces below this many
+        # elements (a BatchedTDVP sets it: its grouped replay needs dense operators;
+        # the spaces it solves one instance at a time keep the block-sparse ones)
+        self.dense_site_below = 0
         self.update_path = self._finds_update_path()
         self.orthogonalization_path = find_orthogonalisation_path(self.update_path, self.state)
         self._orthogonalize_init()
@@ -491,9 +493,9 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
     def _update_site_gen(self, node_id: str, time_step_factor: float = 1):
         node, tensor = self.state[node_id]
         op_node = self.hamiltonian.nodes[node_id]
+        dense = tensor.numel() < self.dense_site_below
         op = operator_tensor(self.hamiltonian, node_id, self.precision,
-                             block=False if self.dense_site_operator else None,
-                             size=tensor.numel())
+                             block=False if dense else None, size=tensor.numel())
         cache = self.partial_tree_cache
         envs = [cache.get_entry(nb, node_id) for nb in node.neighbouring_nodes()]
         tensors, labels, out_l = heff_network(node, op_node, op, envs)
@@ -503,8 +505,7 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         if getattr(self, "relaxed", False) and \
                 tensor.numel() >= getattr(self.config, "relax_min_size", 0):
             op32 = operator_tensor(self.hamiltonian, node_id, E.FP32,
-                                   block=False if self.dense_site_operator else None,
-                                   size=tensor.numel())
+                                   block=False if dense else None, size=tensor.numel())
             t32, l32, o32 = heff_network(node, op_node, op32,
                                          [e.to(torch.complex64) for e in envs])
             t32[0] = tensor
@@ -855,8 +856,6 @@ class BatchedTDVP:
                                  "step, precision and solver options")
         if len({a.relaxed for a in algs}) > 1:
             raise ValueError("batched instances must share the precision")
-        for a in algs:
-            a.dense_site_operator = True
         self.algorithms = algs
         self.krylov_dims: List[int] = []
         self.memory_fraction = 0.5   # share of free device memory a batched solve may use
@@ -958,6 +957,8 @@ class BatchedTDVP:
 
     def run_one_time_step(self):
         E.refresh_stream()
+        for a in self.algorithms:
+            a.dense_site_below = self.SEQUENTIAL_N
         size = self.lockstep_group or len(self.algorithms)
         for g0 in range(0, len(self.algorithms), size):
             self._lockstep(self.algorithms[g0:g0 + size])
