@@ -97,10 +97,12 @@ int ptn_gemm_gather(void *stream, const ptn_gemm_desc *desc);
  *                  1x1 2x1 1x2 2x2 4x1 1x4 1x3 3x1 1x5 (K split over waves by
  *                  the library's rule when desc->ws allows);
  *   PTN_GEMM_TILE  the 64 x 64 workgroup kernel with K cut into `ksplit`
- *                  ranges (1..1024; > 1 needs ksplit*M*N*16 workspace bytes).
+ *                  ranges (1..1024; > 1 needs ksplit*M*N*16 workspace bytes);
+ *   PTN_GEMM_SKINNY the row-streaming kernel for K <= 16 and N <= 64 (FP64
+ *                  vector arithmetic, B in LDS; the rule takes it from M >= 8192).
  * Every configuration computes the same result up to rounding; PTN_EINVAL for
  * a configuration that does not exist or does not fit the workspace.       */
-enum { PTN_GEMM_RULE = 0, PTN_GEMM_WAVE = 1, PTN_GEMM_TILE = 2 };
+enum { PTN_GEMM_RULE = 0, PTN_GEMM_WAVE = 1, PTN_GEMM_TILE = 2, PTN_GEMM_SKINNY = 3 };
 int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *desc, int32_t kind,
                            int32_t bmb, int32_t bnb, int32_t ksplit);
 

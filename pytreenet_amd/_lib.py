@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libptnhip.so"
 
 PTN_C128, PTN_C64 = 0, 1
 PTN_COMPUTE_F64_4M, PTN_COMPUTE_F32_3M, PTN_COMPUTE_BF16_3M, PTN_COMPUTE_BF16X6_3M = 0, 1, 2, 3
-PTN_GEMM_RULE, PTN_GEMM_WAVE, PTN_GEMM_TILE = 0, 1, 2
+PTN_GEMM_RULE, PTN_GEMM_WAVE, PTN_GEMM_TILE, PTN_GEMM_SKINNY = 0, 1, 2, 3
 PTN_OK, PTN_EINVAL, PTN_EHIP, PTN_ESOLVER, PTN_ENOCONV, PTN_ERANGE = 0, -1, -2, -3, -4, -5
 
 EXPORTED = (

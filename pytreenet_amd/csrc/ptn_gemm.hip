@@ -1326,6 +1326,99 @@ void launch_wave_typed(hipStream_t s, const KArgs &a, int bmb, int bnb, dim3 g, 
 
 // the operand dtypes the library's contractions use: all complex128 (FP64), or
 // complex64 with at most one complex128 side (the mixed form)
+// ---- skinny products: short K, narrow N, long M -----------------------------------
+// C (M x N) = A (M x K) B (K x N) with K <= SK_MAXK and N <= SK_MAXN: a stream over
+// the rows of A and C with B in LDS.  Each thread owns rows: it loads the row's K
+// values of A into registers and writes its N outputs, with FP64 vector FMAs (the
+// product has 4 K N flops per 16 (K + N) bytes of a row, far below the MFMA's
+// balance).  The MFMA kernels spend most of a 16 x 16 (or 64 x 64) block on
+// padding and prologue here: a 10-deep K runs in 12 (or 16) and every block reloads
+// its offsets.  Rows are dealt out grid-stride; consecutive threads take
+// consecutive rows, so an m-contiguous A or C is read / written in full lines.
+constexpr int SK_MAXK = 16;
+constexpr int SK_MAXN = 64;
+constexpr int SK_NT = 256;
+constexpr int64_t SK_MIN_M = 8192;   // rows from which the rule takes the skinny kernel
+                                     // (with A and C m-contiguous)
+
+template <typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(SK_NT) void skinny_gemm(KArgs p) {
+  __shared__ double2 sB[SK_MAXK][SK_MAXN];
+  __shared__ int64_t s_aok[SK_MAXK], s_con[SK_MAXN];
+  const int K = (int)p.K, N = (int)p.N;
+  const TA *A = static_cast<const TA *>(p.A);
+  const TB *B = static_cast<const TB *>(p.B);
+  TC *C = static_cast<TC *>(p.C);
+  const double asg = p.aconj ? -1.0 : 1.0, bsg = p.bconj ? -1.0 : 1.0;
+  for (int e = threadIdx.x; e < K * N; e += SK_NT) {
+    const int k = e / N, n = e - k * N;
+    double vr, vi;
+    ptn::cload(B + p.bok[k] + p.bon[n], vr, vi);
+    sB[k][n] = make_double2(vr, bsg * vi);
+  }
+  for (int k = threadIdx.x; k < K; k += SK_NT) s_aok[k] = p.aok[k];
+  for (int n = threadIdx.x; n < N; n += SK_NT) s_con[n] = p.con[n];
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * SK_NT;
+  for (int64_t m = (int64_t)blockIdx.x * SK_NT + threadIdx.x; m < p.M; m += stride) {
+    const int64_t ao = p.aom[m], co = p.com[m];
+    double ar[SK_MAXK], ai[SK_MAXK];
+#pragma unroll
+    for (int k = 0; k < SK_MAXK; ++k) {
+      double vr = 0.0, vi = 0.0;
+      if (k < K) ptn::cload(A + ao + s_aok[k], vr, vi);
+      ar[k] = vr;
+      ai[k] = asg * vi;
+    }
+    for (int n = 0; n < N; ++n) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int k = 0; k < SK_MAXK; ++k) {
+        if (k < K) {
+          const double2 b = sB[k][n];
+          re = fma(ar[k], b.x, re);
+          re = fma(-ai[k], b.y, re);
+          im = fma(ar[k], b.y, im);
+          im = fma(ai[k], b.x, im);
+        }
+      }
+      double ore = p.are * re - p.aim * im;
+      double oim = p.are * im + p.aim * re;
+      TC *dst = C + co + s_con[n];
+      if (p.beta_one) {
+        double xr, xi;
+        ptn::cload(dst, xr, xi);
+        ore += xr;
+        oim += xi;
+      }
+      ptn::cstore(dst, ore, oim);
+    }
+  }
+}
+
+bool skinny_ok(int64_t M, int64_t N, int64_t K) {
+  return K >= 1 && K <= SK_MAXK && N >= 1 && N <= SK_MAXN && M >= 1;
+}
+
+int launch_skinny(hipStream_t s, const KArgs &a, int adt, int bdt, int cdt) {
+  int64_t blocks = (a.M + SK_NT - 1) / SK_NT;
+  if (blocks > 256 * 16) blocks = 256 * 16;   // grid-stride beyond 16 workgroups per CU
+  const dim3 g((unsigned)blocks), b(SK_NT);
+  const int code = adt * 4 + bdt * 2 + cdt;  // 0: complex128, 1: complex64
+  switch (code) {
+    case 0: hipLaunchKernelGGL((skinny_gemm<double2, double2, double2>), g, b, 0, s, a); break;
+    case 7: hipLaunchKernelGGL((skinny_gemm<float2, float2, float2>), g, b, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((skinny_gemm<float2, float2, double2>), g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((skinny_gemm<double2, float2, float2>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((skinny_gemm<float2, double2, float2>), g, b, 0, s, a); break;
+    default:
+      ptn::set_error("ptn_gemm_gather: operand dtype combination %d not supported", code);
+      return PTN_EINVAL;
+  }
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
 int launch_wave(hipStream_t s, KArgs a, int bmb, int bnb, int64_t ws_bytes, bool c_mmajor,
                 int adt, int bdt, int cdt) {
   const int64_t tn = (a.N + 16 * bnb - 1) / (16 * bnb);
@@ -1550,6 +1643,11 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   // latency, where the wider arithmetic costs nothing
   const int code = d->a_dtype * 4 + d->b_dtype * 2 + d->c_dtype;
   const bool wave_types = code == 0 || code == 7 || code == 6 || code == 3 || code == 5;
+  // (rows coalesce only when A and C are m-contiguous: consecutive threads take
+  // consecutive rows)
+  if (wave_types && d->M >= SK_MIN_M && !d->a_kmajor && d->c_mmajor &&
+      skinny_ok(d->M, d->N, d->K))
+    return launch_skinny(s, a, d->a_dtype, d->b_dtype, d->c_dtype);
   if (wave_types && wave_shape(d->M, d->N, d->K, bmb, bnb)) {
     a.ws = static_cast<double2 *>(d->ws);
     return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0, d->a_dtype,
@@ -1629,6 +1727,16 @@ extern "C" int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *d, int3
     a.ws = static_cast<double2 *>(d->ws);
     return launch_wave(s, a, bmb, bnb, d->ws ? d->ws_bytes : 0, d->c_mmajor != 0, d->a_dtype,
                        d->b_dtype, d->c_dtype);
+  }
+  if (kind == PTN_GEMM_SKINNY) {
+    const int code = d->a_dtype * 4 + d->b_dtype * 2 + d->c_dtype;
+    const bool types = code == 0 || code == 7 || code == 6 || code == 3 || code == 5;
+    if (!types || !skinny_ok(d->M, d->N, d->K)) {
+      ptn::set_error("ptn_gemm_gather_config: no skinny kernel for %lld x %lld x %lld",
+                     (long long)d->M, (long long)d->N, (long long)d->K);
+      return PTN_EINVAL;
+    }
+    return launch_skinny(s, a, d->a_dtype, d->b_dtype, d->c_dtype);
   }
   if (kind != PTN_GEMM_TILE || ksplit < 1 || ksplit > 1024) {
     ptn::set_error("ptn_gemm_gather_config: bad kind %d / ksplit %d", kind, ksplit);

@@ -480,12 +480,22 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     _run_gemm(d, M, N, K, precision)
 
 
-def gemm_kernel_class(M: int, N: int, K: int, precision: str) -> str:
+# the row-streaming kernel's limits and the rule's threshold (``ptn_gemm.hip``)
+SKINNY_MAX_K, SKINNY_MAX_N, SKINNY_MIN_M = 16, 64, 8192
+
+
+def gemm_kernel_class(M: int, N: int, K: int, precision: str, a_kmajor: bool = True,
+                      c_mmajor: bool = False) -> str:
     """The kernel ``ptn_gemm_gather`` runs a launch of this shape on (its dispatch
-    rule): "wave" (M or N within one 16-block, or both within two, or, for
+    rule, given the descriptor's layout hints ``a_kmajor`` and ``c_mmajor``): "skinny"
+    (K <= 16, N <= 64, M >= 8192, A and C m-contiguous: rows streamed with B in LDS),
+    "wave" (M or N within one 16-block, or both within two, or, for
     K <= 256, N within three or five 16-blocks below a longer M, or M within three
     below a longer N; either compute mode), "large" (a tile grid of >= 2048 64 x 64 tiles: the LARGE instantiation,
     never split-K) or "small" (the same kernel on a smaller grid, split-K allowed)."""
+    if M >= SKINNY_MIN_M and 1 <= K <= SKINNY_MAX_K and 1 <= N <= SKINNY_MAX_N and \
+            not a_kmajor and c_mmajor:
+        return "skinny"
     bm, bn = -(-M // 16), -(-N // 16)
     if (bm <= 2 and bn <= 2) or bm == 1 or bn == 1:
         return "wave"
@@ -521,6 +531,8 @@ def tune_candidates(d: GemmDesc, M: int, N: int, K: int) -> List[Tuple[int, int,
     """(kind, bmb, bnb, ksplit) configurations worth timing for one launch."""
     cands = [(_lib.PTN_GEMM_RULE, 0, 0, 0)]
     if d.a_dtype * 4 + d.b_dtype * 2 + d.c_dtype in _WAVE_DTYPES:
+        if 1 <= K <= SKINNY_MAX_K and 1 <= N <= SKINNY_MAX_N:
+            cands.append((_lib.PTN_GEMM_SKINNY, 0, 0, 0))
         for bm, bn in _WAVE_BLOCKS:
             if -(-M // (16 * bm)) * -(-N // (16 * bn)) <= 1 << 15:
                 cands.append((_lib.PTN_GEMM_WAVE, bm, bn, 0))
@@ -561,7 +573,9 @@ def _tuned_launch(lib, d: GemmDesc, M: int, N: int, K: int, precision: str) -> N
 
 def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
     lib = _lib.load()
-    if AUTOTUNE and gemm_kernel_class(M, N, K, precision) != "large":
+    cls = gemm_kernel_class(M, N, K, precision, bool(d.a_kmajor), bool(d.c_mmajor)) \
+        if AUTOTUNE or PROFILER.active else None
+    if AUTOTUNE and cls != "large":
         if PROFILER.active:
             PROFILER.flops += 8.0 * M * N * K
             PROFILER.launches += 1
@@ -572,7 +586,7 @@ def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
         fl = 8.0 * M * N * K
         prof.flops += fl
         prof.launches += 1
-        if gemm_kernel_class(M, N, K, precision) == "large":
+        if cls == "large":
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
