@@ -20,7 +20,7 @@ vectors) are rounded while being staged.
 """
 from __future__ import annotations
 
-from typing import Callable, Dict, Iterator, List, Optional, Tuple
+from typing import Callable, Dict, Iterator, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -96,16 +96,44 @@ def operator_tensor(operator: TreeTensorNetworkOperator, node_id: str, precision
     return operator.tensor_in(node_id, dt)
 
 
+# Environment blocks in which the operator on the environment's side is the
+# identity (``TTNO.identity_states``) equal the identity matrix when that side of
+# the state is an isometry towards the node the block points to.  The time
+# evolution keeps that canonical form, so its effective Hamiltonians and
+# environment updates skip those blocks (``engine._split_identity``) for site
+# tensors of at least IDENTITY_MIN_SIZE elements (the ones that meet the
+# block-sparse operator tensor).
+IDENTITY_BLOCKS = True
+IDENTITY_MIN_SIZE = 1 << 16
+
+
+def identity_slots(node_id: str, neighbours: Sequence[Optional[str]],
+                   operator: TreeTensorNetworkOperator, first_slot: int
+                   ) -> Dict[int, Tuple[int, ...]]:
+    """{network slot: identity states} of the environment blocks E[(nb, node_id)]
+    sitting at slots first_slot, first_slot + 1, ... (None: no block there)."""
+    out = {}
+    slot = first_slot
+    for nb in neighbours:
+        if nb is None:
+            continue
+        out[slot] = operator.identity_states(nb, node_id)
+        slot += 1
+    return out
+
+
 def contract_any(node_id: str, next_node_id: str, state: TreeTensorNetworkState,
                  operator: TreeTensorNetworkOperator, dictionary: PartialTreeCachDict,
                  bra_state: Optional[TreeTensorNetworkState] = None,
                  bra_state_conjugated: bool = True, id_trafo_op=None, id_trafo_bra=None,
-                 precision: str = E.FP64) -> torch.Tensor:
+                 precision: str = E.FP64, canonical: bool = False) -> torch.Tensor:
     """Directed environment of the subtree at ``node_id`` with open legs to ``next_node_id``.
 
     ``next_node_id`` may be a node that is not a tree neighbour (the link tensor
     of a one-site update): then the block points along the leg that
-    ``dictionary`` does not hold for ``node_id``.
+    ``dictionary`` does not hold for ``node_id``.  ``canonical`` states that the
+    subtrees behind the blocks used are isometries towards ``node_id`` (and the
+    bra is the ket), so their identity blocks are skipped.
     """
     ket_node, ket = state[node_id]
     op_node = operator.nodes[node_id]
@@ -117,17 +145,22 @@ def contract_any(node_id: str, next_node_id: str, state: TreeTensorNetworkState,
         if len(missing) != 1:
             raise KeyError(f"cannot infer the open leg of {node_id} towards {next_node_id}")
         to = missing[0]
+    ident = None
+    if canonical and bra_state is None and IDENTITY_BLOCKS and ket.numel() >= IDENTITY_MIN_SIZE:
+        ident = identity_slots(node_id, [nb if i != to else None for i, nb in enumerate(nbs)],
+                               operator, 3)
     return env_block(ket_node, ket, op_node,
                      operator_tensor(operator, node_id, precision, size=ket.numel()),
                      [dictionary.get((nb, node_id)) if i != to else None
                       for i, nb in enumerate(nbs)], to,
                      bra=None if bra_state is None else bra_state.tensors[node_id],
-                     bra_conj=bra_state_conjugated, precision=precision)
+                     bra_conj=bra_state_conjugated, precision=precision, identity=ident)
 
 
 def env_block(ket_node: Node, ket: torch.Tensor, op_node: Node, op: torch.Tensor,
               envs: List[Optional[torch.Tensor]], to: int, bra: Optional[torch.Tensor] = None,
-              bra_conj: bool = True, precision: str = E.FP64) -> torch.Tensor:
+              bra_conj: bool = True, precision: str = E.FP64,
+              identity: Optional[Dict[int, Tuple[int, ...]]] = None) -> torch.Tensor:
     k = ket_node.nneighbours()
     a_lab = [("a", i) for i in range(k)] + ["p"]
     b_lab = [("b", i) for i in range(k)] + ["q"]
@@ -141,7 +174,8 @@ def env_block(ket_node: Node, ket: torch.Tensor, op_node: Node, op: torch.Tensor
         labels.append(_env_labels(i))
         conj.append(False)
     return E.contract_network_cached(tensors, labels, _env_labels(to), conj=conj,
-                                     precision=precision, out_dtype=E.storage_dtype(precision))
+                                     precision=precision, out_dtype=E.storage_dtype(precision),
+                                     identity=identity)
 
 
 def heff_network(ket_node: Node, op_node: Node, op: torch.Tensor,
@@ -212,17 +246,24 @@ class SandwichCache(PartialTreeCachDict):
     @classmethod
     def init_cache_but_one(cls, state: TreeTensorNetworkState,
                            hamiltonian: TreeTensorNetworkOperator, node_id: str,
-                           bra_state=None, precision: str = E.FP64) -> "SandwichCache":
+                           bra_state=None, precision: str = E.FP64,
+                           canonical: bool = False) -> "SandwichCache":
         """All directed blocks pointing towards ``node_id``, leaves first."""
         cache = cls(state, hamiltonian, bra_state=bra_state, precision=precision)
+        cache.canonical = canonical
         for nid, to in caching_path(state, node_id):
             cache.update_tree_cache(nid, to)
         return cache
 
+    # set by the time evolution, which keeps the state canonical around the
+    # blocks it builds (``contract_any(canonical=...)``)
+    canonical = False
+
     def update_tree_cache(self, node_id: str, next_node_id: str):
         self.add_entry(node_id, next_node_id,
                        contract_any(node_id, next_node_id, self.state, self.hamiltonian, self,
-                                    bra_state=self.bra_state, precision=self.precision))
+                                    bra_state=self.bra_state, precision=self.precision,
+                                    canonical=self.canonical))
 
 
 def caching_path(state: TreeTensorNetwork, target: str) -> List[Tuple[str, str]]:

@@ -1043,15 +1043,218 @@ def plan_network(labels: Sequence[Sequence[Label]], dims: Dict[Label, int],
     return plan
 
 
+# Identity blocks of environments (``contract_network(identity=...)``): when set,
+# every identity block is checked against the identity matrix on the device
+# (one host synchronisation per split network) and a deviation above this bound
+# raises.  Off in the sweeps; the tests turn it on.
+CHECK_IDENTITY = False
+IDENTITY_CHECK_TOL = 1e-10
+
+
+class _SplitPart:
+    """One term group of a network split at its identity environment blocks."""
+
+    def __init__(self, tensors, labels, out_labels, conj, fixed, full):
+        self.tensors = tensors
+        self.labels = labels
+        self.out_labels = out_labels
+        self.conj = conj
+        self.fixed = fixed        # [(output axis, index)] the part writes a slice at
+        self.full = full          # writes the whole output
+
+
+def _split_identity(tensors, labels, out_labels, conj, identity) -> Optional[List[_SplitPart]]:
+    """The network as a sum of smaller networks, using environment blocks that are
+    identity matrices.
+
+    ``identity[i]`` lists the states s of tensor i's middle (bond-state) leg for
+    which ``tensors[i][:, s, :]`` is the identity (legs (ket, ham, bra), ket and
+    bra of equal dimension).  The network's block-sparse operator tensor W holds
+    live bond-state tuples t; the network is the sum over t of the product of
+    W's block t with every environment's block t_i.  Tuples are grouped by the
+    environments whose block is not an identity:
+
+    * no such environment: the identities join the ket and bra legs directly
+      (an environment update becomes the Gram matrix of the state tensor, an
+      effective Hamiltonian the operator on the physical leg alone);
+    * one environment i: that environment, restricted to the states the group
+      uses, meets the group's operator blocks W'(s_i, open bond legs, out, in)
+      as a dense three-tensor network (a single contraction over (leg i, s_i)
+      for the large tensors);
+    * two or more: the original network restricted to those tuples.
+
+    Open bond-state legs (the new environment's middle leg) keep their full range
+    in W', or are fixed when the group uses one value of them, in which case the
+    group writes that slice of the output.  Returns None when nothing can be
+    split (no block-sparse W, an environment leg of W without identity data, or
+    every tuple in the last group)."""
+    n = len(tensors)
+    bi = [i for i, t in enumerate(tensors) if isinstance(t, BlockSparse)]
+    if len(bi) != 1:
+        return None
+    bi = bi[0]
+    W = tensors[bi]
+    wl = list(labels[bi])
+    blabels = [wl[a] for a in W.block_axes]
+    env_col: Dict[int, int] = {}
+    for i, states in identity.items():
+        ket, ham, bra = labels[i]
+        if ham not in blabels or tensors[i].shape[0] != tensors[i].shape[2]:
+            return None
+        env_col[i] = blabels.index(ham)
+    open_cols = []
+    for c, l in enumerate(blabels):
+        owners = [i for i in range(n) if i != bi and l in labels[i]]
+        if not owners:
+            if l not in out_labels:
+                return None
+            open_cols.append(c)
+        elif not (len(owners) == 1 and env_col.get(owners[0]) == c):
+            return None
+    key = (tuple(sorted((env_col[i], tuple(identity[i])) for i in env_col)),
+           tuple(open_cols))
+    cache = W.__dict__.setdefault("_split_cache", {})
+    plan = cache.get(key)
+    if plan is None:
+        plan = cache[key] = _split_plan(W, env_col, identity, open_cols, blabels)
+    if plan is False:
+        return None
+    dlabels = [l for a, l in enumerate(wl) if a not in W.block_axes]
+    parts: List[_SplitPart] = []
+    for g in plan:
+        if g["residual"] is not None:
+            ts = list(tensors)
+            ts[bi] = g["residual"]
+            parts.append(_SplitPart(ts, [list(l) for l in labels], list(out_labels),
+                                    list(conj), [], True))
+            continue
+        ni = g["ni"]
+        rename = {labels[j][2]: labels[j][0] for j in env_col if j not in ni}
+        ts, ls, cs = [], [], []
+        for j in range(n):
+            if j == bi or (j in env_col and j not in ni):
+                continue
+            t = tensors[j]
+            if j in ni:
+                # the states from the group's first to its last (a view, so a
+                # recorded replay reads the caller's tensor); W' is zero on the
+                # states in between that the group does not use
+                S = g["states"]
+                t = t.narrow(1, S[0], S[-1] + 1 - S[0])
+            ts.append(t)
+            ls.append([rename.get(l, l) for l in labels[j]])
+            cs.append(conj[j])
+        w_labels = ([labels[tuple(ni)[0]][1]] if ni else []) + \
+            [blabels[c] for c in g["kept"]] + dlabels
+        ts.append(g["w"])
+        ls.append(w_labels)
+        cs.append(conj[bi])
+        gone = [blabels[c] for c, _ in g["fixed"]]
+        out_l = [rename.get(l, l) for l in out_labels if l not in gone]
+        fx = [(list(out_labels).index(blabels[c]), v) for c, v in g["fixed"]]
+        parts.append(_SplitPart(ts, ls, out_l, cs, fx, not fx))
+    # a part that writes the whole output goes first; the others accumulate
+    parts.sort(key=lambda p: not p.full)
+    return parts
+
+
+def _split_plan(W: "BlockSparse", env_col, identity, open_cols, blabels):
+    """The operator-side pieces of ``_split_identity`` for one block-sparse tensor
+    (cached on it): tuple groups, the residual tensor, the dense W' per group."""
+    T = W.tuples
+    groups: Dict[frozenset, List[int]] = {}
+    for r in range(len(T)):
+        ni = frozenset(i for i, c in env_col.items() if int(T[r, c]) not in identity[i])
+        groups.setdefault(ni, []).append(r)
+    if all(len(k) >= 2 for k in groups):
+        return False
+    host = W.data.detach().cpu().numpy()
+    plan = []
+    residual = sorted(r for k, rows in groups.items() if len(k) >= 2 for r in rows)
+    if residual:
+        plan.append({"residual": BlockSparse(
+            W.data[torch.as_tensor(residual, device=W.data.device)], W.block_axes,
+            T[residual], W.shape)})
+    for ni, rows in sorted(groups.items(), key=lambda kv: sorted(kv[0])):
+        if len(ni) >= 2:
+            continue
+        S, sidx = [], {}
+        if ni:
+            (i,) = tuple(ni)
+            S = sorted({int(T[r, env_col[i]]) for r in rows})
+            sidx = {s: s - S[0] for s in S}
+        fixed, kept = [], []
+        for c in open_cols:
+            vals = {int(T[r, c]) for r in rows}
+            if len(vals) == 1:
+                fixed.append((c, vals.pop()))
+            else:
+                kept.append(c)
+        shape = ([S[-1] + 1 - S[0]] if ni else []) + [W.shape[W.block_axes[c]] for c in kept] + \
+            list(host.shape[1:])
+        Wd = np.zeros(shape, dtype=host.dtype)
+        for r in rows:
+            idx = ([sidx[int(T[r, env_col[tuple(ni)[0]]])]] if ni else []) + \
+                [int(T[r, c]) for c in kept]
+            Wd[tuple(idx)] += host[r]
+        plan.append({"residual": None, "ni": ni, "states": S, "fixed": fixed, "kept": kept,
+                     "w": torch.as_tensor(Wd).to(W.data.device)})
+    return plan
+
+
+def _check_identity(tensors, identity) -> None:
+    for i, states in identity.items():
+        E_ = tensors[i]
+        eye = torch.eye(E_.shape[0], dtype=E_.dtype, device=E_.device)
+        for s in states:
+            dev = float((E_[:, s, :] - eye).abs().max())
+            if dev > IDENTITY_CHECK_TOL:
+                raise ValueError(f"environment block {s} of network tensor {i} is not the "
+                                 f"identity (max deviation {dev:.2e})")
+
+
+def _contract_split(parts: List[_SplitPart], out_shape, precision, out, out_dtype,
+                    accumulate) -> torch.Tensor:
+    if out is None:
+        out = torch.empty(out_shape, dtype=out_dtype, device=parts[0].tensors[0].device)
+    acc = accumulate
+    if not acc and not parts[0].full:
+        fill_(out, 0.0)
+        acc = True
+    for p in parts:
+        view = out
+        for ax, v in sorted(p.fixed, reverse=True):
+            view = view.select(ax, v)
+        contract_network(p.tensors, p.labels, p.out_labels, conj=p.conj, precision=precision,
+                         out=view, out_dtype=out_dtype, accumulate=acc)
+        acc = True
+    return out
+
+
 def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
                      out_labels: Sequence[Label], *, conj: Optional[Sequence[bool]] = None,
                      precision: str = FP64, out: Optional[torch.Tensor] = None,
-                     out_dtype: torch.dtype = torch.complex128) -> torch.Tensor:
+                     out_dtype: torch.dtype = torch.complex128,
+                     identity: Optional[Dict[int, Sequence[int]]] = None,
+                     accumulate: bool = False) -> torch.Tensor:
     """Contract a whole network; ``conj[i]`` conjugates tensor i on load.
 
-    At most one tensor may be a ``BlockSparse`` (the TTNO node tensor)."""
+    At most one tensor may be a ``BlockSparse`` (the TTNO node tensor).
+    ``identity`` ({tensor index: states}) names environment blocks that are
+    identity matrices (``_split_identity``); ``accumulate`` adds the result to
+    ``out`` instead of overwriting it."""
     n = len(tensors)
     conj = list(conj) if conj is not None else [False] * n
+    if identity:
+        parts = _split_identity(tensors, labels, out_labels, conj, identity)
+        if parts is not None:
+            if CHECK_IDENTITY:
+                _check_identity(tensors, identity)
+            dims = {}
+            for t, l in zip(tensors, labels):
+                dims.update(zip(l, t.shape))
+            return _contract_split(parts, tuple(dims[l] for l in out_labels), precision,
+                                   out, out_dtype, accumulate)
     dims: Dict[Label, int] = {}
     block = None
     for i, (t, l) in enumerate(zip(tensors, labels)):
@@ -1067,6 +1270,8 @@ def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
     if n == 1:
         if block is not None:
             raise ValueError("a lone block-sparse tensor cannot be contracted")
+        if accumulate:
+            raise ValueError("a lone tensor cannot be accumulated")
         perm = [list(labels[0]).index(l) for l in out_labels]
         return permute_copy(tensors[0], perm, out=out, conj=conj[0], dtype=out_dtype)
     plan = plan_network(labels, dims, out_labels, block)
@@ -1087,6 +1292,8 @@ def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
             A_, B_ = B_, A_
         if isinstance(A_, _Blk):
             tb, lb_, cb = B_
+            if last and accumulate:
+                raise ValueError("the last step of an accumulating network is block sparse")
             r = _contract_block(A_, tb, lb_, cb, res if last else None, precision,
                                 out if last else None, out_dtype if last else inter)
             slots.append(r if isinstance(r, _Blk) else
@@ -1104,7 +1311,8 @@ def contract_network(tensors: Sequence, labels: Sequence[Sequence[Label]],
                                             dims)
             r = contract(ta, la_, tb, lb_, res, conj_a=ca, conj_b=cb, precision=precision,
                          out=out if last else None,
-                         out_dtype=out_dtype if last else inter)
+                         out_dtype=out_dtype if last else inter,
+                         accumulate=accumulate and last)
             slots.append((r, res, False))
         slots[sa] = None
         slots[sb] = None
@@ -1168,7 +1376,9 @@ class CompiledNetwork:
 
     def __init__(self, tensors: Sequence, labels, out_labels, *, conj=None,
                  precision: str = FP64, out_dtype: torch.dtype = torch.complex128,
-                 variable: Sequence[int] = (0,)):
+                 variable: Sequence[int] = (0,),
+                 identity: Optional[Dict[int, Sequence[int]]] = None):
+        self.identity = identity
         self.tensors = list(tensors)
         self.labels = labels
         self.out_labels = out_labels
@@ -1206,7 +1416,11 @@ class CompiledNetwork:
                                                     else out)
                 _run_block(op[1], Y, T)
             else:
-                T = op[1] if roles[0] is None else out
+                if roles[0] is None:
+                    T = op[1]
+                else:
+                    e0 = roles[0][2] // out.element_size()
+                    T = out.view(-1).narrow(0, e0, op[1].numel())
                 fill_(T, op[2])
         return out
 
@@ -1227,7 +1441,8 @@ class CompiledNetwork:
         prev, _RECORDER = _RECORDER, rec
         try:
             out = contract_network(ts, self.labels, self.out_labels, conj=self.conj,
-                                   precision=self.precision, out_dtype=self.out_dtype)
+                                   precision=self.precision, out_dtype=self.out_dtype,
+                                   identity=self.identity)
         finally:
             _RECORDER = prev
         self.out_shape = tuple(out.shape)
@@ -1235,12 +1450,24 @@ class CompiledNetwork:
         self.ops = rec.ops
         self.rec_inputs, self.rec_out = ts, out
 
+        out_base = out.untyped_storage().data_ptr()
+
+        var_base = [v.untyped_storage().data_ptr() for v in var]
+
         def role(t, off=0):
             for k, v in enumerate(var):
                 if t is v:
                     return ("var", k, off)
             if t is out:
                 return ("out", 0, off)
+            # slices of a variable input or of the output (a network split at its
+            # identity blocks reads environment slices and writes output slices)
+            base = t.untyped_storage().data_ptr()
+            for k, v in enumerate(var):
+                if base == var_base[k]:
+                    return ("var", k, off + t.data_ptr() - v.data_ptr())
+            if base == out_base:
+                return ("out", 0, off + t.data_ptr() - out.data_ptr())
             return None
 
         self.op_roles = []
@@ -1534,7 +1761,8 @@ _NET_TOO_BIG: set = set()
 def contract_network_cached(tensors: Sequence, labels: Sequence[Sequence[Label]],
                             out_labels: Sequence[Label], *, conj: Optional[Sequence[bool]] = None,
                             precision: str = FP64,
-                            out_dtype: torch.dtype = torch.complex128) -> torch.Tensor:
+                            out_dtype: torch.dtype = torch.complex128,
+                            identity: Optional[Dict[int, Sequence[int]]] = None) -> torch.Tensor:
     """``contract_network`` through a recorded replay when the network is small.
 
     Dense inputs are the replay's variables (matched by shape, stride and dtype);
@@ -1542,16 +1770,18 @@ def contract_network_cached(tensors: Sequence, labels: Sequence[Sequence[Label]]
     var = [i for i, t in enumerate(tensors) if not isinstance(t, BlockSparse)]
     if not var or len(tensors) < 2:
         return contract_network(tensors, labels, out_labels, conj=conj, precision=precision,
-                                out_dtype=out_dtype)
+                                out_dtype=out_dtype, identity=identity)
     key = (tuple(tuple(l) for l in labels), tuple(out_labels),
            tuple(conj) if conj is not None else None, precision, out_dtype,
+           None if not identity else tuple(sorted(identity.items())),
            tuple(("blk", id(t)) if isinstance(t, BlockSparse) else
                  (tuple(t.shape), tuple(t.stride()), t.dtype, t.device) for t in tensors))
     hit = _NET_CACHE.get(key)
     if hit is None:
         if key in _NET_TOO_BIG:
             return contract_network(tensors, labels, out_labels, conj=conj,
-                                    precision=precision, out_dtype=out_dtype)
+                                    precision=precision, out_dtype=out_dtype,
+                                    identity=identity)
         dims: Dict[Label, int] = {}
         for t, lab in zip(tensors, labels):
             for l, d in zip(lab, t.shape):
@@ -1563,9 +1793,10 @@ def contract_network_cached(tensors: Sequence, labels: Sequence[Sequence[Label]]
                 _NET_TOO_BIG.clear()
             _NET_TOO_BIG.add(key)
             return contract_network(tensors, labels, out_labels, conj=conj,
-                                    precision=precision, out_dtype=out_dtype)
+                                    precision=precision, out_dtype=out_dtype,
+                                    identity=identity)
         net = CompiledNetwork(tensors, labels, out_labels, conj=conj, precision=precision,
-                              out_dtype=out_dtype, variable=var)
+                              out_dtype=out_dtype, variable=var, identity=identity)
         _NET_CACHE[key] = (net, nbytes)
         _NET_CACHE_BYTES[0] += nbytes
         while _NET_CACHE_BYTES[0] > NET_CACHE_BUDGET and len(_NET_CACHE) > 1:

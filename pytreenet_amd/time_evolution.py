@@ -33,6 +33,7 @@ from typing import Any, Dict, List, Optional, Sequence, Union
 import numpy as np
 import torch
 
+from . import contractions as _C
 from . import engine as E
 from .contractions import (SandwichCache, bond_apply, contract_any,
                            contract_ket_ham_with_envs, expectation_value, heff_network,
@@ -380,8 +381,19 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             self.state.move_orthogonalization_center(self.update_path[0], mode=SplitMode.KEEP)
 
     def _init_partial_tree_cache(self) -> SandwichCache:
+        # the state is canonical around update_path[0], and the sweeps keep every
+        # block's subtree an isometry towards the node it points to
         return SandwichCache.init_cache_but_one(self.state, self.hamiltonian,
-                                                self.update_path[0], precision=self.precision)
+                                                self.update_path[0], precision=self.precision,
+                                                canonical=True)
+
+    def _site_identity(self, node, tensor: torch.Tensor, op) -> Optional[Dict[int, tuple]]:
+        """Identity environment blocks of a site network [x, W, E_0, ...] (slots 2...)
+        when W is block sparse and the site tensor is large enough."""
+        if not (_C.IDENTITY_BLOCKS and isinstance(op, E.BlockSparse)
+                and tensor.numel() >= _C.IDENTITY_MIN_SIZE):
+            return None
+        return _C.identity_slots(node.identifier, node.neighbouring_nodes(), self.hamiltonian, 2)
 
     def update_tree_cache(self, node_id: str, next_node_id: str):
         self.partial_tree_cache.update_tree_cache(node_id, next_node_id)
@@ -500,8 +512,10 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
         envs = [cache.get_entry(nb, node_id) for nb in node.neighbouring_nodes()]
         tensors, labels, out_l = heff_network(node, op_node, op, envs)
         tensors[0] = tensor
+        ident = self._site_identity(node, tensor, op)
         # recorded on the first Krylov apply, replayed for the others
-        action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision)
+        action = E.CompiledNetwork(tensors, labels, out_l, precision=self.precision,
+                                   identity=ident)
         if getattr(self, "relaxed", False) and \
                 tensor.numel() >= getattr(self.config, "relax_min_size", 0):
             op32 = operator_tensor(self.hamiltonian, node_id, E.FP32,
@@ -513,8 +527,9 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             # and result, like the others (rocBLAS cgemm where it is strided)
             action = self._relaxed(action, E.CompiledNetwork(t32, l32, o32,
                                                              precision=self._low_precision(),
-                                                             out_dtype=torch.complex64),
-                                   (t32, l32, o32))
+                                                             out_dtype=torch.complex64,
+                                                             identity=ident),
+                                   (t32, l32, o32), ident)
 
         self.state.tensors[node_id] = yield from self._evolve(
             tensor, action, self.time_step_size * time_step_factor, EvoDirection.FORWARD)
@@ -526,10 +541,11 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
             raise ValueError(f"relax_low must be 'fp32' or 'bf16x6', not {low!r}")
         return low
 
-    def _relaxed(self, high, low, net32):
+    def _relaxed(self, high, low, net32, ident=None):
         lower = None
         if getattr(self.config, "relax_bf16", False):
-            lower = E.CompiledNetwork(*net32, precision=E.BF16, out_dtype=torch.complex64)
+            lower = E.CompiledNetwork(*net32, precision=E.BF16, out_dtype=torch.complex64,
+                                      identity=ident)
         return E.RelaxedNetwork(high, low, getattr(self.config, "relax_tol", 1e-12),
                                 lower=lower)
 

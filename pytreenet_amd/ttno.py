@@ -125,6 +125,64 @@ class TreeTensorNetworkOperator(TreeTensorNetwork):
             self._blocks[(node_id, dtype)] = hit
         return hit
 
+    def identity_states(self, from_id: str, to_id: str) -> Tuple[int, ...]:
+        """Bond states of the leg of ``from_id`` towards its neighbour ``to_id`` in
+        which the operator on ``from_id``'s side of that bond is exactly the
+        identity.
+
+        A state s qualifies when the node tensor, with that leg at s, has exactly
+        one non-zero block over its other virtual legs, that block equals the
+        identity on (out, in), and each of those other legs sits at a qualifying
+        state of its own neighbour's side (recursively, away from ``to_id``).
+        The environment of that side is then, in state s, the Gram matrix of the
+        side's state tensors: the identity whenever the side is an isometry
+        towards ``to_id`` (the canonical form a time evolution keeps).  In the
+        edge-term construction these are state 0 towards a parent and state 1
+        towards a child."""
+        key = tuple((nid, id(t)) for nid, t in self._tensors.items())
+        if getattr(self, "_ident_key", None) != key:
+            self._ident_key = key
+            self._ident: Dict[Tuple[str, str], Tuple[int, ...]] = {}
+            self._ident_host: Dict[str, np.ndarray] = {}
+        return self._identity_rec(from_id, to_id)
+
+    def _identity_rec(self, c: str, n: str) -> Tuple[int, ...]:
+        hit = self._ident.get((c, n))
+        if hit is not None:
+            return hit
+        node = self._nodes[c]
+        W = self._ident_host.get(c)
+        if W is None:
+            W = self._ident_host[c] = self._tensors[c].detach().cpu().numpy()
+        leg_n = node.neighbour_index(n)
+        need = {}
+        for m in node.neighbouring_nodes():
+            if m == n:
+                continue
+            sm = self._identity_rec(m, c)
+            if len(sm) != 1:
+                self._ident[(c, n)] = ()
+                return ()
+            need[node.neighbour_index(m)] = sm[0]
+        nv = node.nneighbours()
+        d_out, d_in = W.shape[nv], W.shape[nv + 1]
+        eye = np.eye(d_out, d_in, dtype=W.dtype)
+        found = []
+        for s in range(W.shape[leg_n]):
+            sl = np.take(W, s, axis=leg_n)          # other virtual legs, out, in
+            other = [a for a in range(nv) if a != leg_n]
+            flat = sl.reshape(-1, d_out * d_in)
+            nz = np.nonzero(np.any(flat != 0, axis=1))[0]
+            if len(nz) != 1:
+                continue
+            idx = np.unravel_index(nz[0], sl.shape[:nv - 1]) if nv > 1 else ()
+            if any(int(i) != need[a] for i, a in zip(idx, other)):
+                continue
+            if np.array_equal(flat[nz[0]].reshape(d_out, d_in), eye):
+                found.append(s)
+        self._ident[(c, n)] = tuple(found)
+        return self._ident[(c, n)]
+
     def tensor_in(self, node_id: str, dtype: torch.dtype) -> torch.Tensor:
         """The node tensor in ``dtype`` (complex64 copies are cached)."""
         t = self._tensors[node_id]

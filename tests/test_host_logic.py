@@ -295,3 +295,95 @@ def test_device_built_offset_tables_equal_host_tables():
         out = torch.empty(int(np.prod(dims)), dtype=torch.int64)
         E._device_offsets(dims, strides, out)
         assert np.array_equal(out.numpy(), E._offsets(dims, strides)), (dims, strides)
+
+
+def _dense_of(t):
+    from pytreenet_amd import engine as E
+    if isinstance(t, E.BlockSparse):
+        full = np.zeros(t.shape, dtype=complex)
+        data = t.data.numpy()
+        for r, tup in enumerate(t.tuples):
+            idx = [slice(None)] * len(t.shape)
+            for a, v in zip(t.block_axes, tup):
+                idx[a] = int(v)
+            full[tuple(idx)] = data[r]
+        return full
+    return t.numpy()
+
+
+def _einsum_network(tensors, labels, out_labels, conj):
+    letters = {}
+    for lab in list(labels) + [out_labels]:
+        for l in lab:
+            letters.setdefault(repr(l), "abcdefghijklmnopqrstuvwxyzABCDEFGHIJ"[len(letters)])
+    spec = ",".join("".join(letters[repr(l)] for l in lab) for lab in labels) + "->" + \
+        "".join(letters[repr(l)] for l in out_labels)
+    arrs = [np.conj(_dense_of(t)) if c else _dense_of(t) for t, c in zip(tensors, conj)]
+    return np.einsum(spec, *arrs, optimize="optimal")
+
+
+@pytest.mark.parametrize("model", ["xyz", "field"])
+def test_identity_split_sums_to_the_network(model):
+    """The parts of a network split at its identity environment blocks, each
+    contracted on the host with einsum and added into its slice of the output,
+    equal the whole network, for the effective Hamiltonians and every
+    environment update of the root, an inner node and a leaf of a ternary tree
+    (environments random except for their identity blocks).  The identity
+    states are those of the edge-term construction: 0 towards a parent, 1
+    towards a child."""
+    import torch
+    from pytreenet_amd import engine as E
+    from pytreenet_amd.contractions import _env_labels, _op_labels, heff_network
+    from pytreenet_amd.random_states import build_state, random_state_tensors
+    ids, parent, children = complete_tree(3, 2)
+    st = build_state(ids, parent, children,
+                     random_state_tensors(ids, parent, children, 3, seed=0), device="cpu")
+    edges = [(n.parent, nid) for nid, n in st.nodes.items() if n.parent]
+    H = ptn.random_xyz_model(st, seed=1) if model == "xyz" else \
+        ptn.xyz_model(st, {e: (0.3, -0.7, 1.1) for e in edges}, {n: 0.4 for n in st.nodes})
+    rng = np.random.default_rng(0)
+    for nid, node in st.nodes.items():
+        for nb in node.neighbouring_nodes():
+            assert H.identity_states(nid, nb) == ((0,) if node.parent == nb else (1,))
+    for nid in ["site0", "site1", "site4"]:
+        node, opn = st.nodes[nid], H.nodes[nid]
+        W = E.BlockSparse.from_dense(H.tensors[nid], range(opn.nneighbours()))
+        nbs = node.neighbouring_nodes()
+        ident = {2 + i: H.identity_states(nb, nid) for i, nb in enumerate(nbs)}
+        envs = []
+        for i, nb in enumerate(nbs):
+            chi, D = 2 + i % 2, H.tensors[nid].shape[opn.neighbour_index(nb)]
+            e = rng.standard_normal((chi, D, chi)) + 1j * rng.standard_normal((chi, D, chi))
+            for s in ident[2 + i]:
+                e[:, s, :] = np.eye(chi)
+            envs.append(torch.as_tensor(e))
+        x = torch.as_tensor(rng.standard_normal(tuple(e.shape[0] for e in envs) + (2,)) + 0j)
+
+        def check(ts, ls, out, cj, idd):
+            ref = _einsum_network(ts, ls, out, cj)
+            parts = E._split_identity(ts, ls, out, cj, idd)
+            assert parts is not None and parts[0].full
+            tot = np.zeros(ref.shape, dtype=complex)
+            for p in parts:
+                idx = [slice(None)] * tot.ndim
+                for ax, v in p.fixed:
+                    idx[ax] = v
+                tot[tuple(idx)] += _einsum_network(p.tensors, p.labels, p.out_labels, p.conj)
+            np.testing.assert_allclose(tot, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+        tensors, labels, out = heff_network(node, opn, W, envs)
+        tensors[0] = x
+        check(tensors, labels, out, [False] * len(tensors), ident)
+        k = len(nbs)
+        for to in range(k):
+            ts = [x, W, x]
+            ls = [[("a", i) for i in range(k)] + ["p"], _op_labels(node, opn),
+                  [("b", i) for i in range(k)] + ["q"]]
+            idd = {}
+            for i in range(k):
+                if i != to:
+                    idd[len(ts)] = ident[2 + i]
+                    ts.append(envs[i])
+                    ls.append(list(_env_labels(i)))
+            check(ts, ls, list(_env_labels(to)), [False, False, True] + [False] * (len(ts) - 3),
+                  idd)
