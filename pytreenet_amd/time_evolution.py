@@ -694,35 +694,44 @@ class TwoSiteTDVP(TDVPAlgorithm):
             lab[op_node.neighbour_index(other)] = "wl"
             lab[-2], lab[-1] = phys_out, phys_in
             return lab
-        tensors = [None, h.tensor_in(n1, dt), h.tensor_in(n2, dt)]
-        labels = [[("a", i) for i in range(k1 + k2)] + ["p1", "p2"],
-                  op_lab(op1n, nb1, 0, n2, "q1", "p1"),
-                  op_lab(op2n, nb2, k1, n1, "q2", "p2")]
-        for i, nb in enumerate(nb1):
-            tensors.append(cache.get_entry(nb, n1))
-            labels.append([("a", i), ("w", i), ("b", i)])
-        for i, nb in enumerate(nb2):
-            tensors.append(cache.get_entry(nb, n2))
-            labels.append([("a", k1 + i), ("w", k1 + i), ("b", k1 + i)])
+        envs = [cache.get_entry(nb, n1) for nb in nb1] + [cache.get_entry(nb, n2) for nb in nb2]
+        env_l = [[("a", i), ("w", i), ("b", i)] for i in range(k1 + k2)]
+        x_l = [("a", i) for i in range(k1 + k2)] + ["p1", "p2"]
         out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
+        dense_l = [x_l, op_lab(op1n, nb1, 0, n2, "q1", "p1"),
+                   op_lab(op2n, nb2, k1, n1, "q2", "p2")] + env_l
+        # In the canonical form of the sweep the outer environments have identity
+        # blocks: the two operator tensors then meet as one block-sparse two-site
+        # tensor, and the network splits at those blocks (engine._split_identity)
+        split = getattr(cache, "canonical", False) and _C.IDENTITY_BLOCKS
+        ident = None
+        if split:
+            ident = {2 + i: h.identity_states(nb, n1) for i, nb in enumerate(nb1)}
+            ident.update({2 + k1 + i: h.identity_states(nb, n2) for i, nb in enumerate(nb2)})
+            split_l = [x_l, [("w", i) for i in range(k1 + k2)] + ["q1", "q2", "p1", "p2"]] + env_l
+        split_min = max(_C.IDENTITY_MIN_SIZE, getattr(self, "dense_site_below", 0))
+
+        def net(y, dtype, precision, out_dtype):
+            if split and y.numel() >= split_min:
+                w12 = h.two_site_block_tensor(n1, n2, nb1, nb2, dtype)
+                es = envs if dtype == torch.complex128 else [e.to(dtype) for e in envs]
+                return E.CompiledNetwork([y, w12] + es, split_l, out, precision=precision,
+                                         out_dtype=out_dtype, identity=ident)
+            es = envs if dtype == torch.complex128 else [e.to(dtype) for e in envs]
+            return E.CompiledNetwork([y, h.tensor_in(n1, dtype), h.tensor_in(n2, dtype)] + es,
+                                     dense_l, out, precision=precision, out_dtype=out_dtype)
         # (DMRG borrows this method: its objects have no relaxed form)
         if getattr(self, "relaxed", False):
             min_size = getattr(self.config, "relax_min_size", 0)
 
             def make(y):
-                high = E.CompiledNetwork([y] + tensors[1:], labels, out,
-                                         precision=self.precision)
+                high = net(y, dt, self.precision, torch.complex128)
                 if y.numel() < min_size:
                     return high
-                t32 = [h.tensor_in(n1, torch.complex64), h.tensor_in(n2, torch.complex64)] + \
-                    [e.to(torch.complex64) for e in tensors[3:]]
-                return self._relaxed(high, E.CompiledNetwork([y] + t32, labels, out,
-                                                             precision=self._low_precision(),
-                                                             out_dtype=torch.complex64),
-                                     ([y] + t32, labels, out))
+                low = net(y, torch.complex64, self._low_precision(), torch.complex64)
+                return self._relaxed(high, low, (low.tensors, low.labels, out), low.identity)
             return make, k1, k2
-        return (lambda y: E.CompiledNetwork([y] + tensors[1:], labels, out,
-                                            precision=self.precision)), k1, k2
+        return (lambda y: net(y, dt, self.precision, torch.complex128)), k1, k2
 
     def _update_two_site_nodes(self, target_node_id: str, next_node_id: str,
                                time_step_factor: float = 1):

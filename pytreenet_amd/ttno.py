@@ -183,6 +183,41 @@ class TreeTensorNetworkOperator(TreeTensorNetwork):
         self._ident[(c, n)] = tuple(found)
         return self._ident[(c, n)]
 
+    def two_site_block_tensor(self, n1: str, n2: str, nbs1: Sequence[str],
+                              nbs2: Sequence[str], dtype: torch.dtype):
+        """The operator of the neighbouring nodes n1, n2 contracted over their bond,
+        block sparse over its outer bond legs: legs (legs of n1 towards nbs1...,
+        legs of n2 towards nbs2..., out1, out2, in1, in2) in the order given.
+        Built on the host once per pair and dtype (cached)."""
+        from .engine import BlockSparse
+        key = ("two", n1, n2, tuple(nbs1), tuple(nbs2), dtype)
+        t1, t2 = self._tensors[n1], self._tensors[n2]
+        hit = self._blocks.get(key)
+        if hit is not None and hit.source[0] is t1 and hit.source[1] is t2:
+            return hit
+        a1, a2 = self._nodes[n1], self._nodes[n2]
+        W1 = t1.detach().cpu().numpy()
+        W2 = t2.detach().cpu().numpy()
+        k1, k2 = len(nbs1), len(nbs2)
+        # einsum letters: outer legs, the shared bond, physical legs
+        o1 = [chr(ord("a") + i) for i in range(k1)]
+        o2 = [chr(ord("a") + k1 + i) for i in range(k2)]
+        s1 = [None] * a1.nneighbours()
+        for i, nb in enumerate(nbs1):
+            s1[a1.neighbour_index(nb)] = o1[i]
+        s1[a1.neighbour_index(n2)] = "z"
+        s2 = [None] * a2.nneighbours()
+        for i, nb in enumerate(nbs2):
+            s2[a2.neighbour_index(nb)] = o2[i]
+        s2[a2.neighbour_index(n1)] = "z"
+        spec = "".join(s1) + "QP," + "".join(s2) + "RS->" + "".join(o1 + o2) + "QRPS"
+        W12 = np.einsum(spec, W1, W2)
+        dense = torch.as_tensor(W12).to(device=t1.device, dtype=dtype)
+        hit = BlockSparse.from_dense(dense, range(k1 + k2))
+        hit.source = (t1, t2)
+        self._blocks[key] = hit
+        return hit
+
     def tensor_in(self, node_id: str, dtype: torch.dtype) -> torch.Tensor:
         """The node tensor in ``dtype`` (complex64 copies are cached)."""
         t = self._tensors[node_id]

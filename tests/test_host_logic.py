@@ -387,3 +387,57 @@ def test_identity_split_sums_to_the_network(model):
                     ls.append(list(_env_labels(i)))
             check(ts, ls, list(_env_labels(to)), [False, False, True] + [False] * (len(ts) - 3),
                   idd)
+
+
+def test_two_site_operator_tensor_and_its_split():
+    """The merged two-site operator (block sparse over the outer bonds) equals the two
+    operator tensors contracted over their bond, and the two-site network split at
+    its identity environment blocks sums to the network with both tensors."""
+    import torch
+    from pytreenet_amd import engine as E
+    from pytreenet_amd.contractions import _op_labels
+    from pytreenet_amd.random_states import build_state, random_state_tensors
+    ids, parent, children = complete_tree(3, 2)
+    st = build_state(ids, parent, children,
+                     random_state_tensors(ids, parent, children, 3, seed=0), device="cpu")
+    edges = [(n.parent, nid) for nid, n in st.nodes.items() if n.parent]
+    H = ptn.xyz_model(st, {e: (0.3, -0.7, 1.1) for e in edges}, {n: 0.4 for n in st.nodes})
+    rng = np.random.default_rng(3)
+    for n1, n2 in [("site0", "site1"), ("site1", "site4"), ("site2", "site0")]:
+        a1, a2 = H.nodes[n1], H.nodes[n2]
+        nb1 = [n for n in st.nodes[n1].neighbouring_nodes() if n != n2]
+        nb2 = [n for n in st.nodes[n2].neighbouring_nodes() if n != n1]
+        k1, k2 = len(nb1), len(nb2)
+        W12 = H.two_site_block_tensor(n1, n2, nb1, nb2, torch.complex128)
+        envs, ident = [], {}
+        for i, (nb, me) in enumerate([(nb, n1) for nb in nb1] + [(nb, n2) for nb in nb2]):
+            node = H.nodes[me]
+            chi, D = 2 + i % 2, H.tensors[me].shape[node.neighbour_index(nb)]
+            e = rng.standard_normal((chi, D, chi)) + 1j * rng.standard_normal((chi, D, chi))
+            ident[2 + i] = H.identity_states(nb, me)
+            for s in ident[2 + i]:
+                e[:, s, :] = np.eye(chi)
+            envs.append(torch.as_tensor(e))
+        x = torch.as_tensor(rng.standard_normal(tuple(e.shape[0] for e in envs) + (2, 2)) + 0j)
+        env_l = [[("a", i), ("w", i), ("b", i)] for i in range(k1 + k2)]
+        x_l = [("a", i) for i in range(k1 + k2)] + ["p1", "p2"]
+        out = [("b", i) for i in range(k1 + k2)] + ["q1", "q2"]
+
+        def op_lab(node, own, offset, other, qo, pi):
+            lab = [None] * (node.nneighbours() + 2)
+            for i, nb in enumerate(own):
+                lab[node.neighbour_index(nb)] = ("w", offset + i)
+            lab[node.neighbour_index(other)] = "wl"
+            lab[-2], lab[-1] = qo, pi
+            return lab
+        ref = _einsum_network([x, H.tensors[n1], H.tensors[n2]] + envs,
+                              [x_l, op_lab(a1, nb1, 0, n2, "q1", "p1"),
+                               op_lab(a2, nb2, k1, n1, "q2", "p2")] + env_l,
+                              out, [False] * (3 + k1 + k2))
+        split_l = [x_l, [("w", i) for i in range(k1 + k2)] + ["q1", "q2", "p1", "p2"]] + env_l
+        whole = _einsum_network([x, W12] + envs, split_l, out, [False] * (2 + k1 + k2))
+        np.testing.assert_allclose(whole, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+        parts = E._split_identity([x, W12] + envs, split_l, out, [False] * (2 + k1 + k2), ident)
+        assert parts is not None
+        tot = sum(_einsum_network(p.tensors, p.labels, p.out_labels, p.conj) for p in parts)
+        np.testing.assert_allclose(tot, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
