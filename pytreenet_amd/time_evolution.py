@@ -933,9 +933,12 @@ class BatchedTDVP:
         for a, r in zip(algorithms, reqs):
             if r.x.numel() >= self.SEQUENTIAL_N:
                 from .krylov import STATS
-                n0 = STATS.applies
+                n0, p0 = STATS.applies, _K.pending_count()
                 r.result = a._krylov(r.x, r.action, r.dt, r.direction)
-                r.applies = STATS.applies - n0
+                # a one-launch solve inside a deferred step hands out its Krylov
+                # dimension when the step ends (resolved by _lockstep_checked)
+                r.applies = (_K.PendingApplies(p0) if _K.pending_count() > p0
+                             else STATS.applies - n0)
                 self.krylov_dims.append(r.applies)
                 # every instance's network stays referenced until all are answered:
                 # drop this one's recorded intermediates now
@@ -986,7 +989,46 @@ class BatchedTDVP:
             a.dense_site_below = self.SEQUENTIAL_N
         size = self.lockstep_group or len(self.algorithms)
         for g0 in range(0, len(self.algorithms), size):
-            self._lockstep(self.algorithms[g0:g0 + size])
+            self._lockstep_checked(self.algorithms[g0:g0 + size])
+
+    def _lockstep_checked(self, algorithms: Sequence[TDVPAlgorithm]):
+        """One lockstep group's time step with the CholeskyQR pivot checks and the
+        one-launch Krylov infos read once at its end, as a single instance's step
+        does (``TDVPAlgorithm.run_one_time_step``): a split that collapsed restores
+        every instance of the group and steps the group again with each split
+        checked as it happens."""
+        from .splitting import deferred_qr_checks
+        if not TDVPAlgorithm.DEFER_QR_CHECKS:
+            self._lockstep(algorithms)
+            return
+        snaps = [a._snapshot() for a in algorithms]
+        nlogs = [len(a.apply_log) for a in algorithms]
+        nk = len(self.krylov_dims)
+        with deferred_qr_checks() as checks, _K.deferred_small_info() as small:
+            try:
+                self._lockstep(algorithms)
+            except Exception:
+                if not checks.failed():
+                    raise
+                failed = True
+            else:
+                failed = checks.failed()
+            if not failed:
+                ms = _K.resolve_pending(small)
+                for a, nlog in zip(algorithms, nlogs):
+                    a.apply_log[nlog:] = [
+                        (k, n, ms[x.index] if isinstance(x, _K.PendingApplies) else x)
+                        for k, n, x in a.apply_log[nlog:]]
+                    if isinstance(a._last_applies, _K.PendingApplies):
+                        a._last_applies = ms[a._last_applies.index]
+                self.krylov_dims[nk:] = [ms[x.index] if isinstance(x, _K.PendingApplies)
+                                         else x for x in self.krylov_dims[nk:]]
+        if failed:
+            TDVPAlgorithm.steps_redone += 1
+            for a, snap in zip(algorithms, snaps):
+                a._restore(snap)
+            del self.krylov_dims[nk:]
+            self._lockstep(algorithms)
 
     def _lockstep(self, algorithms: Sequence[TDVPAlgorithm]):
         gens = [a._time_step_gen() for a in algorithms]

@@ -239,3 +239,44 @@ def test_batched_config4_shape_matches_single(gpu):
         np.testing.assert_allclose(ob, oa, rtol=0, atol=1e-12 * max(1.0, np.abs(oa).max()))
         assert a.kept_dims == b.kept_dims
         assert max(a.kept_dims) > 8
+
+
+def test_batched_deferred_qr_checks_redo_a_collapsed_group(gpu):
+    """The lockstep reads the CholeskyQR pivot statuses of a group once, at the end
+    of its step.  One-site instances on a ternary tree at chi 16 (root and level-1
+    splits above the one-launch QR size), one of them zero-padded beyond bond
+    dimension 3 so its splits collapse: the group is restored and stepped again
+    with checked splits, and every instance equals its single run (which redoes
+    its own step the same way)."""
+    _say("batched deferred QR checks with a collapsing instance")
+    from pytreenet_amd.time_evolution import TDVPAlgorithm
+    ids, parent, children = complete_tree(3, 2)
+
+    def make(b):
+        tensors = random_state_tensors(ids, parent, children, 16, seed=70 + b)
+        if b == 1:
+            for nid, t in tensors.items():
+                t = t.copy()
+                for ax in range(t.ndim - 1):
+                    idx = [slice(None)] * t.ndim
+                    idx[ax] = slice(3, None)
+                    t[tuple(idx)] = 0
+                tensors[nid] = t
+        st = build_state(ids, parent, children, tensors, device="cuda")
+        H = random_field_heisenberg(st, 2.0, seed=170 + b)
+        cfg = ptn.TDVPConfig(time_evo_mode=ptn.TimeEvoMode.KRYLOV)
+        a = ptn.SecondOrderOneSiteTDVP(st, H, 0.05, 0.1, [], config=cfg,
+                                       solver_options={"max_krylov_dim": 8, "krylov_tol": 1e-10})
+        return a
+
+    single = [make(b) for b in range(3)]
+    batch = [make(b) for b in range(3)]
+    for a in single:
+        a.run_one_time_step()
+    n0 = TDVPAlgorithm.steps_redone
+    ptn.BatchedTDVP(batch).run(1)
+    assert TDVPAlgorithm.steps_redone > n0
+    for a, b in zip(single, batch):
+        oa, ob = _observables(a), _observables(b)
+        np.testing.assert_allclose(ob, oa, rtol=0, atol=1e-12 * max(1.0, np.abs(oa).max()))
+        assert [x[2] for x in a.apply_log] == [x[2] for x in b.apply_log]
