@@ -306,6 +306,10 @@ def main():
                     help="launch the non-LARGE GEMMs on autotuned configurations")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 8 ptn_permute copies of 1 GiB (HBM counter calibration)")
+    ap.add_argument("--launch-log", default=None,
+                    help="write every LARGE gather-GEMM launch of the process in issue order "
+                         "(M, N, K, precision, beta_one, c_mmajor, phase) to this JSON file, "
+                         "to match per-dispatch profiler counters to shapes")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -446,6 +450,9 @@ def main():
                                           "batching configuration's): the set rocprofv3 --stats "
                                           "averages over"}
     E.PROFILER.stop()
+    if args.launch_log:
+        with open(args.launch_log, "w") as f:
+            json.dump(E.PROFILER.order, f)
     print(json.dumps(line), flush=True)
 
 

@@ -184,6 +184,9 @@ class LaunchProfiler:
         self.flops = 0.0
         self.launches = 0
         self.events: Dict[tuple, list] = {}
+        # every LARGE gather launch in issue order: (M, N, K, precision, beta_one,
+        # c_mmajor, phase), to match per-dispatch profiler counters to shapes
+        self.order: List[tuple] = []
 
     def start(self, phase: str = "default", reset: bool = True):
         if reset:
@@ -587,6 +590,8 @@ def _run_gemm(d: GemmDesc, M: int, N: int, K: int, precision: str):
         prof.flops += fl
         prof.launches += 1
         if cls == "large":
+            prof.order.append((M, N, K, precision, int(d.beta_one), int(d.c_mmajor),
+                               prof.phase))
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()

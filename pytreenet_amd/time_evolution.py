@@ -994,9 +994,10 @@ class BatchedTDVP:
     def _lockstep_checked(self, algorithms: Sequence[TDVPAlgorithm]):
         """One lockstep group's time step with the CholeskyQR pivot checks and the
         one-launch Krylov infos read once at its end, as a single instance's step
-        does (``TDVPAlgorithm.run_one_time_step``): a split that collapsed restores
-        every instance of the group and steps the group again with each split
-        checked as it happens."""
+        does (``TDVPAlgorithm.run_one_time_step``).  Each instance's split statuses
+        are kept apart: an instance with a collapsed split is restored and steps
+        again on its own (the single-instance path, which checks its splits as it
+        goes when it has to); the other instances keep their batched step."""
         from .splitting import deferred_qr_checks
         if not TDVPAlgorithm.DEFER_QR_CHECKS:
             self._lockstep(algorithms)
@@ -1004,16 +1005,22 @@ class BatchedTDVP:
         snaps = [a._snapshot() for a in algorithms]
         nlogs = [len(a.apply_log) for a in algorithms]
         nk = len(self.krylov_dims)
-        with deferred_qr_checks() as checks, _K.deferred_small_info() as small:
+        flags: List[List[torch.Tensor]] = [[] for _ in algorithms]
+        err = None
+        with deferred_qr_checks(), _K.deferred_small_info() as small:
             try:
-                self._lockstep(algorithms)
-            except Exception:
-                if not checks.failed():
-                    raise
-                failed = True
-            else:
-                failed = checks.failed()
-            if not failed:
+                self._lockstep(algorithms, flags)
+            except Exception as exc:      # downstream of a collapsed split, possibly
+                err = exc
+            bad = [False] * len(algorithms)
+            allf = [(i, f) for i, fl in enumerate(flags) for f in fl]
+            if allf:
+                vals = torch.cat([f for _, f in allf]).view(len(allf), -1).sum(1).cpu()
+                for (i, _), v in zip(allf, vals.tolist()):
+                    bad[i] = bad[i] or v != 0
+            if err is not None and not any(bad):
+                raise err
+            if err is None:
                 ms = _K.resolve_pending(small)
                 for a, nlog in zip(algorithms, nlogs):
                     a.apply_log[nlog:] = [
@@ -1023,18 +1030,36 @@ class BatchedTDVP:
                         a._last_applies = ms[a._last_applies.index]
                 self.krylov_dims[nk:] = [ms[x.index] if isinstance(x, _K.PendingApplies)
                                          else x for x in self.krylov_dims[nk:]]
-        if failed:
+        if err is not None:
+            # which instance failed downstream is unknown: the group steps again with
+            # every split checked as it happens
             TDVPAlgorithm.steps_redone += 1
             for a, snap in zip(algorithms, snaps):
                 a._restore(snap)
             del self.krylov_dims[nk:]
             self._lockstep(algorithms)
+            return
+        for a, snap, b in zip(algorithms, snaps, bad):
+            if b:
+                TDVPAlgorithm.steps_redone += 1
+                a._restore(snap)
+                a.run_one_time_step()
 
-    def _lockstep(self, algorithms: Sequence[TDVPAlgorithm]):
+    def _lockstep(self, algorithms: Sequence[TDVPAlgorithm],
+                  flags: Optional[List[List[torch.Tensor]]] = None):
+        """Advance all instances' sweeps together.  ``flags``: per instance, the
+        list its CholeskyQR splits append their deferred statuses to."""
+        from .splitting import _DEFER
         gens = [a._time_step_gen() for a in algorithms]
-        reqs = []
-        for g in gens:
-            reqs.append(next(g, None))
+
+        def advance(i, first=False):
+            if flags is not None:
+                _DEFER.flags = flags[i]
+            try:
+                return next(gens[i]) if first else gens[i].send(None)
+            except StopIteration:
+                return None
+        reqs = [advance(i, True) for i in range(len(gens))]
         done = 0
         while True:
             if all(r is None for r in reqs):
@@ -1052,12 +1077,7 @@ class BatchedTDVP:
             done += 1
             if self.progress is not None:
                 self.progress(done, kind)
-            reqs = []
-            for g in gens:
-                try:
-                    reqs.append(g.send(None))
-                except StopIteration:
-                    reqs.append(None)
+            reqs = [advance(i) for i in range(len(gens))]
 
     def run(self, steps: int):
         for _ in range(steps):
