@@ -2,9 +2,10 @@
 //
 // * ptn_potrf_lower / ptn_trtri_lower: the small n x n Cholesky factor and its
 //   inverse used by the CholeskyQR3 node split (Gram matrices and triangular
-//   updates of that split run as gather-GEMMs).  One workgroup factors the
-//   matrix in place in global memory (L2 resident for n <= 1024); the inverse
-//   runs one workgroup per column.
+//   updates of that split run as gather-GEMMs).  n <= 16: one workgroup factors
+//   the matrix in global memory (L2 resident); above, one launch per block column
+//   of 16 with one workgroup per block row (left-looking).  The inverse runs one
+//   workgroup per 16 columns (n <= 512) or per column.
 // * ptn_qr / ptn_svd: Householder QR and thin SVD through rocSOLVER (AMD's
 //   LAPACK on ROCm) with one lazily created handle per device -- the QR is the
 //   fallback for rank-deficient panels, the SVD serves the truncated splits.
@@ -413,6 +414,243 @@ __global__ __launch_bounds__(PT) void potrf_blocked_kernel(int n, const double2 
   if (tid == 0) *status = sh_fail;
 }
 
+// Left-looking blocked Cholesky: one launch per block column J of CB columns, one
+// workgroup per block row I >= J (workgroup 0: the diagonal block).  Each
+// workgroup forms its blocks from G minus the products of the finished panels
+// L[., 0..J) (staged through LDS in chunks of CKC columns), factors the CB x CB
+// diagonal block in one wave (lane i holds row i; pivots and columns pass by
+// shuffles; every workgroup does this, so none waits for another), and solves its
+// own panel block X L_JJ^H = A_IJ, 32 lanes per row.  The matrix is spread over
+// up to n / CB CUs per launch instead of one; same shift, pivot test and status as
+// the kernels above.
+constexpr int CKC = 32;
+template <int CB>
+__global__ __launch_bounds__(256) void potrf_left_kernel(int n, int J, const double2 *G,
+                                                         int64_t ldg, double2 *L,
+                                                         double shift_factor,
+                                                         double min_pivot, int *status) {
+  __shared__ double2 D[CB][CB + 1];
+  __shared__ double2 P[CB][CB + 1];
+  __shared__ double2 Lj[CB][CKC + 1];
+  __shared__ double2 Li[CB][CKC + 1];
+  __shared__ double red[4];
+  __shared__ double sh_shift;
+  __shared__ int sh_fail;
+  const int tid = threadIdx.x;
+  const int I = J + blockIdx.x * CB;
+  const bool diag = blockIdx.x == 0;
+  const int jb = (n - J) < CB ? (n - J) : CB;
+  const int ib = (n - I) < CB ? (n - I) : CB;
+  double tr = 0;
+  if (shift_factor != 0.0)
+    for (int i = tid; i < n; i += 256) tr += G[(int64_t)i * ldg + i].x;
+  for (int off = 32; off > 0; off >>= 1) tr += __shfl_down(tr, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = tr;
+  __syncthreads();
+  if (tid == 0) {
+    sh_shift = shift_factor * (red[0] + red[1] + red[2] + red[3]);
+    sh_fail = 0;
+  }
+  __syncthreads();
+  // thread tid owns elements (tid / CB + RS q, tid % CB), q < EPT, of D and of P
+  constexpr int RS = 256 / CB, EPT = CB * CB / 256;
+  double2 aD[EPT], aP[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int r = tid / CB + RS * q, c = tid % CB;
+    double2 v = make_double2(0, 0);
+    if (r < jb && c <= r) {
+      v = G[(int64_t)(J + r) * ldg + J + c];
+      if (r == c) v = make_double2(v.x + sh_shift, 0.0);
+    }
+    aD[q] = v;
+    aP[q] = (!diag && r < ib && c < jb) ? G[(int64_t)(I + r) * ldg + J + c] : make_double2(0, 0);
+  }
+  for (int k0 = 0; k0 < J; k0 += CKC) {
+    for (int t = tid; t < CB * CKC; t += 256) {
+      const int r = t / CKC, kk = t % CKC;
+      const bool in = k0 + kk < J;   // only the finished columns
+      Lj[r][kk] = (in && r < jb) ? L[(int64_t)(J + r) * n + k0 + kk] : make_double2(0, 0);
+      Li[r][kk] = (in && !diag && r < ib) ? L[(int64_t)(I + r) * n + k0 + kk]
+                                          : make_double2(0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int r = tid / CB + RS * q, c = tid % CB;
+      double dr = 0, di = 0, pr = 0, pi = 0;
+#pragma unroll 8
+      for (int kk = 0; kk < CKC; ++kk) {
+        const double2 b = Lj[c][kk], a = Lj[r][kk], x = Li[r][kk];
+        // a conj(b), x conj(b)
+        dr += a.x * b.x + a.y * b.y;
+        di += a.y * b.x - a.x * b.y;
+        pr += x.x * b.x + x.y * b.y;
+        pi += x.y * b.x - x.x * b.y;
+      }
+      if (c <= r) aD[q] = make_double2(aD[q].x - dr, aD[q].y - di);
+      aP[q] = make_double2(aP[q].x - pr, aP[q].y - pi);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int r = tid / CB + RS * q, c = tid % CB;
+    D[r][c] = c <= r ? aD[q] : make_double2(0, 0);
+    P[r][c] = aP[q];
+  }
+  __syncthreads();
+  // the diagonal block, factored by wave 0: lane i holds row i
+  if (tid < 64) {
+    const int i = tid;
+    double2 row[CB];
+#pragma unroll
+    for (int k = 0; k < CB; ++k)
+      row[k] = (i < jb && k <= i) ? D[i][k] : make_double2(0, 0);
+    int fail = 0;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const bool act = j < jb;
+      double d = __shfl(row[j].x, j, 64);
+      if (act && !(d > min_pivot)) {  // also catches NaN
+        fail = 1;
+        d = 1.0;
+      }
+      if (!act) d = 1.0;
+      const double rt = sqrt(d), inv = 1.0 / rt;
+      if (act && i == j) row[j] = make_double2(rt, 0.0);
+      if (act && i > j) {
+        row[j].x *= inv;
+        row[j].y *= inv;
+      }
+#pragma unroll
+      for (int k = j + 1; k < CB; ++k) {
+        const double br = __shfl(row[j].x, k, 64), bi = __shfl(row[j].y, k, 64);
+        if (act && k <= i && i < jb) {
+          row[k].x -= row[j].x * br + row[j].y * bi;
+          row[k].y -= row[j].y * br - row[j].x * bi;
+        }
+      }
+    }
+    if (i < CB) {
+#pragma unroll
+      for (int k = 0; k < CB; ++k) D[i][k] = (i < jb && k <= i) ? row[k] : make_double2(0, 0);
+    }
+    if (i < jb && fail) sh_fail = 1;
+  }
+  __syncthreads();
+  if (diag) {
+    // rows J..J+jb of L from column J on: the factored block, zero to its right
+    const int w = n - J;
+    for (int t = tid; t < jb * w; t += 256) {
+      const int r = t / w, cc = t % w;
+      L[(int64_t)(J + r) * n + J + cc] = cc < CB ? D[r][cc] : make_double2(0, 0);
+    }
+    if (tid == 0) {
+      if (J == 0)
+        *status = sh_fail;
+      else if (sh_fail)
+        *status = 1;
+    }
+    return;
+  }
+  // panel block: row r solves x L_JJ^H = p, lane k (of 32) owns x_k
+  const int k = tid & (CB - 1), base = (tid & 63) & ~(CB - 1);
+  const double dkk = D[k][k].x;
+  for (int r = tid / CB; r < CB; r += RS) {
+    double2 v = P[r][k];
+#pragma unroll
+    for (int l = 0; l < CB; ++l) {
+      if (k == l && l < jb) {
+        v.x /= dkk;
+        v.y /= dkk;
+      }
+      const double xr = __shfl(v.x, base | l, 64), xi = __shfl(v.y, base | l, 64);
+      if (k > l && l < jb) {
+        const double2 dl = D[k][l];
+        v.x -= xr * dl.x + xi * dl.y;
+        v.y -= xi * dl.x - xr * dl.y;
+      }
+    }
+    if (r < ib && k < jb) L[(int64_t)(I + r) * n + J + k] = v;
+  }
+}
+
+// X = L^{-1} (lower), n <= TRB_MAXN: workgroup c solves L X[:, 16c..16c+15] = E by
+// forward substitution over 16-row blocks.  For a row block I, the rows
+// L[I..I+15, J..I) are staged through LDS in coalesced chunks of TRB_KC columns
+// and thread (r, c) subtracts L[I+r, J..I) X[J..I, c] (the earlier row blocks of
+// its column, kept in LDS); then the 16 x 16 diagonal solve runs in registers:
+// the 16 lanes of one column sit in one wave and pass each finished x_k along
+// with a shuffle.  A few barriers per row block instead of one per row.
+constexpr int TRB = 16;
+constexpr int TRB_MAXN = 512;
+constexpr int TRB_KC = 64;
+__global__ __launch_bounds__(256) void trtri_blocked_kernel(int n, const double2 *L,
+                                                            double2 *X) {
+  __shared__ double2 xs[TRB_MAXN * TRB];        // rows J.. of this column block
+  __shared__ double2 lp[TRB][TRB_KC + 1];       // a chunk of the row block's L rows
+  const int J = blockIdx.x * TRB;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int r = lane & (TRB - 1);               // row within the row block
+  const int c = (tid >> 6) * 4 + (lane >> 4);   // column within the column block
+  const int col = J + c;
+  // rows above the column block are zero in X
+  for (int t = tid; t < J * TRB; t += 256) {
+    const int row = t / TRB, cc = J + t % TRB;
+    if (cc < n) X[(int64_t)row * n + cc] = make_double2(0, 0);
+  }
+  for (int I = J; I < n; I += TRB) {
+    const int row = I + r;
+    const bool live = row < n;
+    double br = (live && row == col) ? 1.0 : 0.0, bi = 0.0;
+    for (int k0 = J; k0 < I; k0 += TRB_KC) {
+      const int kc = (I - k0) < TRB_KC ? (I - k0) : TRB_KC;
+      for (int t = tid; t < TRB * TRB_KC; t += 256) {
+        const int rr = t / TRB_KC, kk = t - rr * TRB_KC;
+        lp[rr][kk] = (I + rr < n && kk < kc) ? L[(int64_t)(I + rr) * n + k0 + kk]
+                                             : make_double2(0, 0);
+      }
+      __syncthreads();
+      for (int kk = 0; kk < kc; ++kk) {
+        const double2 l = lp[r][kk], xk = xs[(k0 + kk - J) * TRB + c];
+        br -= l.x * xk.x - l.y * xk.y;
+        bi -= l.x * xk.y + l.y * xk.x;
+      }
+      __syncthreads();
+    }
+    // diagonal block: L[I+r][I+k], k < r, and the pivot 1 / L[I+r][I+r]
+    double2 lrow[TRB];
+#pragma unroll
+    for (int k = 0; k < TRB; ++k)
+      lrow[k] = (live && k < r && I + k < n) ? L[(int64_t)row * n + I + k] : make_double2(0, 0);
+    const double2 d = live ? L[(int64_t)row * n + row] : make_double2(1, 0);
+    const double den = d.x * d.x + d.y * d.y;
+    const double2 dinv = make_double2(d.x / den, -d.y / den);
+    double xr = 0, xi = 0;
+#pragma unroll
+    for (int k = 0; k < TRB; ++k) {
+      if (r == k) {
+        xr = br * dinv.x - bi * dinv.y;
+        xi = br * dinv.y + bi * dinv.x;
+      }
+      // x_k from lane k of this 16-lane group
+      const double kr = __shfl(xr, (lane & ~(TRB - 1)) | k, 64);
+      const double ki = __shfl(xi, (lane & ~(TRB - 1)) | k, 64);
+      if (r > k) {
+        br -= lrow[k].x * kr - lrow[k].y * ki;
+        bi -= lrow[k].x * ki + lrow[k].y * kr;
+      }
+    }
+    if (live) {
+      xs[(row - J) * TRB + c] = make_double2(xr, xi);
+      if (col < n) X[(int64_t)row * n + col] = make_double2(xr, xi);
+    }
+    __syncthreads();
+  }
+}
+
 // X = L^{-1} (lower); block j solves L x = e_j by forward substitution
 constexpr int TT = 256;
 __global__ __launch_bounds__(TT) void trtri_lower_kernel(int n, const double2 *L, double2 *X) {
@@ -465,10 +703,20 @@ extern "C" int ptn_potrf_lower(void *stream, int32_t n, const void *G, int64_t l
     return n > 1024 ? PTN_ERANGE : PTN_EINVAL;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(potrf_blocked_kernel, dim3(1), dim3(PT), 0, s, n,
-                     static_cast<const double2 *>(G), ldg, static_cast<double2 *>(L),
-                     shift_factor, min_pivot, static_cast<int *>(status));
-  PTN_LAUNCH_CHECK();
+  constexpr int cb = 16;
+  if (n <= cb) {
+    hipLaunchKernelGGL(potrf_blocked_kernel, dim3(1), dim3(PT), 0, s, n,
+                       static_cast<const double2 *>(G), ldg, static_cast<double2 *>(L),
+                       shift_factor, min_pivot, static_cast<int *>(status));
+    PTN_LAUNCH_CHECK();
+    return PTN_OK;
+  }
+  for (int J = 0; J < n; J += cb) {
+    hipLaunchKernelGGL(potrf_left_kernel<cb>, dim3((n - J + cb - 1) / cb), dim3(256), 0, s, n, J,
+                       static_cast<const double2 *>(G), ldg, static_cast<double2 *>(L),
+                       shift_factor, min_pivot, static_cast<int *>(status));
+    PTN_LAUNCH_CHECK();
+  }
   return PTN_OK;
 }
 
@@ -478,6 +726,12 @@ extern "C" int ptn_trtri_lower(void *stream, int32_t n, const void *L, void *X) 
     return n > 1024 ? PTN_ERANGE : PTN_EINVAL;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n <= TRB_MAXN) {
+    hipLaunchKernelGGL(trtri_blocked_kernel, dim3((n + TRB - 1) / TRB), dim3(256), 0, s, n,
+                       static_cast<const double2 *>(L), static_cast<double2 *>(X));
+    PTN_LAUNCH_CHECK();
+    return PTN_OK;
+  }
   hipLaunchKernelGGL(trtri_lower_kernel, dim3(n), dim3(TT), 0, s, n,
                      static_cast<const double2 *>(L), static_cast<double2 *>(X));
   PTN_LAUNCH_CHECK();

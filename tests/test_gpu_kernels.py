@@ -269,10 +269,13 @@ def test_cholesky_qr3_flags_rank_deficient(gpu):
     assert not ok
 
 
-def test_potrf_trtri(gpu):
-    _say("potrf/trtri")
-    rng = np.random.default_rng(63)
-    a = _crand(rng, (300, 200))
+@pytest.mark.parametrize("n", [200, 17, 256, 512, 600])
+def test_potrf_trtri(gpu, n):
+    """Cholesky factor and its inverse (row-blocked inverse up to n = 512, one
+    workgroup per column above) against numpy; X is exactly lower triangular."""
+    _say(f"potrf/trtri n={n}")
+    rng = np.random.default_rng(63 + n)
+    a = _crand(rng, (n + 100, n))
     g = a.conj().T @ a
     status = torch.zeros(1, dtype=torch.int32, device="cuda")
     L, X = S._potrf(_dev(g), 0.0, 0.0, status)
@@ -280,7 +283,9 @@ def test_potrf_trtri(gpu):
     assert status.item() == 0
     np.testing.assert_allclose(L @ L.conj().T, g, atol=1e-10 * np.abs(g).max())
     assert np.abs(np.triu(L, 1)).max() == 0
-    np.testing.assert_allclose(X @ L, np.eye(200), atol=1e-11)
+    assert np.abs(np.triu(X, 1)).max() == 0
+    np.testing.assert_allclose(X @ L, np.eye(n), atol=1e-11)
+    np.testing.assert_allclose(X, np.linalg.inv(L), atol=1e-11 * np.abs(X).max())
 
 
 def _layout(rng, rows, cols, colmajor, dtype):
