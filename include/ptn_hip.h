@@ -212,7 +212,10 @@ int ptn_krylov_orthogonalize(void *stream, int64_t n, const void *Q, int64_t ldq
  * q_{j+1} = w / ||w|| (time_evolution/time_evolution.py:818) written to
  * `qnext` (n complex128; not written when NULL; zero when ||w|| = 0), so the
  * next H_eff application can be queued before the host reads `out`.  Short
- * vectors (n <= 8192) take one single-workgroup launch.                     */
+ * vectors (n <= 8192) take one single-workgroup launch.  With `qnext` given,
+ * w's content on return is unspecified: for bases of up to 16 vectors the
+ * second projection is written straight into qnext, with beta_sq =
+ * ||w'||^2 - ||h2||^2 (w' after the first pass; Q orthonormal).            */
 int ptn_krylov_orthonormalize(void *stream, int64_t n, const void *Q, int64_t ldq,
                               int32_t nvec, void *w, void *qnext, void *out, void *work);
 

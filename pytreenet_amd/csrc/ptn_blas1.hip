@@ -360,9 +360,9 @@ __global__ __launch_bounds__(RT) void subtract_dots(int64_t n, const double2 *Q,
   __shared__ double2 hs[FUSE_V];
   for (int v = threadIdx.x; v < nvec; v += RT) hs[v] = h[v];
   __syncthreads();
-  double2 acc[FUSE_V];
+  double2 acc[FUSE_V + 1];
 #pragma unroll
-  for (int v = 0; v < FUSE_V; ++v) acc[v] = make_double2(0, 0);
+  for (int v = 0; v <= FUSE_V; ++v) acc[v] = make_double2(0, 0);
   const int64_t stride = (int64_t)gridDim.x * RT;
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
     double2 wi = w[i];
@@ -387,8 +387,54 @@ __global__ __launch_bounds__(RT) void subtract_dots(int64_t n, const double2 *Q,
         acc[v].y += t.y;
       }
     }
+    acc[FUSE_V].x += wi.x * wi.x + wi.y * wi.y;
   }
-  block_reduce_store<FUSE_V>(acc, nvec, partial + (int64_t)blockIdx.x * pstride);
+  // |w'|^2 lands in column nvec, next to the dots
+#pragma unroll
+  for (int v = 0; v < FUSE_V; ++v)
+    if (v == nvec) acc[v] = acc[FUSE_V];
+  block_reduce_store<FUSE_V + 1>(acc, nvec + 1, partial + (int64_t)blockIdx.x * pstride);
+}
+
+// After subtract_dots: h2[0..nvec) = Q^H w', h2[nvec] = (|w'|^2, 0).  The second
+// projection removes exactly |h2|^2 from |w'|^2 (Q orthonormal), so
+// beta^2 = |w'|^2 - |h2|^2 is known before w'' = w' - Q h2 is formed; out and
+// inv = 1 / beta as finish_kernel writes them.
+__global__ __launch_bounds__(RT) void finish_pyth_kernel(int nvec, const double2 *h1,
+                                                         const double2 *h2, double2 *out,
+                                                         double2 *inv) {
+  if (threadIdx.x == 0) {
+    double bsq = h2[nvec].x;
+    for (int v = 0; v < nvec; ++v) bsq -= h2[v].x * h2[v].x + h2[v].y * h2[v].y;
+    if (!(bsq > 0.0)) bsq = 0.0;
+    out[nvec] = make_double2(h1[nvec].x, bsq);
+    *inv = make_double2(bsq > 0.0 ? 1.0 / sqrt(bsq) : 0.0, 0.0);
+  }
+  for (int v = threadIdx.x; v < nvec; v += RT)
+    out[v] = make_double2(h1[v].x + h2[v].x, h1[v].y + h2[v].y);
+}
+
+// qnext = (w - sum_v h[v] Q[v]) * inv: the second subtraction and the
+// normalisation in one sweep (w itself is left as it was)
+__global__ __launch_bounds__(RT) void subtract_scale(int64_t n, const double2 *Q, int64_t ldq,
+                                                     int nvec, const double2 *h,
+                                                     const double2 *w, const double2 *inv,
+                                                     double2 *qnext) {
+  __shared__ double2 hs[FUSE_V];
+  for (int v = threadIdx.x; v < nvec; v += RT) hs[v] = h[v];
+  __syncthreads();
+  const double c = inv->x;
+  const int64_t stride = (int64_t)gridDim.x * RT;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
+    double2 wi = w[i];
+    for (int v = 0; v < nvec; ++v) {
+      const double2 q = Q[(int64_t)v * ldq + i];
+      const double2 hv = hs[v];
+      wi.x -= hv.x * q.x - hv.y * q.y;
+      wi.y -= hv.x * q.y + hv.y * q.x;
+    }
+    qnext[i] = make_double2(c * wi.x, c * wi.y);
+  }
 }
 
 __device__ __forceinline__ void combine_body(int64_t n, const double2 *Q, int64_t ldq, int nvec,
@@ -617,8 +663,17 @@ extern "C" int ptn_krylov_orthonormalize(void *stream, int64_t n, const void *Qv
     hipLaunchKernelGGL(subtract_dots, dim3(nb), dim3(RT), 0, s, n, Q, ldq, nvec, h1, w, partial,
                        pstride);
     PTN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_cols, dim3(nvec), dim3(RT), 0, s, partial, pstride, nb, h2);
+    hipLaunchKernelGGL(reduce_cols, dim3(nvec + 1), dim3(RT), 0, s, partial, pstride, nb, h2);
     PTN_LAUNCH_CHECK();
+    if (qnext) {
+      // beta from |w'|^2 - |h2|^2, then w'' and its normalisation in one sweep
+      hipLaunchKernelGGL(finish_pyth_kernel, dim3(1), dim3(RT), 0, s, nvec, h1, h2, out, inv);
+      PTN_LAUNCH_CHECK();
+      hipLaunchKernelGGL(subtract_scale, dim3(nb), dim3(RT), 0, s, n, Q, ldq, nvec, h2, w, inv,
+                         qnext);
+      PTN_LAUNCH_CHECK();
+      return PTN_OK;
+    }
     hipLaunchKernelGGL(subtract_proj, dim3(nb), dim3(RT), 0, s, n, Q, ldq, nvec, h2, w, partial,
                        pstride);
     PTN_LAUNCH_CHECK();

@@ -120,9 +120,12 @@ def test_krylov_orthogonalize(gpu, n, nvec):
     np.testing.assert_allclose(tw.cpu().numpy(), w2, atol=1e-10 * np.linalg.norm(w))
 
 
-@pytest.mark.parametrize("n,nvec,pad", [(4, 2, 0), (8192, 9, 3), (8193, 3, 0), (100000, 11, 5)])
+@pytest.mark.parametrize("n,nvec,pad", [(4, 2, 0), (8192, 9, 3), (8193, 3, 0), (100000, 11, 5),
+                                         (100000, 16, 0), (70000, 40, 2)])
 def test_krylov_orthonormalize(gpu, n, nvec, pad):
-    """One-workgroup (n <= 8192) and multi-block forms, basis rows with stride > n."""
+    """One-workgroup (n <= 8192) and multi-block forms (three sweeps with beta from
+    |w'|^2 - |h2|^2 for bases up to 16 vectors, four sweeps above), basis rows with
+    stride > n."""
     _say(f"krylov orthonormalize n={n} nvec={nvec} pad={pad}")
     import ctypes
     from pytreenet_amd import _lib
@@ -150,7 +153,8 @@ def test_krylov_orthonormalize(gpu, n, nvec, pad):
     np.testing.assert_allclose(h[:nv], h1 + h2, atol=1e-12 * scale)
     assert abs(h[nv].real - scale ** 2) < 1e-12 * scale ** 2
     assert abs(h[nv].imag - np.vdot(w2, w2).real) < 1e-12 * scale ** 2
-    np.testing.assert_allclose(tw.cpu().numpy(), w2, atol=1e-12 * scale)
+    # (w's content on return is left unspecified when qnext is given: the
+    # multi-block form writes the second projection straight into qnext)
     got = tQ.cpu().numpy()
     np.testing.assert_allclose(got[nv, :n], w2 / np.linalg.norm(w2), atol=1e-12)
     np.testing.assert_array_equal(got[:nv], Qbuf[:nv])       # basis untouched
