@@ -11,13 +11,15 @@ krylov_tol = 1e-10.  One step = forward + backward sweep.
 Runs from the same state the FP64 path (complex128, FP64 MFMA), the relaxed
 path (FP64 except the Krylov applications of H_eff whose error can change the
 local result by at most RELAX_TOL relative to its input; those run as 3M FP32
-MFMA or rocBLAS cgemm on complex64 copies of the environments) and the mixed path (complex64 environments, 3M FP32 MFMA;
-Krylov/QR/SVD FP64).  Each reduced-precision path reports its energy error
-against FP64 at the same time.  ``value`` is the sweeps/s of the fastest path
-with |dE| <= 1e-8 (FP64 meets it by definition).  The relaxed path with the
-bf16 level is run and reported beside them.
+MFMA on complex64 copies of the environments) and the mixed path (complex64
+environments, 3M FP32 MFMA; Krylov/QR/SVD FP64).  Each reduced-precision path
+reports its errors against FP64 at the same time (energy, <Z_i> of every site,
+infidelity).  ``value`` is the sweeps/s of the fastest path within 1e-8 of the
+FP64 run on all three (FP64 meets the bar by definition).  The relaxed path
+with the bf16 level is run and reported beside them.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--chi 256] [--no-cpu]
+                    [--c4-instances N] [--c4-group G] [--c4-single S] [--launch-log PATH]
 """
 from __future__ import annotations
 
@@ -38,13 +40,13 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 # dispatch of the relaxed path, averaged with the timed steps' shape mix (the
 # per-shape table is under profiles/); None where not measured
 TRAFFIC_BYTES_PER_LAUNCH = {
-    # 14 x (131072, 1280, 256) at 6.34 GB (3 of them with strided stores at 11.6 GB),
-    # 12 x (131072, 256, 1280) at 4.82 GB, 40 x (131072, 256, 256) at 1.30 GB and
-    # 28 x (8192, 1280, 256) at 0.36 GB per step: 1.69x the 1.41 GB of operands
-    # read once and C written once
-    ("gather", "fp64"): 2.39e9, ("blas", "fp64"): None,
-    # 28 x K=1280 at 2.35 GB and 56 x K=256 at 0.55 GB per step
-    ("gather", "fp32"): 1.15e9, ("blas", "fp32"): None,
+    # relaxed path's timed mix per step: 12 x (65536, 512, 512) accumulating at
+    # 2.66 GB, 14 x (65536, 512, 512) at 2.13 GB, 16 x (131072, 256, 256) at 1.29 GB:
+    # 1.96 GB per launch, 1.59x the 1.23 GB of operands read once, C written once
+    # (and read once where the launch accumulates)
+    ("gather", "fp64"): 1.96e9, ("blas", "fp64"): None,
+    # 28 x (65536, 512, 512) accumulating at 1.63 GB and 14 non-accumulating at 1.27 GB
+    ("gather", "fp32"): 1.51e9, ("blas", "fp32"): None,
     ("gather", "bf16"): None}
 KERNEL_NAME = {("gather", "fp64"): "gemm_direct_f64_kernel<LARGE=1> (ptn_gemm_gather, FP64 3M)",
                ("blas", "fp64"): "rocblas_zgemm (ptn_gemm_strided)",
@@ -214,30 +216,42 @@ def c4_instances(count: int, seed0: int = 1000):
     return algs
 
 
-def run_c4(count: int):
+def run_c4(count: int, group=None, single=None):
     """Instance-steps/s of the batching configuration on ``count`` instances: the
     first step (bonds grow from 8 to 128) and the second (at chi_max), batched in
-    lockstep and one instance at a time from the same states."""
+    lockstep (in lockstep groups of ``group`` instances when given) and one
+    instance at a time from the same states (on the first ``single`` of them when
+    given: that rate does not depend on the number of instances)."""
     import torch
     import pytreenet_amd as ptn
     out = {"instances": count}
+    if group:
+        out["lockstep_group"] = group
     for mode in ("batched", "one_at_a_time"):
-        algs = c4_instances(count)
+        n = count if mode == "batched" or not single else min(single, count)
+        algs = c4_instances(n)
         torch.cuda.synchronize()
         for step in range(2):
             t0 = time.perf_counter()
             if mode == "batched":
-                ptn.BatchedTDVP(algs).run(1)
+                bt = ptn.BatchedTDVP(algs)
+                bt.lockstep_group = group
+                bt.progress = lambda done, kind: (
+                    log(f"[c4 batched] {done} lockstep points, {time.perf_counter() - t0:.0f}s")
+                    if done % 50 == 0 else None)
+                bt.run(1)
             else:
                 for a in algs:
                     a.run_one_time_step()
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             key = "growing" if step == 0 else "at_chi_max"
-            out[f"{mode}_{key}_instance_steps_per_s"] = count / dt
-            log(f"[c4 {mode}] step {step}: {dt:.2f}s")
+            out[f"{mode}_{key}_instance_steps_per_s"] = n / dt
+            log(f"[c4 {mode}] step {step}: {dt:.2f}s for {n} instances")
         del algs
         torch.cuda.empty_cache()
+    if single and single < count:
+        out["one_at_a_time_instances"] = min(single, count)
     out["speedup_at_chi_max"] = (out["batched_at_chi_max_instance_steps_per_s"] /
                                  out["one_at_a_time_at_chi_max_instance_steps_per_s"])
     return out
@@ -299,6 +313,11 @@ def main():
     ap.add_argument("--c4-instances", type=int, default=16,
                     help="instances of the batching configuration measured beside the "
                          "bench (0: skip)")
+    ap.add_argument("--c4-group", type=int, default=None,
+                    help="lockstep group size of the batching configuration's batched run")
+    ap.add_argument("--c4-single", type=int, default=None,
+                    help="instances of the batching configuration run one at a time "
+                         "(default: all)")
     ap.add_argument("--relax-low", choices=("fp32", "bf16x6"), default="fp32",
                     help="arithmetic of the relaxed paths' FP32-accurate level: the FP32 "
                          "MFMA, or six products of three-piece bf16 splits on the bf16 MFMA")
@@ -436,7 +455,7 @@ def main():
         E.clear_caches()
         torch.cuda.empty_cache()
         E.PROFILER.phase = "batching-config"
-        line["batched_config"] = run_c4(args.c4_instances)
+        line["batched_config"] = run_c4(args.c4_instances, args.c4_group, args.c4_single)
     if roofline is not None:
         # every LARGE launch of the dominant (backend, precision) in this process: the
         # set rocprofv3 --stats averages over for that kernel
