@@ -427,11 +427,18 @@ __global__ __launch_bounds__(RT) void subtract_scale(int64_t n, const double2 *Q
   const int64_t stride = (int64_t)gridDim.x * RT;
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
     double2 wi = w[i];
-    for (int v = 0; v < nvec; ++v) {
-      const double2 q = Q[(int64_t)v * ldq + i];
-      const double2 hv = hs[v];
-      wi.x -= hv.x * q.x - hv.y * q.y;
-      wi.y -= hv.x * q.y + hv.y * q.x;
+    // all of an element's basis values are requested before the first is used
+    double2 q[FUSE_V];
+#pragma unroll
+    for (int v = 0; v < FUSE_V; ++v)
+      q[v] = v < nvec ? Q[(int64_t)v * ldq + i] : make_double2(0, 0);
+#pragma unroll
+    for (int v = 0; v < FUSE_V; ++v) {
+      if (v < nvec) {
+        const double2 hv = hs[v];
+        wi.x -= hv.x * q[v].x - hv.y * q[v].y;
+        wi.y -= hv.x * q[v].y + hv.y * q[v].x;
+      }
     }
     qnext[i] = make_double2(c * wi.x, c * wi.y);
   }
