@@ -88,6 +88,15 @@ typedef struct ptn_gemm_desc {
 
 int ptn_gemm_gather(void *stream, const ptn_gemm_desc *desc);
 
+/* descs[0..count) issued in order on `stream`, each exactly as by
+ * ptn_gemm_gather: one call for a run of pairwise contractions, the
+ * consecutive np.tensordot calls of one network contraction in the reference
+ * (contractions/state_operator_contraction.py:227-273 and :407-694, the H_eff
+ * and environment networks replayed inside the Krylov loop and the sweeps).
+ * Stops at the first failing descriptor and returns its error, whose message
+ * names the descriptor's index; the launches before it stay issued.        */
+int ptn_gemm_gather_many(void *stream, const ptn_gemm_desc *descs, int32_t count);
+
 /* The same contraction on a kernel configuration chosen by the caller (the
  * project's autotuner times the candidates once per shape and replays the
  * fastest).  No counterpart in the reference, whose np.tensordot has no

@@ -18,7 +18,7 @@ PTN_GEMM_RULE, PTN_GEMM_WAVE, PTN_GEMM_TILE, PTN_GEMM_SKINNY = 0, 1, 2, 3
 PTN_OK, PTN_EINVAL, PTN_EHIP, PTN_ESOLVER, PTN_ENOCONV, PTN_ERANGE = 0, -1, -2, -3, -4, -5
 
 EXPORTED = (
-    "ptn_gemm_gather", "ptn_gemm_gather_config", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
+    "ptn_gemm_gather", "ptn_gemm_gather_many", "ptn_gemm_gather_config", "ptn_permute", "ptn_fill", "ptn_krylov_work_bytes",
     "ptn_krylov_orthogonalize", "ptn_krylov_orthonormalize",
     "ptn_krylov_orthonormalize_noalloc", "ptn_normalize",
     "ptn_krylov_combine", "ptn_norm_sq", "ptn_dotc", "ptn_krylov_small_work_bytes",
@@ -85,6 +85,7 @@ def load(path: os.PathLike | None = None) -> ctypes.CDLL:
     vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
     sig = {
         "ptn_gemm_gather": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc)]),
+        "ptn_gemm_gather_many": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc), i32]),
         "ptn_gemm_gather_config": (ctypes.c_int, [vp, ctypes.POINTER(GemmDesc), i32, i32, i32,
                                                   i32]),
         "ptn_gemm_strided": (ctypes.c_int, [vp, ctypes.POINTER(StridedDesc)]),

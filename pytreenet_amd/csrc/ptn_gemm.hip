@@ -27,6 +27,8 @@
 // sharing an A row panel share one L2.
 #include "ptn_common.h"
 
+#include <stdio.h>
+
 #include <vector>
 
 namespace {
@@ -1769,6 +1771,25 @@ extern "C" int ptn_gemm_gather_config(void *stream, const ptn_gemm_desc *d, int3
   if (d->a_dtype == PTN_C128)
     return dispatch_b<PTN_COMPUTE_F32_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
   return dispatch_b<PTN_COMPUTE_F32_3M, float2>(s, a, tiles, d->b_dtype, d->c_dtype, ks);
+}
+
+extern "C" const char *ptn_last_error(void);
+
+extern "C" int ptn_gemm_gather_many(void *stream, const ptn_gemm_desc *descs, int32_t count) {
+  if (count < 0 || (count > 0 && !descs)) {
+    ptn::set_error("ptn_gemm_gather_many: bad arguments (count=%d)", count);
+    return PTN_EINVAL;
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    const int rc = ptn_gemm_gather(stream, descs + i);
+    if (rc != PTN_OK) {
+      char msg[400];
+      snprintf(msg, sizeof(msg), "%s", ptn_last_error());
+      ptn::set_error("ptn_gemm_gather_many: descriptor %d: %s", i, msg);
+      return rc;
+    }
+  }
+  return PTN_OK;
 }
 
 extern "C" int64_t ptn_gemm_grouped_work_bytes(int32_t count) {

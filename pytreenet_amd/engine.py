@@ -1403,31 +1403,85 @@ class CompiledNetwork:
         if self.ops is None or sig != self.sig:
             return self._record(var, sig)
         out = torch.empty(self.out_shape, dtype=self.out_dtype, device=var[0].device)
-        for op, roles in zip(self.ops, self.op_roles):
-            kind = op[0]
-            if kind == "gemm" or kind == "blas":
-                d = op[1]
-                for field, role in roles:
-                    t = var[role[1]] if role[0] == "var" else out
-                    setattr(d, field, t.data_ptr() + role[2])
-                if kind == "gemm":
-                    _run_gemm(d, op[5], op[6], op[7], op[8])
-                else:
-                    _run_blas(d, op[5], op[6], op[7], op[8])
-            elif kind == "block":
-                Y = op[2] if roles[0] is None else (var[roles[0][1]] if roles[0][0] == "var"
-                                                    else out)
-                T = op[3] if roles[1] is None else (var[roles[1][1]] if roles[1][0] == "var"
-                                                    else out)
-                _run_block(op[1], Y, T)
-            else:
-                if roles[0] is None:
-                    T = op[1]
-                else:
-                    e0 = roles[0][2] // out.element_size()
-                    T = out.view(-1).narrow(0, e0, op[1].numel())
-                fill_(T, op[2])
+        if AUTOTUNE:
+            for op, roles in zip(self.ops, self.op_roles):
+                self._replay_op(op, roles, var, out)
+            return out
+        prof = PROFILER
+        for item in self.runs:
+            if item[0] == "op":
+                self._replay_op(self.ops[item[1]], self.op_roles[item[1]], var, out)
+                continue
+            _, arr, count, flops, patches = item
+            for view, field, role in patches:
+                t = var[role[1]] if role[0] == "var" else out
+                setattr(view, field, t.data_ptr() + role[2])
+            if prof.active:
+                prof.flops += flops
+                prof.launches += count
+            _lib.check(_lib.load().ptn_gemm_gather_many(ctypes.c_void_p(stream_handle()), arr,
+                                                        count), "ptn_gemm_gather_many")
         return out
+
+    @staticmethod
+    def _replay_op(op, roles, var, out) -> None:
+        kind = op[0]
+        if kind == "gemm" or kind == "blas":
+            d = op[1]
+            for field, role in roles:
+                t = var[role[1]] if role[0] == "var" else out
+                setattr(d, field, t.data_ptr() + role[2])
+            if kind == "gemm":
+                _run_gemm(d, op[5], op[6], op[7], op[8])
+            else:
+                _run_blas(d, op[5], op[6], op[7], op[8])
+        elif kind == "block":
+            Y = op[2] if roles[0] is None else (var[roles[0][1]] if roles[0][0] == "var"
+                                                else out)
+            T = op[3] if roles[1] is None else (var[roles[1][1]] if roles[1][0] == "var"
+                                                else out)
+            _run_block(op[1], Y, T)
+        else:
+            if roles[0] is None:
+                T = op[1]
+            else:
+                e0 = roles[0][2] // out.element_size()
+                T = out.view(-1).narrow(0, e0, op[1].numel())
+            fill_(T, op[2])
+
+    def _build_runs(self) -> None:
+        """Group the recorded gather-GEMMs into runs replayed by one
+        ptn_gemm_gather_many call each (descriptors copied into one array; the
+        pointers of variable inputs and of the output are patched per call).
+        LARGE launches stay single, so the launch profiler can time them; block
+        applies, fills and strided library GEMMs keep their own calls."""
+        self.runs = []
+        seg = []
+
+        def flush():
+            if not seg:
+                return
+            arr = (GemmDesc * len(seg))()
+            views, patches, flops = [], [], 0.0
+            for i, k in enumerate(seg):
+                op = self.ops[k]
+                arr[i] = op[1]
+                views.append(arr[i])
+                flops += 8.0 * op[5] * op[6] * op[7]
+                for field, role in self.op_roles[k]:
+                    patches.append((views[i], field, role))
+            self.runs.append(("many", arr, len(seg), flops, patches))
+            seg.clear()
+
+        for k, op in enumerate(self.ops):
+            if op[0] == "gemm" and gemm_kernel_class(
+                    op[5], op[6], op[7], op[8], bool(op[1].a_kmajor),
+                    bool(op[1].c_mmajor)) != "large":
+                seg.append(k)
+            else:
+                flush()
+                self.runs.append(("op", k))
+        flush()
 
     def release(self) -> None:
         """Drop the recording and the intermediates it keeps (a later call records
@@ -1436,6 +1490,7 @@ class CompiledNetwork:
         self.sig = None
         self.rec_inputs = self.rec_out = None
         self.op_roles = None
+        self.runs = None
 
     def _record(self, var, sig) -> torch.Tensor:
         global _RECORDER
@@ -1490,6 +1545,7 @@ class CompiledNetwork:
                 self.op_roles.append((role(op[2]), role(op[3])))
             else:
                 self.op_roles.append((role(op[1]),))
+        self._build_runs()
         return out
 
 

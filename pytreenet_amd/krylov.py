@@ -17,7 +17,10 @@ device->host copy of the (j+3) scalars per step feeds the host-side m x m
 propagator (m <= max dim, FP64), as in the reference.  For short vectors
 (n <= LOOKAHEAD_N) the next application is queued before that copy is read:
 if the stop rule then ends the recurrence, that one application is discarded
--- the accepted q_0..q_{m-1} and T_m are the same either way.  The two-pass CGS
+-- the accepted q_0..q_{m-1} and T_m are the same either way.  Longer vectors
+queue it only when the last two Saad estimates make a stop at this step
+unlikely (``_stop_unlikely``), since a discarded application costs more there
+than the device idling while the host reads the scalars.  The two-pass CGS
 and the reference's two-pass MGS differ only in rounding.
 """
 from __future__ import annotations
@@ -308,6 +311,26 @@ def resolve_pending(defer: _SmallDefer) -> List[int]:
 
 LOOKAHEAD_N = 1 << 16
 MAXV_PER_CALL = 128      # basis vectors per ptn_krylov_orthonormalize call
+# longer vectors: the next application is queued before the scalars of a step are
+# read when the estimate, extrapolated SPECULATE_STEPS steps at the rate of the
+# last two, stays above krylov_tol (0 switches this off)
+SPECULATE_STEPS = 2
+# a speculative reduced-precision application must satisfy the relaxed rule with
+# this much to spare (its weight uses the previous step's ||T||)
+SPECULATE_MARGIN = 4.0
+
+
+def _stop_unlikely(est: float, est_prev: float, krylov_tol: float) -> bool:
+    """Whether the Saad estimate of the step being finished, predicted from the
+    previous two, stays above krylov_tol (the recurrence then continues)."""
+    if SPECULATE_STEPS <= 0:
+        return False
+    if krylov_tol <= 0.0:
+        return True
+    if not (math.isfinite(est) and math.isfinite(est_prev)) or est_prev <= 0.0:
+        return False
+    rate = min(est / est_prev, 1.0)
+    return est * rate ** SPECULATE_STEPS > krylov_tol
 
 
 def _orthonormalize_chunked(lib, stream, n: int, Q: torch.Tensor, nvec: int, w: torch.Tensor,
@@ -406,11 +429,17 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
         low = relaxed.low
         tau = abs(complex(exp_factor) * time_difference)
     est = math.inf          # Saad estimate of the latest step
+    est_prev = math.inf     # and of the step before
     t_norm = 0.0            # largest column sum of |T| so far
 
-    def apply(k):
+    def apply(k, speculative=False):
+        """H_eff q_k; ``speculative``: queued before the scalars of step k-1 are
+        read -- a relaxed application then runs in reduced precision only if the
+        rule holds with SPECULATE_MARGIN to spare, and None is returned when it
+        would have to run in full precision (the caller then waits and decides
+        on the newer estimate)."""
         if low is not None and k >= 2:
-            weight = tau * est * t_norm
+            weight = tau * est * t_norm * (SPECULATE_MARGIN if speculative else 1.0)
             net = None
             if relaxed.lower is not None and weight * relaxed.lower_eps <= \
                     relaxed.relax_tol * beta0:
@@ -421,8 +450,10 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
             if net is not None:
                 STATS.low_applies += 1
                 # complex64 input: every product of the reduced-precision network
-                # then has complex64 operands (strided FP32 ones run as rocBLAS cgemm)
+                # then has complex64 operands
                 return net(Q[k].view(shape).to(torch.complex64))
+            if speculative:
+                return None
         return action(Q[k].view(shape))
 
     for j in range(max_steps):
@@ -451,8 +482,11 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
             _orthonormalize_chunked(lib, stream, n, Q, j + 1, w, packed, work)
         host_buf.copy_(packed[: j + 3], non_blocking=True)
         done.record()
-        if lookahead and j + 1 < max_steps:
-            w_next = apply(j + 1)
+        if j + 1 < max_steps:
+            if lookahead:
+                w_next = apply(j + 1)
+            elif _stop_unlikely(est, est_prev, krylov_tol):
+                w_next = apply(j + 1, speculative=True)
         done.synchronize()
         host = host_buf.numpy()
         if j == 0:
@@ -468,7 +502,7 @@ def krylov_exp_action(psi: torch.Tensor, action: Callable[[torch.Tensor], torch.
             break
         if krylov_tol > 0.0 or low is not None:
             col = small_propagator(hess[:m, :m], time_difference, exp_factor, hermitian)
-            est = beta0 * beta * abs(col[m - 1])
+            est_prev, est = est, beta0 * beta * abs(col[m - 1])
             t_norm = max(t_norm, float(np.abs(hess[: j + 2, j]).sum()))
             if krylov_tol > 0.0 and est < krylov_tol:
                 break

@@ -125,3 +125,45 @@ def test_block_apply_staged_matches_unstaged_and_dense(gpu, dtype, layout):
     dense = np.einsum(f"{layout},wqp->abq", Y.cpu().numpy().astype(complex), w)
     tol = 1e-13 if dtype == torch.complex128 else 1e-6
     np.testing.assert_allclose(got.cpu().numpy(), dense, rtol=0, atol=tol * np.abs(dense).max())
+
+
+def test_network_replay_runs_as_one_native_call(gpu):
+    """A recorded network's small gather-GEMMs replay through ptn_gemm_gather_many
+    (one call per run of launches): the result equals the per-launch replay and a
+    fresh contraction, call after call with new inputs and output buffers; a bad
+    descriptor inside a run is reported with its index."""
+    import ctypes
+    from pytreenet_amd import _lib
+    print("[engine] replay through ptn_gemm_gather_many", flush=True)
+    rng = np.random.default_rng(6)
+    labels = [["a", "b", "p"], ["a", "w", "A"], ["b", "v", "B"], ["w", "v", "q", "p"]]
+    out = ["A", "B", "q"]
+    envs = [_dev(_crand(rng, (9, 4, 7))), _dev(_crand(rng, (6, 3, 5)))]
+    W = _dev(_crand(rng, (4, 3, 2, 2)))
+    x = _dev(_crand(rng, (9, 6, 2)))
+    net = E.CompiledNetwork([x, envs[0], envs[1], W], labels, out)
+    net(x)
+    runs = [r for r in net.runs if r[0] == "many"]
+    assert runs and sum(r[2] for r in runs) >= 2
+    for rep in range(3):
+        y = _dev(_crand(rng, (9, 6, 2)))
+        res = net(y)
+        got = res.cpu().numpy()
+        ref = E.contract_network([y, envs[0], envs[1], W], labels, out).cpu().numpy()
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+        per_op = torch.empty_like(res)
+        for op, roles in zip(net.ops, net.op_roles):
+            net._replay_op(op, roles, (y,), per_op)
+        assert np.array_equal(per_op.cpu().numpy(), got)
+    # (res and y stay alive: the run's descriptors point at them.)  The second
+    # descriptor of a two-launch run carries an unknown compute mode
+    arr = (_lib.GemmDesc * 2)()
+    arr[0] = runs[0][1][0]
+    arr[1] = runs[0][1][0]
+    arr[1].compute = 77
+    stream = ctypes.c_void_p(E.stream_handle())
+    rc = _lib.load().ptn_gemm_gather_many(stream, arr, 2)
+    torch.cuda.synchronize()
+    assert rc != 0
+    assert "descriptor 1" in _lib.load().ptn_last_error().decode()
+    assert _lib.load().ptn_gemm_gather_many(stream, arr, 0) == 0
