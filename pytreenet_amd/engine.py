@@ -224,6 +224,7 @@ PROFILER = LaunchProfiler()
 
 
 def clear_caches():
+    _FAST.clear()
     _CACHE.d.clear()
     _PLANS.clear()
     _AFFINE.clear()
@@ -285,6 +286,16 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
     """sum over shared labels of opA(A) * opB(B), laid out as ``out_labels``."""
     require_device(A, B)
     la, lb, lo = tuple(la), tuple(lb), tuple(out_labels)
+    fk = None
+    if out is None and not accumulate:
+        # a call that allocates its output: the launch of an earlier call with the
+        # same labels, layouts, dtypes and options is replayed from its descriptor
+        fk = (la, lb, lo, A.shape, A.stride(), A.dtype, B.shape, B.stride(), B.dtype,
+              conj_a, conj_b, precision, out_dtype, complex(alpha), A.device,
+              USE_BLAS, BLAS_FP64, _BLAS_OFF)
+        hit = _FAST.get(fk)
+        if hit is not None:
+            return _fast_launch(hit, A, B, precision)
     if len(la) != A.dim() or len(lb) != B.dim():
         raise ValueError("label count does not match tensor rank")
     if len(set(la)) != len(la) or len(set(lb)) != len(lb):
@@ -342,8 +353,31 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
                                                 (mdims, key[7]), (ndims, key[8]))]
         aff = _AFFINE[key] = (False if any(x is None for x in st)
                               else tuple((0, x) for x in st))
-    _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate, alpha,
-            aff or None, tabs, (cm and not cn, cn and not cm))
+    used = _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate,
+                   alpha, aff or None, tabs, (cm and not cn, cn and not cm))
+    if fk is not None and used is not None:
+        if len(_FAST) >= FAST_CAP:
+            _FAST.clear()
+        d, swapped, Ml, Nl = used
+        _FAST[fk] = (oshape, out.dtype, GemmDesc.from_buffer_copy(d), swapped, Ml, Nl, K, tabs)
+    return out
+
+
+# launch descriptors of earlier allocating contract() calls, by labels, layouts,
+# dtypes and options (their offset tables are held here too)
+_FAST: Dict[tuple, tuple] = {}
+FAST_CAP = 4096
+
+
+def _fast_launch(hit, A: torch.Tensor, B: torch.Tensor, precision: str) -> torch.Tensor:
+    oshape, odt, proto, swapped, M, N, K, tabs = hit
+    out = torch.empty(oshape, dtype=odt, device=A.device)
+    X, Y = (B, A) if swapped else (A, B)
+    d = GemmDesc.from_buffer_copy(proto)
+    d.A, d.B, d.C = X.data_ptr(), Y.data_ptr(), out.data_ptr()
+    if _RECORDER is not None:
+        _RECORDER.ops.append(("gemm", d, X, Y, out, M, N, K, precision, tabs))
+    _run_gemm(d, M, N, K, precision)
     return out
 
 
@@ -440,7 +474,9 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     rocBLAS GEMM.  Otherwise ptn_gemm_gather runs, with the operand of the
     longer free dimension on the M side (64-row tiles waste less on a short N
     than on a short M).  ``c_major`` tells whether consecutive m (n) are
-    contiguous in C: a layout hint for the store order of small products."""
+    contiguous in C: a layout hint for the store order of small products.
+    Returns (descriptor, swapped, M, N) of a gather launch (``swapped``: A and B
+    exchanged their roles), None for a strided library GEMM."""
     if affine is not None:
         hit = _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine)
         if hit is not None:
@@ -448,8 +484,10 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
             if _RECORDER is not None:
                 _RECORDER.ops.append(("blas", sd, A, B, C, M, N, K, precision, offs))
             _run_blas(sd, M, N, K, precision)
-            return
+            return None
+    swapped = False
     if M < N and N >= 64:
+        swapped = True
         A, B = B, A
         M, N = N, M
         kmaj_a, kmaj_b = kmaj_b, kmaj_a
@@ -481,6 +519,7 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     if _RECORDER is not None:
         _RECORDER.ops.append(("gemm", d, A, B, C, M, N, K, precision, tabs))
     _run_gemm(d, M, N, K, precision)
+    return d, swapped, M, N
 
 
 # the row-streaming kernel's limits and the rule's threshold (``ptn_gemm.hip``)
