@@ -320,14 +320,22 @@ SPECULATE_STEPS = 2
 SPECULATE_MARGIN = 4.0
 
 
+# with fewer than two estimates, the first one must exceed krylov_tol this much
+SPECULATE_FIRST = 1e6
+
+
 def _stop_unlikely(est: float, est_prev: float, krylov_tol: float) -> bool:
     """Whether the Saad estimate of the step being finished, predicted from the
-    previous two, stays above krylov_tol (the recurrence then continues)."""
+    previous two, stays above krylov_tol (the recurrence then continues).  Before
+    the first estimate (step 0) only a breakdown could stop the recurrence; after
+    it (step 1) the first estimate has to lie SPECULATE_FIRST above krylov_tol."""
     if SPECULATE_STEPS <= 0:
         return False
-    if krylov_tol <= 0.0:
+    if krylov_tol <= 0.0 or est == math.inf:
         return True
-    if not (math.isfinite(est) and math.isfinite(est_prev)) or est_prev <= 0.0:
+    if not math.isfinite(est_prev):
+        return math.isfinite(est) and est > krylov_tol * SPECULATE_FIRST
+    if est_prev <= 0.0:
         return False
     rate = min(est / est_prev, 1.0)
     return est * rate ** SPECULATE_STEPS > krylov_tol
