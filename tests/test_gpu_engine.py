@@ -167,3 +167,37 @@ def test_network_replay_runs_as_one_native_call(gpu):
     assert rc != 0
     assert "descriptor 1" in _lib.load().ptn_last_error().decode()
     assert _lib.load().ptn_gemm_gather_many(stream, arr, 0) == 0
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_contract_replays_descriptor_of_same_layout(gpu, swap):
+    """A second allocating contract() with the same labels, layouts, dtypes and
+    options reuses the first call's launch descriptor (M < N swaps the operands'
+    roles): new data give the new product, a different layout or option a new
+    descriptor, and a recording network captures the replayed launch."""
+    print(f"[engine] contract descriptor replay swap={swap}", flush=True)
+    rng = np.random.default_rng(7 + swap)
+    sa, sb = ((30, 7, 2), (7, 2, 90)) if swap else ((90, 7, 2), (7, 2, 30))
+    E.clear_caches()
+    n0 = len(E._FAST)
+    for rep in range(3):
+        a, b = _crand(rng, sa), _crand(rng, sb)
+        got = E.contract(_dev(a), "mkp", _dev(b), "kpn", "nm").cpu().numpy()
+        ref = np.einsum("mkp,kpn->nm", a, b)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+        assert len(E._FAST) == n0 + 1
+        if swap:
+            assert next(iter(E._FAST.values()))[3]
+    # conjugation is part of the key: a new entry, the conjugated product
+    got = E.contract(_dev(a), "mkp", _dev(b), "kpn", "nm", conj_a=True).cpu().numpy()
+    ref = np.einsum("mkp,kpn->nm", a.conj(), b)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    assert len(E._FAST) == n0 + 2
+    # a network recorded through contract() replays correctly with new inputs
+    x = _dev(_crand(rng, sa))
+    net = E.CompiledNetwork([x, _dev(b)], [["m", "k", "p"], ["k", "p", "n"]], ["n", "m"])
+    net(x)
+    y = _crand(rng, sa)
+    got = net(_dev(y)).cpu().numpy()
+    ref = np.einsum("mkp,kpn->nm", y, b)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())

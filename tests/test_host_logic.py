@@ -441,3 +441,29 @@ def test_two_site_operator_tensor_and_its_split():
         assert parts is not None
         tot = sum(_einsum_network(p.tensors, p.labels, p.out_labels, p.conj) for p in parts)
         np.testing.assert_allclose(tot, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+def test_krylov_early_queueing_rule():
+    """When a long-vector Krylov solve queues the next application before reading a
+    step's scalars: always without a tolerance or before the first estimate, after
+    it only far above the tolerance, later when the estimate extrapolated two steps
+    at the rate of the last two stays above it; never when switched off."""
+    from pytreenet_amd import krylov as K
+    inf = float("inf")
+    tol = 1e-10
+    assert K._stop_unlikely(inf, inf, tol)                 # step 0
+    assert K._stop_unlikely(0.5, 0.4, 0.0)                 # no tolerance
+    assert K._stop_unlikely(1e-3, inf, tol)                # step 1, far above
+    assert not K._stop_unlikely(1e-5, inf, tol)            # step 1, within 1e6 of tol
+    # rate 1e-2: 1e-4 -> 1e-8 in two steps (above), 1e-6 -> 1e-10 (not above)
+    assert K._stop_unlikely(1e-4, 1e-2, tol)
+    assert not K._stop_unlikely(1e-6, 1e-4, tol)
+    # an estimate that grew is extrapolated flat
+    assert K._stop_unlikely(1e-9, 1e-12, tol)
+    assert not K._stop_unlikely(1e-11, 1e-12, tol)
+    old = K.SPECULATE_STEPS
+    try:
+        K.SPECULATE_STEPS = 0
+        assert not K._stop_unlikely(inf, inf, tol)
+    finally:
+        K.SPECULATE_STEPS = old
