@@ -59,7 +59,11 @@ enum {
  * the coalesced load orientation (1: consecutive k are contiguous; 0:
  * consecutive m (A) / n (B) are contiguous).  The leg permutations of the
  * contraction are thereby fused into the LDS-staged tile loads; no transposed
- * copy is written to HBM.
+ * copy is written to HBM.  c_lower = 1 marks a square product (M == N) of
+ * which only the part on and below the diagonal is read (the Gram matrix of a
+ * CholeskyQR split): the library may then skip the 64 x 64 output tiles that
+ * lie strictly above the diagonal, whose elements of C are left unwritten.
+ * The elements it writes are those of the full product bit for bit.
  *
  * Replaces every np.tensordot on the hot path:
  *   contractions/state_operator_contraction.py:227-273 (contract_ket_ham_with_envs),
@@ -72,7 +76,7 @@ typedef struct ptn_gemm_desc {
   int64_t M, N, K;
   const void *A;
   const int64_t *a_off_m, *a_off_k;
-  int32_t a_dtype, a_conj, a_kmajor, _pad0;
+  int32_t a_dtype, a_conj, a_kmajor, c_lower;
   const void *B;
   const int64_t *b_off_k, *b_off_n;
   int32_t b_dtype, b_conj, b_kmajor, _pad1;

@@ -56,7 +56,7 @@ class GemmDesc(ctypes.Structure):
         ("M", ctypes.c_int64), ("N", ctypes.c_int64), ("K", ctypes.c_int64),
         ("A", ctypes.c_void_p), ("a_off_m", ctypes.c_void_p), ("a_off_k", ctypes.c_void_p),
         ("a_dtype", ctypes.c_int32), ("a_conj", ctypes.c_int32), ("a_kmajor", ctypes.c_int32),
-        ("_pad0", ctypes.c_int32),
+        ("c_lower", ctypes.c_int32),
         ("B", ctypes.c_void_p), ("b_off_k", ctypes.c_void_p), ("b_off_n", ctypes.c_void_p),
         ("b_dtype", ctypes.c_int32), ("b_conj", ctypes.c_int32), ("b_kmajor", ctypes.c_int32),
         ("_pad1", ctypes.c_int32),

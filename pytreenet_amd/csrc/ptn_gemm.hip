@@ -74,6 +74,7 @@ struct KArgs {
   int64_t kchunk;   // K range per split (multiple of BK)
   double2 *ws;      // split-K partials [ksplit][M][N], null when gridDim.y == 1
   int aconj, akmaj, bconj, bkmaj, beta_one, remap;
+  int lower;        // tiles on and below the diagonal only (square product, tm >= tn)
   int cmaj;         // consecutive m contiguous in C (store-order hint)
 };
 
@@ -85,6 +86,15 @@ struct KArgs {
 // (5 MB for N = 1280 at K = 256, more than one XCD's 4 MB L2).
 __device__ __forceinline__ void tile_coords(const KArgs &p, int64_t bid, int64_t &tm,
                                             int64_t &tn) {
+  if (p.lower) {
+    // bid = tm (tm + 1) / 2 + tn with tn <= tm: the lower triangle row by row
+    int64_t r = (int64_t)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > bid) --r;
+    while ((r + 1) * (r + 2) / 2 <= bid) ++r;
+    tm = r;
+    tn = bid - r * (r + 1) / 2;
+    return;
+  }
   if (p.group_m > 1) {
     const int64_t span = p.group_m * p.tiles_n;
     const int64_t g = bid / span;
@@ -1066,6 +1076,7 @@ __global__ __launch_bounds__(256) void splitk_reduce(KArgs p, int ksplit) {
       im += v.y;
     }
     const int64_t m = t / p.N, n = t - m * p.N;
+    if (p.lower && m / BM < n / BN) continue;   // a skipped tile: no partials
     double ore = p.are * re - p.aim * im;
     double oim = p.are * im + p.aim * re;
     TC *dst = C + p.com[m] + p.con[n];
@@ -1625,6 +1636,7 @@ int to_kargs(const ptn_gemm_desc *d, KArgs &a, int64_t &tiles) {
     return PTN_ERANGE;
   }
   a.remap = 0;
+  a.lower = 0;
   a.cmaj = d->c_mmajor != 0;
   a.ws = nullptr;
   a.kchunk = d->K > 0 ? ((d->K + BK - 1) / BK) * BK : BK;
@@ -1681,8 +1693,16 @@ extern "C" int ptn_gemm_gather(void *stream, const ptn_gemm_desc *d) {
   if (ksplit < 1) ksplit = 1;
   a.ws = ksplit > 1 ? static_cast<double2 *>(d->ws) : nullptr;
   if (d->K == 0) a.kchunk = BK;
-  if (d->compute == PTN_COMPUTE_F64_4M)
+  if (d->compute == PTN_COMPUTE_F64_4M) {
+    // lower-triangle hint: the same tiles and K splits as the full product,
+    // launched for the tiles with tm >= tn only
+    if (d->c_lower && d->M == d->N && a.tiles_m == a.tiles_n) {
+      a.lower = 1;
+      tiles = a.tiles_m * (a.tiles_m + 1) / 2;
+      a.remap = (tiles % 8 == 0) ? 1 : 0;
+    }
     return launch<PTN_COMPUTE_F64_4M, double2, double2, double2>(s, a, tiles, ksplit);
+  }
   if (d->compute == PTN_COMPUTE_BF16_3M) {
     if (d->a_dtype == PTN_C128)
       return dispatch_b<PTN_COMPUTE_BF16_3M, double2>(s, a, tiles, d->b_dtype, d->c_dtype,

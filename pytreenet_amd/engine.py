@@ -282,8 +282,12 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
              out_labels: Sequence[Label], *, conj_a: bool = False, conj_b: bool = False,
              precision: str = FP64, out: Optional[torch.Tensor] = None,
              out_dtype: Optional[torch.dtype] = None, accumulate: bool = False,
-             alpha: complex = 1.0) -> torch.Tensor:
-    """sum over shared labels of opA(A) * opB(B), laid out as ``out_labels``."""
+             alpha: complex = 1.0, lower: bool = False) -> torch.Tensor:
+    """sum over shared labels of opA(A) * opB(B), laid out as ``out_labels``.
+
+    ``lower``: only the part on and below the diagonal of a square result (first
+    output legs x last, equal extents) will be read, e.g. a Gram matrix: the
+    64 x 64 tiles strictly above it may be left unwritten."""
     require_device(A, B)
     la, lb, lo = tuple(la), tuple(lb), tuple(out_labels)
     fk = None
@@ -291,7 +295,7 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
         # a call that allocates its output: the launch of an earlier call with the
         # same labels, layouts, dtypes and options is replayed from its descriptor
         fk = (la, lb, lo, A.shape, A.stride(), A.dtype, B.shape, B.stride(), B.dtype,
-              conj_a, conj_b, precision, out_dtype, complex(alpha), A.device,
+              conj_a, conj_b, precision, out_dtype, complex(alpha), A.device, lower,
               USE_BLAS, BLAS_FP64, _BLAS_OFF)
         hit = _FAST.get(fk)
         if hit is not None:
@@ -354,7 +358,8 @@ def contract(A: torch.Tensor, la: Sequence[Label], B: torch.Tensor, lb: Sequence
         aff = _AFFINE[key] = (False if any(x is None for x in st)
                               else tuple((0, x) for x in st))
     used = _launch(A, B, out, ptrs, M, N, K, ka, kb, conj_a, conj_b, precision, accumulate,
-                   alpha, aff or None, tabs, (cm and not cn, cn and not cm))
+                   alpha, aff or None, tabs, (cm and not cn, cn and not cm),
+                   lower and M == N and list(m_l) == list(lo[:len(m_l)]))
     if fk is not None and used is not None:
         if len(_FAST) >= FAST_CAP:
             _FAST.clear()
@@ -461,7 +466,7 @@ def _run_blas(sd: StridedDesc, M: int, N: int, K: int, precision: str):
 
 
 def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, accumulate,
-            alpha, affine=None, tabs=None, c_major=(False, False)):
+            alpha, affine=None, tabs=None, c_major=(False, False), lower=False):
     """One contraction launch from device offset tables
     ptrs = (a_off_m, a_off_k, b_off_k, b_off_n, c_off_m, c_off_n).
 
@@ -475,9 +480,10 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     longer free dimension on the M side (64-row tiles waste less on a short N
     than on a short M).  ``c_major`` tells whether consecutive m (n) are
     contiguous in C: a layout hint for the store order of small products.
+    ``lower``: only C's tiles with m >= n are needed (a square Gram product).
     Returns (descriptor, swapped, M, N) of a gather launch (``swapped``: A and B
     exchanged their roles), None for a strided library GEMM."""
-    if affine is not None:
+    if affine is not None and not lower:
         hit = _blas_desc(A, B, C, M, N, K, conj_a, conj_b, precision, accumulate, alpha, affine)
         if hit is not None:
             sd, offs = hit
@@ -514,6 +520,7 @@ def _launch(A, B, C, ptrs, M, N, K, kmaj_a, kmaj_b, conj_a, conj_b, precision, a
     alpha = complex(alpha)
     d.alpha_re, d.alpha_im = alpha.real, alpha.imag
     d.compute = _COMPUTE[precision]
+    d.c_lower = int(lower)
     ws = _splitk_workspace(A.device)
     d.ws, d.ws_bytes = ws.data_ptr(), ws.numel() * ws.element_size()
     if _RECORDER is not None:

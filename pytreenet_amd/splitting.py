@@ -216,7 +216,8 @@ def cholesky_qr3_leg(t: torch.Tensor, leg: int):
     q = t
     Ls = []
     for p in range(3):
-        G = E.contract(q, la, q, lb, ["c", "c2"], conj_a=True)
+        # (the Cholesky reads the lower triangle only)
+        G = E.contract(q, la, q, lb, ["c", "c2"], conj_a=True, lower=True)
         shift = 11.0 * (m * n + n * (n + 1)) * _U if p == 0 else 0.0
         L, X = _potrf(G, shift, 0.0 if p == 0 else 1e-14, status[p:p + 1])
         Ls.append(L)

@@ -665,3 +665,27 @@ def test_gather_gemm_bf16x6_3m(gpu, case):
     want = base + (-0.5 + 0.25j) * ref
     err = np.abs(out.cpu().numpy() - want).max() / np.abs(want).max()
     assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("n,rows", [(256, 20000), (200, 3000), (40, 5000)])
+def test_gram_lower_tiles(gpu, n, rows):
+    """A Gram product with the lower-triangle hint: every element on and below the
+    diagonal equals the full product bit for bit; above it an element is either
+    the full product's or left as it was (tiles strictly above the diagonal may be
+    skipped, and are on the split-K tile kernel)."""
+    _say(f"gram lower tiles n={n} rows={rows}")
+    rng = np.random.default_rng(n + rows)
+    a = _dev(_crand(rng, (rows, n)))
+    full = E.contract(a, "kc", a, "kd", "cd", conj_a=True).cpu().numpy()
+    sentinel = 7.0 - 3.0j
+    out = torch.full((n, n), sentinel, dtype=torch.complex128, device="cuda")
+    E.contract(a, "kc", a, "kd", "cd", conj_a=True, out=out, lower=True)
+    got = out.cpu().numpy()
+    low = np.tril(np.ones((n, n), dtype=bool))
+    assert np.array_equal(got[low], full[low])
+    assert np.all((got == full) | (got == sentinel))
+    if n == 256:
+        # split-K on the 64 x 64 tile kernel: the tiles above the diagonal were skipped
+        assert np.all(got[:64, 64:] == sentinel)
+    ref = a.cpu().numpy().conj().T @ a.cpu().numpy()
+    np.testing.assert_allclose(got[low], ref[low], rtol=0, atol=1e-12 * np.abs(ref).max())
