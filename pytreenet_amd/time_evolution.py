@@ -524,7 +524,7 @@ class TDVPAlgorithm(TTNOBasedTimeEvolution):
                                          [e.to(torch.complex64) for e in envs])
             t32[0] = tensor
             # complex64 output: the last product then has complex64 operands
-            # and result, like the others (rocBLAS cgemm where it is strided)
+            # and result, like the others
             action = self._relaxed(action, E.CompiledNetwork(t32, l32, o32,
                                                              precision=self._low_precision(),
                                                              out_dtype=torch.complex64,
