@@ -85,6 +85,31 @@ __device__ __forceinline__ double2 cmul_conj_a(double2 a, double2 b) {
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+// streaming accesses of the long-vector sweeps: every basis vector and w is
+// read once per sweep and the written vector is not read again within it, so
+// the loads and stores are nontemporal (the 537 MB vectors of a large local
+// space do not fit any cache level anyway; measured 3-10% faster sweeps)
+#ifndef PTN_NT_SWEEP
+#define PTN_NT_SWEEP 1
+#endif
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 sweep_load(const double2 *p) {
+#if PTN_NT_SWEEP
+  const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p));
+  return make_double2(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void sweep_store(double2 *p, double2 x) {
+#if PTN_NT_SWEEP
+  d2v v = {x.x, x.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<d2v *>(p));
+#else
+  *p = x;
+#endif
+}
+
 // block-wide sum of `cnt` complex values per thread; thread v < cnt stores the
 // v-th sum to dst[v].  NT is the block size (a multiple of 64).
 template <int NT, int CNT>
@@ -130,11 +155,11 @@ __device__ __forceinline__ void dots_partial_body(int64_t n, const double2 *Q, i
   for (int v = 0; v <= KV; ++v) acc[v] = make_double2(0, 0);
   const int64_t stride = (int64_t)nblk * RT;
   for (int64_t i = (int64_t)blk * RT + threadIdx.x; i < n; i += stride) {
-    const double2 wi = w[i];
+    const double2 wi = sweep_load(w + i);
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
       if (v < cnt) {
-        const double2 q = Q[(int64_t)(v0 + v) * ldq + i];
+        const double2 q = sweep_load(Q + (int64_t)(v0 + v) * ldq + i);
         const double2 t = cmul_conj_a(q, wi);
         acc[v].x += t.x;
         acc[v].y += t.y;
@@ -365,11 +390,11 @@ __global__ __launch_bounds__(RT) void subtract_dots(int64_t n, const double2 *Q,
   for (int v = 0; v <= FUSE_V; ++v) acc[v] = make_double2(0, 0);
   const int64_t stride = (int64_t)gridDim.x * RT;
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
-    double2 wi = w[i];
+    double2 wi = sweep_load(w + i);
     double2 q[FUSE_V];
 #pragma unroll
     for (int v = 0; v < FUSE_V; ++v)
-      q[v] = v < nvec ? Q[(int64_t)v * ldq + i] : make_double2(0, 0);
+      q[v] = v < nvec ? sweep_load(Q + (int64_t)v * ldq + i) : make_double2(0, 0);
 #pragma unroll
     for (int v = 0; v < FUSE_V; ++v) {
       if (v < nvec) {
@@ -378,7 +403,7 @@ __global__ __launch_bounds__(RT) void subtract_dots(int64_t n, const double2 *Q,
         wi.y -= hv.x * q[v].y + hv.y * q[v].x;
       }
     }
-    w[i] = wi;
+    sweep_store(w + i, wi);
 #pragma unroll
     for (int v = 0; v < FUSE_V; ++v) {
       if (v < nvec) {
@@ -426,12 +451,12 @@ __global__ __launch_bounds__(RT) void subtract_scale(int64_t n, const double2 *Q
   const double c = inv->x;
   const int64_t stride = (int64_t)gridDim.x * RT;
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += stride) {
-    double2 wi = w[i];
+    double2 wi = sweep_load(w + i);
     // all of an element's basis values are requested before the first is used
     double2 q[FUSE_V];
 #pragma unroll
     for (int v = 0; v < FUSE_V; ++v)
-      q[v] = v < nvec ? Q[(int64_t)v * ldq + i] : make_double2(0, 0);
+      q[v] = v < nvec ? sweep_load(Q + (int64_t)v * ldq + i) : make_double2(0, 0);
 #pragma unroll
     for (int v = 0; v < FUSE_V; ++v) {
       if (v < nvec) {
@@ -440,7 +465,7 @@ __global__ __launch_bounds__(RT) void subtract_scale(int64_t n, const double2 *Q
         wi.y -= hv.x * q[v].y + hv.y * q[v].x;
       }
     }
-    qnext[i] = make_double2(c * wi.x, c * wi.y);
+    sweep_store(qnext + i, make_double2(c * wi.x, c * wi.y));
   }
 }
 
@@ -453,11 +478,11 @@ __device__ __forceinline__ void combine_body(int64_t n, const double2 *Q, int64_
   for (int64_t i = (int64_t)blk * RT + threadIdx.x; i < n; i += stride) {
     double2 s = make_double2(0, 0);
     for (int v = 0; v < nvec; ++v) {
-      const double2 q = Q[(int64_t)v * ldq + i];
+      const double2 q = sweep_load(Q + (int64_t)v * ldq + i);
       s.x += cs[v].x * q.x - cs[v].y * q.y;
       s.y += cs[v].x * q.y + cs[v].y * q.x;
     }
-    out[i] = s;
+    sweep_store(out + i, s);
   }
 }
 
